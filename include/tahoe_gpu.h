@@ -1,0 +1,156 @@
+/* tahoe_gpu.h -- C ABI of the MI355X (gfx950) continuum-solid Newton step.
+ *
+ * One handle (tg_ctx) holds one finite-element problem resident on one GPU:
+ * mesh, equation numbering, the upper-triangle MSR stiffness, the solver's
+ * block operator and all Newton/PCG vectors.  Every call on a handle is
+ * ordered on the handle's own HIP stream; calls on one handle are not
+ * thread-safe, different handles may be driven from different threads.
+ *
+ * Host arrays are plain C arrays; all sizes are element counts.
+ * Node-major arrays are [nnd][3]; equation-space arrays are [n_eq] in the
+ * reference's global equation order.  Connectivity is 0-based.
+ *
+ * Every function returns a status equal to the reference's ExceptionT::CodeT
+ * (toolbox/src/common/ExceptionT.h:17-30): 0 ok, 1 general fail,
+ * 3 out of memory, 4 out of range, 5 size mismatch, 6 bad input value,
+ * 7 bad Jacobian determinant.  tg_last_error() gives the message.
+ *
+ * The reference interface each entry point replaces is cited beside it
+ * (paths relative to the Tahoe source tree).
+ */
+#ifndef TAHOE_GPU_H
+#define TAHOE_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    TG_OK = 0,
+    TG_GENERAL_FAIL = 1,
+    TG_OUT_OF_MEMORY = 3,
+    TG_OUT_OF_RANGE = 4,
+    TG_SIZE_MISMATCH = 5,
+    TG_BAD_INPUT = 6,
+    TG_BAD_JACOBIAN = 7
+};
+
+/* element formulations: XML names total_lagrangian / updated_lagrangian_Q1P0
+ * (tahoe/src/elements/ElementListT.cpp:744-745) */
+enum { TG_TOTAL_LAGRANGIAN = 0, TG_Q1P0 = 1 };
+
+typedef struct tg_ctx tg_ctx;
+
+typedef struct {
+    int iterations;      /* PCG iterations taken */
+    double rel_residual; /* ||b - A x|| / ||b|| (recurrence) */
+    int converged;
+} tg_lin_stats;
+
+/* NLSolver parameters (tahoe/src/solvers/NLSolver.cpp:675-757) plus the
+ * Jacobi-PCG tolerance that replaces the direct factorisation. */
+typedef struct {
+    double rel_tolerance;        /* rel_tolerance */
+    double abs_tolerance;        /* abs_tolerance */
+    double divergence_tolerance; /* divergence_tolerance */
+    int max_iterations;          /* max_iterations */
+    double lin_rtol;             /* PCG ||r||/||b|| */
+    int lin_maxit;
+} tg_newton_opts;
+
+typedef struct {
+    int status;           /* 0 converged, 1 failed (diverged / iteration limit), 7 bad Jacobian */
+    int iterations;       /* Newton corrections */
+    double error0, error; /* ||R|| before the first and after the last correction */
+    long lin_iterations;  /* PCG iterations summed over corrections */
+} tg_newton_stats;
+
+/* per-kernel-class device time, summed over launches since tg_timing_reset */
+typedef struct {
+    double residual_ms;  long residual_launches;
+    double stiffness_ms; long stiffness_launches;
+    double expand_ms;    long expand_launches;
+    double spmv_ms;      long spmv_launches;
+    double vector_ms;    long vector_launches;
+} tg_timing;
+
+/* ---- handle ---------------------------------------------------------- */
+int tg_create(tg_ctx** ctx, int device);
+int tg_destroy(tg_ctx* ctx);
+const char* tg_last_error(const tg_ctx* ctx);
+
+/* ---- problem definition ---------------------------------------------- */
+/* Mesh and element-block materials.  Replaces reading the geometry and
+ * ContinuumElementT's element cards (ContinuumElementT.cpp:107). */
+int tg_set_mesh(tg_ctx* ctx, int nnd, int nel, const double* X, const int* conn, const int* mat);
+/* Simo_isotropic moduli for material id (IsotropicT.cpp:32-45 gives mu,kappa). */
+int tg_set_material(tg_ctx* ctx, int id, double mu, double kappa);
+/* TG_TOTAL_LAGRANGIAN or TG_Q1P0 */
+int tg_set_formulation(tg_ctx* ctx, int formulation);
+/* Kinematic BC cards: prescribed value = values[k] * t (KBC_CardT::Value,
+ * tahoe/src/primitives/KBC_CardT.cpp:53-64, linear schedule). */
+int tg_set_kbc(tg_ctx* ctx, int nkbc, const int* nodes, const int* dofs, const double* values);
+
+/* Equation numbering, MSR pattern, element colouring and device storage.
+ * Replaces FEManagerT::SetEquationSystem (FEManagerT.cpp:2248-2313) ->
+ * NodeManagerT::SetEquationNumbers (NodeManagerT.cpp:712-760) and
+ * GlobalMatrixT::Initialize -> MSRMatrixT::Initialize (MSRMatrixT.cpp:35-44). */
+int tg_initialize(tg_ctx* ctx, int* n_eq, long* msr_length);
+/* FieldT::Equations(): [nnd][3], 1-based, -1 prescribed (print_eqnos). */
+int tg_get_eqnos(const tg_ctx* ctx, int* eqnos);
+/* Aztec bindx of the upper MSR (MSRBuilderT::GenerateMSR, MSRBuilderT.cpp:134-181). */
+int tg_get_msr_bindx(const tg_ctx* ctx, int* bindx);
+
+/* ---- state ------------------------------------------------------------ */
+int tg_set_displacement(tg_ctx* ctx, const double* u);
+int tg_get_displacement(tg_ctx* ctx, double* u);
+/* Prescribed displacements at time t (nStaticIntegrator::ConsistentKBC,
+ * tahoe/src/integrators/static/nStaticIntegrator.cpp:18-46). */
+int tg_apply_kbc(tg_ctx* ctx, double t);
+/* Q1P0 converged-state history (SimoQ1P0::CloseStep/ResetStep, SimoQ1P0.cpp:42-61). */
+int tg_close_step(tg_ctx* ctx);
+int tg_reset_step(tg_ctx* ctx);
+
+/* ---- element group ---------------------------------------------------- */
+/* Residual R = -f_int of the current displacement (ElementBaseT::FormRHS,
+ * ElementBaseT.cpp:147 -> SolidElementT::ElementRHSDriver SolidElementT.cpp:1166,
+ * assembled as SolverT::AssembleRHS SolverT.cpp:446-477).  R_out [n_eq] and
+ * norm_out (SolverT::Residual, SolverT.cpp:848) may be NULL. */
+int tg_form_rhs(tg_ctx* ctx, double* R_out, double* norm_out);
+/* Stiffness into the device MSR (ElementBaseT::FormLHS ElementBaseT.cpp:114 ->
+ * SolidElementT::ElementLHSDriver SolidElementT.cpp:1100 -> MSRMatrixT::Assemble
+ * MSRMatrixT.cpp:66-165, after GlobalMatrixT::Clear). */
+int tg_form_lhs(tg_ctx* ctx);
+
+/* ---- global matrix ---------------------------------------------------- */
+/* MSR values val[msr_length] (the data GenerateRCV/PrintLHS dump, MSRMatrixT.cpp:449-524). */
+int tg_get_msr_values(tg_ctx* ctx, double* val);
+/* b = K x in equation space (GlobalMatrixT::Multx, MSRMatrixT.cpp:385-420). */
+int tg_multx(tg_ctx* ctx, const double* x, double* b);
+/* MSRMatrixT::CopyDiagonal (MSRMatrixT.cpp:351-378). */
+int tg_copy_diagonal(tg_ctx* ctx, double* diag);
+/* Solve K x = b by Jacobi-PCG; rhs holds b on entry and x on exit
+ * (GlobalMatrixT::Solve, GlobalMatrixT.cpp:77-114). */
+int tg_solve(tg_ctx* ctx, double* rhs, double rtol, int maxit, tg_lin_stats* stats);
+
+/* ---- solver ----------------------------------------------------------- */
+/* One load step's Newton loop, device resident: R, ||R||, K, PCG, u += du
+ * until ExitIteration's tests hold (NLSolver::Solve, NLSolver.cpp:57-264). */
+int tg_newton(tg_ctx* ctx, const tg_newton_opts* opts, tg_newton_stats* stats);
+/* One Newton correction without the convergence loop: R, K, PCG, u += du.
+ * stats->error0 gets ||R|| before the update (used by benchmarks). */
+int tg_newton_correction(tg_ctx* ctx, const tg_newton_opts* opts, tg_newton_stats* stats);
+
+/* ---- instrumentation -------------------------------------------------- */
+int tg_synchronize(tg_ctx* ctx);
+int tg_timing_enable(tg_ctx* ctx, int enable);
+int tg_timing_reset(tg_ctx* ctx);
+int tg_timing_get(tg_ctx* ctx, tg_timing* out);
+/* layout self-test of the FP64 MFMA tile the stiffness kernel relies on:
+ * returns 0 and max|error| in *err when D = A B matches on the device. */
+int tg_selftest_mfma(int device, double* err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TAHOE_GPU_H */

@@ -1,0 +1,299 @@
+"""ctypes binding of libtahoe_gpu.so (C ABI in include/tahoe_gpu.h).
+
+The library must have been built in-tree (``python -m tahoe_amd.build``);
+loading fails loudly otherwise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtahoe_gpu.so")
+
+TOTAL_LAGRANGIAN, Q1P0 = 0, 1
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+
+# exported symbols, in declaration order of include/tahoe_gpu.h
+SYMBOLS = [
+    "tg_create", "tg_destroy", "tg_last_error", "tg_set_mesh", "tg_set_material", "tg_set_formulation",
+    "tg_set_kbc", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
+    "tg_get_displacement", "tg_apply_kbc", "tg_close_step", "tg_reset_step", "tg_form_rhs", "tg_form_lhs",
+    "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_correction",
+    "tg_synchronize", "tg_timing_enable", "tg_timing_reset", "tg_timing_get", "tg_selftest_mfma",
+]
+
+
+class LinStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("rel_residual", C.c_double), ("converged", C.c_int)]
+
+
+class NewtonOpts(C.Structure):
+    _fields_ = [("rel_tolerance", C.c_double), ("abs_tolerance", C.c_double),
+                ("divergence_tolerance", C.c_double), ("max_iterations", C.c_int),
+                ("lin_rtol", C.c_double), ("lin_maxit", C.c_int)]
+
+
+class NewtonStats(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("error0", C.c_double), ("error", C.c_double),
+                ("lin_iterations", C.c_long)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("residual_ms", C.c_double), ("residual_launches", C.c_long),
+                ("stiffness_ms", C.c_double), ("stiffness_launches", C.c_long),
+                ("expand_ms", C.c_double), ("expand_launches", C.c_long),
+                ("spmv_ms", C.c_double), ("spmv_launches", C.c_long),
+                ("vector_ms", C.c_double), ("vector_launches", C.c_long)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TahoeGpuError(RuntimeError):
+    """Non-zero status from the library; ``code`` is the ExceptionT::CodeT value."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"status {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m tahoe_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.tg_create.argtypes = [C.POINTER(vp), C.c_int]
+        L.tg_destroy.argtypes = [vp]
+        L.tg_last_error.argtypes = [vp]
+        L.tg_last_error.restype = C.c_char_p
+        L.tg_set_mesh.argtypes = [vp, C.c_int, C.c_int, D, I, I]
+        L.tg_set_material.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+        L.tg_set_formulation.argtypes = [vp, C.c_int]
+        L.tg_set_kbc.argtypes = [vp, C.c_int, I, I, D]
+        L.tg_initialize.argtypes = [vp, I, C.POINTER(C.c_long)]
+        L.tg_get_eqnos.argtypes = [vp, I]
+        L.tg_get_msr_bindx.argtypes = [vp, I]
+        L.tg_set_displacement.argtypes = [vp, D]
+        L.tg_get_displacement.argtypes = [vp, D]
+        L.tg_apply_kbc.argtypes = [vp, C.c_double]
+        L.tg_close_step.argtypes = [vp]
+        L.tg_reset_step.argtypes = [vp]
+        L.tg_form_rhs.argtypes = [vp, D, D]
+        L.tg_form_lhs.argtypes = [vp]
+        L.tg_get_msr_values.argtypes = [vp, D]
+        L.tg_multx.argtypes = [vp, D, D]
+        L.tg_copy_diagonal.argtypes = [vp, D]
+        L.tg_solve.argtypes = [vp, D, C.c_double, C.c_int, C.POINTER(LinStats)]
+        L.tg_newton.argtypes = [vp, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)]
+        L.tg_newton_correction.argtypes = [vp, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)]
+        L.tg_synchronize.argtypes = [vp]
+        L.tg_timing_enable.argtypes = [vp, C.c_int]
+        L.tg_timing_reset.argtypes = [vp]
+        L.tg_timing_get.argtypes = [vp, C.POINTER(Timing)]
+        L.tg_selftest_mfma.argtypes = [C.c_int, D]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(D)
+
+
+def _i(a):
+    return a.ctypes.data_as(I)
+
+
+def selftest_mfma(device=0):
+    err = C.c_double()
+    rc = lib().tg_selftest_mfma(device, C.byref(err))
+    return rc, err.value
+
+
+class Solid:
+    """One finite-element problem resident on a GPU (a tg_ctx handle)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        rc = lib().tg_create(C.byref(self._h), device)
+        if rc:
+            raise TahoeGpuError(rc, "tg_create failed")
+        self.n_eq = None
+        self.msr_len = None
+
+    def close(self):
+        if self._h:
+            lib().tg_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ok(self, rc):
+        if rc:
+            raise TahoeGpuError(rc, lib().tg_last_error(self._h).decode())
+
+    # ---- definition ----
+    def set_mesh(self, X, conn, mat=None):
+        self._X = np.ascontiguousarray(X, dtype=np.float64)
+        self._conn = np.ascontiguousarray(conn, dtype=np.int32)
+        self.nnd, self.nel = self._X.shape[0], self._conn.shape[0]
+        m = None if mat is None else np.ascontiguousarray(mat, dtype=np.int32)
+        self._ok(lib().tg_set_mesh(self._h, self.nnd, self.nel, _d(self._X), _i(self._conn),
+                                   None if m is None else _i(m)))
+
+    def set_material(self, mid, mu, kappa):
+        self._ok(lib().tg_set_material(self._h, mid, float(mu), float(kappa)))
+
+    def set_formulation(self, f):
+        self._ok(lib().tg_set_formulation(self._h, f))
+
+    def set_kbc(self, nodes, dofs, values):
+        n = np.ascontiguousarray(nodes, dtype=np.int32)
+        d = np.ascontiguousarray(dofs, dtype=np.int32)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        self._ok(lib().tg_set_kbc(self._h, len(n), _i(n), _i(d), _d(v)))
+
+    def initialize(self):
+        n = C.c_int()
+        L = C.c_long()
+        self._ok(lib().tg_initialize(self._h, C.byref(n), C.byref(L)))
+        self.n_eq, self.msr_len = n.value, L.value
+        return self.n_eq, self.msr_len
+
+    def eqnos(self):
+        e = np.zeros((self.nnd, 3), dtype=np.int32)
+        self._ok(lib().tg_get_eqnos(self._h, _i(e)))
+        return e
+
+    def msr_bindx(self):
+        b = np.zeros(self.msr_len, dtype=np.int32)
+        self._ok(lib().tg_get_msr_bindx(self._h, _i(b)))
+        return b
+
+    # ---- state ----
+    def set_displacement(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._ok(lib().tg_set_displacement(self._h, _d(u)))
+
+    def displacement(self):
+        u = np.zeros((self.nnd, 3))
+        self._ok(lib().tg_get_displacement(self._h, _d(u)))
+        return u
+
+    def apply_kbc(self, t):
+        self._ok(lib().tg_apply_kbc(self._h, float(t)))
+
+    def close_step(self):
+        self._ok(lib().tg_close_step(self._h))
+
+    def reset_step(self):
+        self._ok(lib().tg_reset_step(self._h))
+
+    # ---- element group / matrix ----
+    def form_rhs(self, want=True):
+        R = np.zeros(self.n_eq) if want else None
+        nrm = C.c_double()
+        self._ok(lib().tg_form_rhs(self._h, _d(R) if want else None, C.byref(nrm)))
+        return R, nrm.value
+
+    def form_rhs_device(self):
+        self._ok(lib().tg_form_rhs(self._h, None, None))
+
+    def form_lhs(self):
+        self._ok(lib().tg_form_lhs(self._h))
+
+    def msr_values(self):
+        v = np.zeros(self.msr_len)
+        self._ok(lib().tg_get_msr_values(self._h, _d(v)))
+        return v
+
+    def multx(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        b = np.zeros(self.n_eq)
+        self._ok(lib().tg_multx(self._h, _d(x), _d(b)))
+        return b
+
+    def copy_diagonal(self):
+        d = np.zeros(self.n_eq)
+        self._ok(lib().tg_copy_diagonal(self._h, _d(d)))
+        return d
+
+    def solve(self, b, rtol=1e-12, maxit=100000):
+        x = np.array(b, dtype=np.float64, copy=True)
+        st = LinStats()
+        self._ok(lib().tg_solve(self._h, _d(x), rtol, maxit, C.byref(st)))
+        return x, st
+
+    # ---- solver ----
+    @staticmethod
+    def opts(rel_tol=1e-10, abs_tol=1e-14, div_tol=10.0, max_iterations=20, lin_rtol=1e-12, lin_maxit=200000):
+        return NewtonOpts(rel_tol, abs_tol, div_tol, max_iterations, lin_rtol, lin_maxit)
+
+    def newton(self, opts=None):
+        o = opts or self.opts()
+        st = NewtonStats()
+        rc = lib().tg_newton(self._h, C.byref(o), C.byref(st))
+        self._ok(rc)
+        return st
+
+    def newton_correction(self, opts=None):
+        o = opts or self.opts()
+        st = NewtonStats()
+        self._ok(lib().tg_newton_correction(self._h, C.byref(o), C.byref(st)))
+        return st
+
+    def synchronize(self):
+        self._ok(lib().tg_synchronize(self._h))
+
+    def timing_enable(self, on=True):
+        self._ok(lib().tg_timing_enable(self._h, 1 if on else 0))
+
+    def timing_reset(self):
+        self._ok(lib().tg_timing_reset(self._h))
+
+    def timing(self):
+        t = Timing()
+        self._ok(lib().tg_timing_get(self._h, C.byref(t)))
+        return t.as_dict()
+
+
+def static_solve(solid: Solid, num_steps: int, opts=None, on_step=None):
+    """FEManagerT::Solve's static time loop (tahoe/src/main/FEManagerT.cpp:138-269):
+    for t = s/num_steps apply the kinematic BCs and run Newton; returns the
+    converged displacement per step and the Newton iteration counts."""
+    out_u, iters = [], []
+    for s in range(1, num_steps + 1):
+        solid.apply_kbc(s / num_steps)
+        st = solid.newton(opts)
+        if st.status != 0:
+            raise TahoeGpuError(1, f"Newton failed at step {s}: status {st.status}")
+        solid.close_step()
+        iters.append(st.iterations)
+        out_u.append(solid.displacement())
+        if on_step:
+            on_step(s, st)
+    return np.array(out_u), iters
+
+
+def mu_kappa(E: float, nu: float):
+    """Shear and bulk moduli from Young's modulus and Poisson's ratio
+    (IsotropicT::Set_E_nu, tahoe/src/elements/continuum/solid/materials/
+    primitives/IsotropicT.cpp:32-45)."""
+    if E < 0.0 or nu > 0.5 or nu < -1.0:
+        raise TahoeGpuError(1, "bad E/nu")  # IsotropicT.cpp:38-39
+    mu = 0.5 * E / (1.0 + nu)
+    lam = 2.0 * mu * nu / (1.0 - 2.0 * nu)
+    return mu, lam + 2.0 / 3.0 * mu

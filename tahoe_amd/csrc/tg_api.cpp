@@ -1,0 +1,580 @@
+// C ABI of libtahoe_gpu.so (declared in include/tahoe_gpu.h).
+//
+// Each entry point converts internal errors into the reference's
+// ExceptionT::CodeT values.  Host<->device copies happen only where the
+// reference interface hands over host arrays; the Newton loop itself keeps
+// every vector on the device.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "tg_internal.h"
+
+namespace tg {
+
+void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return;
+    const int code = (e == hipErrorOutOfMemory) ? TG_OUT_OF_MEMORY : TG_GENERAL_FAIL;
+    fail(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- event timing -----------------------------------------------------------
+
+struct PendingEvent {
+    int which;
+    hipEvent_t e0, e1;
+};
+
+}  // namespace tg
+
+// per-context list of recorded event pairs (kept outside tg_ctx to keep the
+// struct layout independent of instrumentation)
+#include <unordered_map>
+#include <mutex>
+static std::mutex g_ev_mu;
+static std::unordered_map<const tg_ctx*, std::vector<tg::PendingEvent>> g_events;
+
+namespace tg {
+
+TimedRegion::TimedRegion(tg_ctx* ctx, int w) : c(ctx), which(w)
+{
+    if (!c->timing.enabled) return;
+    TG_HIP(hipEventCreate(&e0));
+    TG_HIP(hipEventCreate(&e1));
+    TG_HIP(hipEventRecord(e0, c->stream));
+}
+
+TimedRegion::~TimedRegion()
+{
+    if (!e0) return;
+    if (hipEventRecord(e1, c->stream) != hipSuccess) return;
+    std::lock_guard<std::mutex> lk(g_ev_mu);
+    g_events[c].push_back(PendingEvent{which, e0, e1});
+}
+
+static void drain_events(tg_ctx* c)
+{
+    std::vector<PendingEvent> ev;
+    {
+        std::lock_guard<std::mutex> lk(g_ev_mu);
+        auto it = g_events.find(c);
+        if (it == g_events.end()) return;
+        ev.swap(it->second);
+    }
+    TG_HIP(hipStreamSynchronize(c->stream));
+    for (auto& p : ev) {
+        float ms = 0.0f;
+        TG_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
+        tg_timing& a = c->timing.acc;
+        switch (p.which) {
+            case T_RESIDUAL: a.residual_ms += ms; a.residual_launches++; break;
+            case T_STIFFNESS: a.stiffness_ms += ms; a.stiffness_launches++; break;
+            case T_EXPAND: a.expand_ms += ms; a.expand_launches++; break;
+            case T_SPMV: a.spmv_ms += ms; a.spmv_launches++; break;
+            default: a.vector_ms += ms; a.vector_launches++; break;
+        }
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
+    }
+}
+
+static void set_device(tg_ctx* c) { TG_HIP(hipSetDevice(c->device)); }
+
+static void require_init(const tg_ctx* c)
+{
+    if (!c->initialized) fail(TG_GENERAL_FAIL, "tg_initialize has not been called");
+}
+
+static void check_status(tg_ctx* c)
+{
+    TG_HIP(hipMemcpyAsync(c->hstatus, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    if (c->hstatus[0] != 0) {
+        const int code = c->hstatus[0];
+        TG_HIP(hipMemsetAsync(c->dstatus.p, 0, sizeof(int), c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+        fail(code, "zero or negative jacobian");
+    }
+}
+
+}  // namespace tg
+
+using namespace tg;
+
+template <class F>
+static int guarded(tg_ctx* c, F f)
+{
+    try {
+        if (c) set_device(c);
+        f();
+        return TG_OK;
+    } catch (const Error& e) {
+        if (c) c->err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (c) c->err = "host allocation failed";
+        return TG_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        return TG_GENERAL_FAIL;
+    }
+}
+
+extern "C" {
+
+int tg_create(tg_ctx** out, int device)
+{
+    if (!out) return TG_BAD_INPUT;
+    *out = nullptr;
+    tg_ctx* c = new (std::nothrow) tg_ctx();
+    if (!c) return TG_OUT_OF_MEMORY;
+    c->device = device;
+    int rc = guarded(c, [&] {
+        int n = 0;
+        TG_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) fail(TG_OUT_OF_RANGE, "no such device");
+        TG_HIP(hipSetDevice(device));
+        TG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        TG_HIP(hipHostMalloc((void**)&c->hscal, sizeof(double) * 16, hipHostMallocDefault));
+        TG_HIP(hipHostMalloc((void**)&c->hstatus, sizeof(int) * 4, hipHostMallocDefault));
+        c->dstatus.alloc(4);
+        TG_HIP(hipMemset(c->dstatus.p, 0, sizeof(int) * 4));
+        c->dscal.alloc(16);
+        TG_HIP(hipMemset(c->dscal.p, 0, sizeof(double) * 16));
+    });
+    if (rc != TG_OK) {
+        tg_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return TG_OK;
+}
+
+int tg_destroy(tg_ctx* c)
+{
+    if (!c) return TG_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    {
+        std::lock_guard<std::mutex> lk(g_ev_mu);
+        auto it = g_events.find(c);
+        if (it != g_events.end()) {
+            for (auto& p : it->second) {
+                (void)hipEventDestroy(p.e0);
+                (void)hipEventDestroy(p.e1);
+            }
+            g_events.erase(it);
+        }
+    }
+    if (c->hscal) (void)hipHostFree(c->hscal);
+    if (c->hstatus) (void)hipHostFree(c->hstatus);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;  // DBuf destructors free device memory
+    return TG_OK;
+}
+
+const char* tg_last_error(const tg_ctx* c) { return c ? c->err.c_str() : "null handle"; }
+
+int tg_set_mesh(tg_ctx* c, int nnd, int nel, const double* X, const int* conn, const int* mat)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (nnd <= 0 || nel <= 0 || !X || !conn) fail(TG_BAD_INPUT, "empty mesh");
+        for (long k = 0; k < 8L * nel; k++)
+            if (conn[k] < 0 || conn[k] >= nnd) fail(TG_OUT_OF_RANGE, "connectivity refers to a missing node");
+        for (int e = 0; e < nel; e++)
+            for (int a = 0; a < 8; a++)
+                for (int b = a + 1; b < 8; b++)
+                    if (conn[8L * e + a] == conn[8L * e + b])
+                        fail(TG_BAD_INPUT, "degenerate element (repeated node) is not supported");
+        c->nnd = nnd;
+        c->nel = nel;
+        c->X.assign(X, X + 3L * nnd);
+        c->conn.assign(conn, conn + 8L * nel);
+        c->mat.assign((size_t)nel, 0);
+        if (mat)
+            for (int e = 0; e < nel; e++) {
+                if (mat[e] < 0) fail(TG_BAD_INPUT, "negative material id");
+                c->mat[e] = mat[e];
+            }
+        c->initialized = false;
+    });
+}
+
+int tg_set_material(tg_ctx* c, int id, double mu, double kappa)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (id < 0 || id > 1023) fail(TG_OUT_OF_RANGE, "material id out of range");
+        if (!(mu >= 0.0) || !(kappa >= 0.0)) fail(TG_GENERAL_FAIL, "negative modulus");  // IsotropicT.cpp:51
+        if ((int)c->mu.size() <= id) {
+            c->mu.resize(id + 1, -1.0);
+            c->kappa.resize(id + 1, -1.0);
+        }
+        c->mu[id] = mu;
+        c->kappa[id] = kappa;
+        c->initialized = false;
+    });
+}
+
+int tg_set_formulation(tg_ctx* c, int f)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (f != TG_TOTAL_LAGRANGIAN && f != TG_Q1P0) fail(TG_BAD_INPUT, "unknown formulation");
+        c->formulation = f;
+        c->initialized = false;
+    });
+}
+
+int tg_set_kbc(tg_ctx* c, int nkbc, const int* nodes, const int* dofs, const double* values)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (nkbc < 0 || (nkbc > 0 && (!nodes || !dofs || !values))) fail(TG_BAD_INPUT, "bad KBC arrays");
+        for (int k = 0; k < nkbc; k++) {
+            if (nodes[k] < 0 || nodes[k] >= c->nnd) fail(TG_OUT_OF_RANGE, "KBC node out of range");
+            if (dofs[k] < 0 || dofs[k] > 2) fail(TG_OUT_OF_RANGE, "KBC dof out of range");
+        }
+        c->kbc_nodes.assign(nodes, nodes + nkbc);
+        c->kbc_dofs.assign(dofs, dofs + nkbc);
+        c->kbc_vals.assign(values, values + nkbc);
+        c->initialized = false;
+    });
+}
+
+int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (c->nnd == 0) fail(TG_GENERAL_FAIL, "no mesh");
+        int nmat = 0;
+        for (int m : c->mat) nmat = std::max(nmat, m + 1);
+        if ((int)c->mu.size() < nmat) fail(TG_BAD_INPUT, "element refers to an undefined material");
+        for (int m = 0; m < nmat; m++)
+            if (c->mu[m] < 0.0) fail(TG_BAD_INPUT, "element refers to an undefined material");
+        c->initialized = false;
+        // equations, pattern, colours
+        c->n_eq = equation_numbers(c->nnd, c->kbc_nodes, c->kbc_dofs, c->eqnos);
+        msr_pattern(c->n_eq, c->nnd, c->nel, c->conn.data(), c->eqnos.data(), c->bindx);
+        c->msr_len = (long)c->bindx.size();
+        c->ncolors = color_elements(c->nnd, c->nel, c->conn.data(), c->color_ptr, c->color_elems);
+        // device storage
+        const size_t nn = 3 * (size_t)c->nnd;
+        c->dX.alloc(nn);
+        c->du.alloc(nn);
+        c->du_last.alloc(nn);
+        c->dR.alloc(nn);
+        c->dconn.alloc(8 * (size_t)c->nel);
+        c->dmat.alloc((size_t)c->nel);
+        c->deqn.alloc(nn);
+        c->dcolor_elems.alloc((size_t)c->nel);
+        c->dmatprops.alloc((size_t)std::max(1, nmat));
+        c->dbindx.alloc((size_t)c->msr_len);
+        c->dval.alloc((size_t)c->msr_len);
+        c->dslot.alloc(300 * (size_t)c->nel);
+        for (auto* v : {&c->vx, &c->vr, &c->vz, &c->vp, &c->vq, &c->vdinv, &c->vwork}) v->alloc(nn);
+        const size_t sblocks = ((size_t)c->nnd + 255) / 256;
+        c->dpart.alloc(2 * std::max<size_t>(sblocks, 1024) + 16);
+        std::vector<int> eqn0(nn);
+        for (size_t k = 0; k < nn; k++) eqn0[k] = c->eqnos[k] > 0 ? c->eqnos[k] - 1 : -1;
+        std::vector<double2> props((size_t)std::max(1, nmat));
+        for (int m = 0; m < nmat; m++) props[m] = make_double2(c->mu[m], c->kappa[m]);
+        std::vector<int> kidx(c->kbc_nodes.size());
+        for (size_t k = 0; k < kidx.size(); k++) kidx[k] = 3 * c->kbc_nodes[k] + c->kbc_dofs[k];
+        c->dkbc_idx.alloc(kidx.size());
+        c->dkbc_val.alloc(kidx.size());
+        TG_HIP(hipMemcpy(c->dX.p, c->X.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
+        TG_HIP(hipMemset(c->du.p, 0, sizeof(double) * nn));
+        TG_HIP(hipMemset(c->du_last.p, 0, sizeof(double) * nn));
+        TG_HIP(hipMemcpy(c->dconn.p, c->conn.data(), sizeof(int) * c->conn.size(), hipMemcpyHostToDevice));
+        TG_HIP(hipMemcpy(c->dmat.p, c->mat.data(), sizeof(int) * c->mat.size(), hipMemcpyHostToDevice));
+        TG_HIP(hipMemcpy(c->deqn.p, eqn0.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
+        TG_HIP(hipMemcpy(c->dcolor_elems.p, c->color_elems.data(), sizeof(int) * c->nel, hipMemcpyHostToDevice));
+        TG_HIP(hipMemcpy(c->dmatprops.p, props.data(), sizeof(double2) * props.size(), hipMemcpyHostToDevice));
+        TG_HIP(hipMemcpy(c->dbindx.p, c->bindx.data(), sizeof(int) * c->msr_len, hipMemcpyHostToDevice));
+        if (!kidx.empty()) {
+            TG_HIP(hipMemcpy(c->dkbc_idx.p, kidx.data(), sizeof(int) * kidx.size(), hipMemcpyHostToDevice));
+            TG_HIP(hipMemcpy(c->dkbc_val.p, c->kbc_vals.data(), sizeof(double) * kidx.size(), hipMemcpyHostToDevice));
+        }
+        launch_build_slots(c);
+        build_bsell(c);
+        TG_HIP(hipStreamSynchronize(c->stream));
+        c->initialized = true;
+        if (n_eq) *n_eq = c->n_eq;
+        if (msr_len) *msr_len = c->msr_len;
+    });
+}
+
+int tg_get_eqnos(const tg_ctx* c, int* eqnos)
+{
+    if (!c || !eqnos) return TG_BAD_INPUT;
+    if (!c->initialized) return TG_GENERAL_FAIL;
+    std::memcpy(eqnos, c->eqnos.data(), sizeof(int) * c->eqnos.size());
+    return TG_OK;
+}
+
+int tg_get_msr_bindx(const tg_ctx* c, int* bindx)
+{
+    if (!c || !bindx) return TG_BAD_INPUT;
+    if (!c->initialized) return TG_GENERAL_FAIL;
+    std::memcpy(bindx, c->bindx.data(), sizeof(int) * c->bindx.size());
+    return TG_OK;
+}
+
+int tg_set_displacement(tg_ctx* c, const double* u)
+{
+    if (!c || !u) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        TG_HIP(hipMemcpyAsync(c->du.p, u, sizeof(double) * 3 * (size_t)c->nnd, hipMemcpyHostToDevice, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int tg_get_displacement(tg_ctx* c, double* u)
+{
+    if (!c || !u) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        TG_HIP(hipMemcpyAsync(u, c->du.p, sizeof(double) * 3 * (size_t)c->nnd, hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int tg_apply_kbc(tg_ctx* c, double t)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        launch_apply_kbc(c, t);
+    });
+}
+
+// FieldT::CloseStep / ResetStep keep and restore the last converged
+// displacement (tahoe/src/nodes/FieldT.cpp:469-472, 618); SimoQ1P0 additionally keeps the
+// element volumes (SimoQ1P0.cpp:42-61), which the Simo material never reads
+// (it needs no "last" deformation gradient), so the displacement is the
+// whole state here.
+int tg_close_step(tg_ctx* c)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        TG_HIP(hipMemcpyAsync(c->du_last.p, c->du.p, sizeof(double) * 3 * (size_t)c->nnd, hipMemcpyDeviceToDevice,
+                              c->stream));
+    });
+}
+
+int tg_reset_step(tg_ctx* c)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        TG_HIP(hipMemcpyAsync(c->du.p, c->du_last.p, sizeof(double) * 3 * (size_t)c->nnd, hipMemcpyDeviceToDevice,
+                              c->stream));
+    });
+}
+
+int tg_form_rhs(tg_ctx* c, double* R_out, double* norm_out)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        launch_residual(c, c->du.p, c->dR.p);
+        if (norm_out || R_out) check_status(c);
+        if (norm_out) *norm_out = std::sqrt(device_dot(c, c->dR.p, c->dR.p, 3L * c->nnd));
+        if (R_out) {
+            node_to_eq(c, c->dR.p, c->vwork.p);
+            TG_HIP(hipMemcpyAsync(R_out, c->vwork.p, sizeof(double) * c->n_eq, hipMemcpyDeviceToHost, c->stream));
+            TG_HIP(hipStreamSynchronize(c->stream));
+        }
+    });
+}
+
+int tg_form_lhs(tg_ctx* c)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        launch_stiffness(c, c->du.p);
+    });
+}
+
+int tg_get_msr_values(tg_ctx* c, double* val)
+{
+    if (!c || !val) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        check_status(c);
+        TG_HIP(hipMemcpyAsync(val, c->dval.p, sizeof(double) * c->msr_len, hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int tg_multx(tg_ctx* c, const double* x, double* b)
+{
+    if (!c || !x || !b) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        check_status(c);
+        TG_HIP(hipMemcpyAsync(c->vwork.p, x, sizeof(double) * c->n_eq, hipMemcpyHostToDevice, c->stream));
+        eq_to_node(c, c->vwork.p, c->vx.p);
+        launch_expand(c);
+        launch_spmv(c, c->vx.p, c->vq.p, nullptr, nullptr);
+        node_to_eq(c, c->vq.p, c->vwork.p);
+        TG_HIP(hipMemcpyAsync(b, c->vwork.p, sizeof(double) * c->n_eq, hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int tg_copy_diagonal(tg_ctx* c, double* diag)
+{
+    if (!c || !diag) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        TG_HIP(hipMemcpyAsync(diag, c->dval.p, sizeof(double) * c->n_eq, hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int tg_solve(tg_ctx* c, double* rhs, double rtol, int maxit, tg_lin_stats* st)
+{
+    if (!c || !rhs) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        check_status(c);
+        TG_HIP(hipMemcpyAsync(c->vwork.p, rhs, sizeof(double) * c->n_eq, hipMemcpyHostToDevice, c->stream));
+        eq_to_node(c, c->vwork.p, c->vz.p);
+        launch_expand(c);
+        pcg_solve(c, c->vz.p, c->vx.p, rtol, maxit, st);
+        node_to_eq(c, c->vx.p, c->vwork.p);
+        TG_HIP(hipMemcpyAsync(rhs, c->vwork.p, sizeof(double) * c->n_eq, hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+// One correction: R(u) -> ||R||, K(u), PCG, u += du.  Returns ||R|| before
+// the update in *err_before; leaves R(u) of the *old* u in dR.
+static void correction(tg_ctx* c, const tg_newton_opts* o, double* err_before, long* lin_its)
+{
+    launch_residual(c, c->du.p, c->dR.p);
+    check_status(c);
+    *err_before = std::sqrt(device_dot(c, c->dR.p, c->dR.p, 3L * c->nnd));
+    launch_stiffness(c, c->du.p);
+    launch_expand(c);
+    tg_lin_stats st{};
+    pcg_solve(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
+    check_status(c);
+    *lin_its += st.iterations;
+    launch_axpy(c, c->du.p, c->vx.p, 1.0, 3L * c->nnd);
+}
+
+int tg_newton_correction(tg_ctx* c, const tg_newton_opts* o, tg_newton_stats* s)
+{
+    if (!c || !o) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        double e0 = 0.0;
+        long lin = 0;
+        correction(c, o, &e0, &lin);
+        TG_HIP(hipStreamSynchronize(c->stream));
+        if (s) {
+            s->status = 0;
+            s->iterations = 1;
+            s->error0 = e0;
+            s->error = e0;
+            s->lin_iterations = lin;
+        }
+    });
+}
+
+// NLSolver::Solve (NLSolver.cpp:57-264) with ExitIteration (NLSolver.cpp:675-757):
+// error0 = ||R(u0)||; converged at once when error0 < abs_tolerance; after
+// each correction fail when error/error0 > divergence_tolerance, converge when
+// error/error0 < rel_tolerance or error < abs_tolerance, fail when the
+// correction count reaches max_iterations.
+int tg_newton(tg_ctx* c, const tg_newton_opts* o, tg_newton_stats* s)
+{
+    if (!c || !o) return TG_BAD_INPUT;
+    tg_newton_stats out{};
+    int rc = guarded(c, [&] {
+        require_init(c);
+        launch_residual(c, c->du.p, c->dR.p);
+        check_status(c);
+        const double err0 = std::sqrt(device_dot(c, c->dR.p, c->dR.p, 3L * c->nnd));
+        out.error0 = out.error = err0;
+        if (err0 < o->abs_tolerance) return;
+        int iter = -1;
+        long lin = 0;
+        for (;;) {
+            launch_stiffness(c, c->du.p);
+            launch_expand(c);
+            tg_lin_stats st{};
+            pcg_solve(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
+            check_status(c);
+            lin += st.iterations;
+            launch_axpy(c, c->du.p, c->vx.p, 1.0, 3L * c->nnd);
+            iter++;
+            launch_residual(c, c->du.p, c->dR.p);
+            check_status(c);
+            const double err = std::sqrt(device_dot(c, c->dR.p, c->dR.p, 3L * c->nnd));
+            out.error = err;
+            out.iterations = iter + 1;
+            out.lin_iterations = lin;
+            const double rel = err / err0;
+            if (rel > o->divergence_tolerance) {
+                out.status = 1;
+                return;
+            }
+            if (rel < o->rel_tolerance || err < o->abs_tolerance) return;
+            if (iter >= o->max_iterations) {
+                out.status = 1;
+                return;
+            }
+        }
+    });
+    if (rc == TG_BAD_JACOBIAN) out.status = TG_BAD_JACOBIAN;
+    if (s) *s = out;
+    return rc;
+}
+
+int tg_synchronize(tg_ctx* c)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] { TG_HIP(hipStreamSynchronize(c->stream)); });
+}
+
+int tg_timing_enable(tg_ctx* c, int enable)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        drain_events(c);
+        c->timing.enabled = enable != 0;
+    });
+}
+
+int tg_timing_reset(tg_ctx* c)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        drain_events(c);
+        c->timing.acc = tg_timing{};
+    });
+}
+
+int tg_timing_get(tg_ctx* c, tg_timing* out)
+{
+    if (!c || !out) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        drain_events(c);
+        *out = c->timing.acc;
+    });
+}
+
+}  // extern "C"

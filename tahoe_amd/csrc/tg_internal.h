@@ -1,0 +1,151 @@
+// Internal declarations shared by the host setup code, the C ABI and the
+// kernels of libtahoe_gpu.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/tahoe_gpu.h"
+
+namespace tg {
+
+// Status raised inside the library; converted to the C status at the ABI.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+
+#define TG_HIP(call) ::tg::hip_check((call), #call)
+
+// Device buffer owned by a context.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count)
+    {
+        release();
+        if (count == 0) return;
+        TG_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { release(); }
+};
+
+// Sliced-ELL layout of the full (both triangles) 3x3-block operator used by
+// the PCG SpMV.  Block rows = nodes; slice s covers block rows
+// [64 s, 64 s + 64); within a slice, block k of lane l lives at
+// (slice_base[s] + k) * 64 + l; its 9 values are stored component-major
+// over the slice: val[((slice_base[s] + k) * 9 + c) * 64 + l].
+struct Bsell {
+    int nslices = 0;
+    long nblocks_padded = 0;               // sum over slices of width * 64
+    std::vector<int> slice_base, slice_width;
+    DBuf<int> d_slice_base, d_slice_width;
+    DBuf<int> d_col;                       // [nblocks_padded] column node, -1 = padding
+    DBuf<int> d_src;                       // [nblocks_padded * 9] MSR val index, -1 = zero
+    DBuf<double> d_val;                    // [nblocks_padded * 9]
+};
+
+struct Timing {
+    bool enabled = false;
+    tg_timing acc{};
+};
+
+}  // namespace tg
+
+struct tg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // problem definition (host)
+    int nnd = 0, nel = 0;
+    std::vector<double> X;
+    std::vector<int> conn, mat;
+    std::vector<double> mu, kappa;
+    int formulation = TG_TOTAL_LAGRANGIAN;
+    std::vector<int> kbc_nodes, kbc_dofs;
+    std::vector<double> kbc_vals;
+
+    // equation system
+    bool initialized = false;
+    int n_eq = 0;
+    std::vector<int> eqnos;   // [nnd*3] 1-based, -1 prescribed
+    std::vector<int> bindx;   // upper MSR pattern
+    long msr_len = 0;
+    std::vector<int> color_ptr, color_elems;
+    int ncolors = 0;
+
+    // device storage
+    tg::DBuf<double> dX, du, dR;        // node space [nnd*3]
+    tg::DBuf<double> du_last;           // last converged displacement (CloseStep/ResetStep)
+    tg::DBuf<int> dconn, dmat, deqn;    // deqn: 0-based equation or -1
+    tg::DBuf<int> dcolor_elems;
+    tg::DBuf<double2> dmatprops;        // (mu, kappa) per material
+    tg::DBuf<int> dkbc_idx;             // 3*node+dof per card
+    tg::DBuf<double> dkbc_val;
+    tg::DBuf<int> dbindx;
+    tg::DBuf<double> dval;
+    tg::DBuf<int> dslot;                // [nel*300] element upper entry -> val index (-1 none)
+    tg::DBuf<int> dperm;                // [n_eq] equation -> node-space index
+    tg::Bsell op;
+
+    // PCG work vectors in node space
+    tg::DBuf<double> vx, vr, vz, vp, vq, vdinv, vwork;
+    tg::DBuf<double> dpart;             // partial reduction sums
+    tg::DBuf<double> dscal;             // device scalars
+    tg::DBuf<int> dstatus;              // device error flag (7 = bad Jacobian)
+    double* hscal = nullptr;            // pinned host mirror of a few scalars
+    int* hstatus = nullptr;
+
+    // Q1P0 per-element history
+    tg::DBuf<double> dvol, dvol_last;
+
+    tg::Timing timing;
+};
+
+namespace tg {
+
+// host setup (tg_setup.cpp)
+int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
+void msr_pattern(int n_eq, int nnd, int nel, const int* conn, const int* eqnos, std::vector<int>& bindx);
+int color_elements(int nnd, int nel, const int* conn, std::vector<int>& ptr, std::vector<int>& elems);
+void build_bsell(tg_ctx* c);
+
+// kernels launched from the ABI (k_element.hip, k_solver.hip)
+void launch_residual(tg_ctx* c, const double* u, double* R_node);
+void launch_stiffness(tg_ctx* c, const double* u);
+void launch_build_slots(tg_ctx* c);
+void launch_expand(tg_ctx* c);
+void launch_spmv(tg_ctx* c, const double* x, double* y, double* dot_out_partial, const double* dot_with);
+double device_dot(tg_ctx* c, const double* a, const double* b, long n);
+void pcg_solve(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
+void eq_to_node(tg_ctx* c, const double* src_eq, double* dst_node);
+void node_to_eq(tg_ctx* c, const double* src_node, double* dst_eq);
+void launch_axpy(tg_ctx* c, double* y, const double* x, double a, long n);
+void launch_apply_kbc(tg_ctx* c, double t);
+
+// event-timed launch region
+struct TimedRegion {
+    tg_ctx* c;
+    int which;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TimedRegion(tg_ctx* ctx, int w);
+    ~TimedRegion();
+};
+enum { T_RESIDUAL = 0, T_STIFFNESS, T_EXPAND, T_SPMV, T_VECTOR };
+
+}  // namespace tg

@@ -1,0 +1,229 @@
+// Host-side setup of the equation system: equation numbers, the upper MSR
+// pattern, element colouring and the solver's sliced-ELL block structure.
+// Runs once per mesh (the reference does the same work in
+// FEManagerT::SetEquationSystem, tahoe/src/main/FEManagerT.cpp:2248-2313).
+#include <algorithm>
+#include <cstring>
+#include <thread>
+
+#include "tg_internal.h"
+
+namespace tg {
+
+static int worker_count()
+{
+    unsigned n = std::thread::hardware_concurrency();
+    if (n == 0) n = 4;
+    return (int)std::min(n, 16u);
+}
+
+template <class F>
+static void parallel_for(long n, F f)
+{
+    int nt = worker_count();
+    if (n < 4096 || nt == 1) {
+        f(0L, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    long chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+        long b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([=] { f(b, e); });
+    }
+    for (auto& x : th) x.join();
+}
+
+// NodeManagerT::SetEquationNumbers (tahoe/src/nodes/NodeManagerT.cpp:712-760)
+// -> FieldT::InitEquations/SetEquations (tahoe/src/nodes/FieldT.cpp:635-690):
+// dofs with a kinematic card get -1, the others 1.. in node-major, dof-minor
+// order.
+int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos)
+{
+    eqnos.assign((size_t)nnd * 3, 0);
+    for (size_t k = 0; k < kn.size(); k++) eqnos[(size_t)kn[k] * 3 + kd[k]] = -1;
+    int n = 0;
+    for (auto& e : eqnos)
+        if (e == 0) e = ++n;
+    return n;
+}
+
+// node -> incident elements (CSR)
+static void node_elements(int nnd, int nel, const int* conn, std::vector<int>& ptr, std::vector<int>& list)
+{
+    ptr.assign((size_t)nnd + 1, 0);
+    for (long k = 0; k < (long)nel * 8; k++) ptr[conn[k] + 1]++;
+    for (int a = 0; a < nnd; a++) ptr[a + 1] += ptr[a];
+    list.resize((size_t)ptr[nnd]);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < nel; e++)
+        for (int a = 0; a < 8; a++) list[(size_t)fill[conn[8 * e + a]]++] = e;
+}
+
+// sorted unique neighbour nodes (including the node itself)
+static int node_neighbours(int a, const std::vector<int>& ptr, const std::vector<int>& list, const int* conn,
+                           int* out)
+{
+    int n = 0;
+    for (int p = ptr[a]; p < ptr[a + 1]; p++) {
+        const int* c = conn + 8L * list[p];
+        for (int b = 0; b < 8; b++) out[n++] = c[b];
+    }
+    std::sort(out, out + n);
+    return (int)(std::unique(out, out + n) - out);
+}
+
+// Upper-triangle MSR pattern of GraphT::MakeGraph(active, add_self, upper_only)
+// (toolbox/src/graph/GraphT.cpp:376-456) packed by MSRBuilderT::GenerateMSR
+// (toolbox/src/graph/MSRBuilderT.cpp:134-181): row i lists every active
+// equation j > i sharing an element with i, ascending; bindx[0] = n_eq + 1.
+void msr_pattern(int n_eq, int nnd, int nel, const int* conn, const int* eqnos, std::vector<int>& bindx)
+{
+    std::vector<int> ptr, list;
+    node_elements(nnd, nel, conn, ptr, list);
+    std::vector<int> rowlen((size_t)n_eq, 0);
+    auto columns = [&](int a, int row, int* nb, int nnb, int* cols) {
+        int m = 0;
+        for (int k = 0; k < nnb; k++)
+            for (int j = 0; j < 3; j++) {
+                int col = eqnos[3L * nb[k] + j];
+                if (col > row) cols[m++] = col - 1;
+            }
+        // equation numbers need not follow node order in general
+        std::sort(cols, cols + m);
+        (void)a;
+        return m;
+    };
+    const int MAXNB = 8 * 64;
+    parallel_for(nnd, [&](long b, long e) {
+        std::vector<int> nb(MAXNB), cols(3 * MAXNB);
+        for (long a = b; a < e; a++) {
+            int nnb = node_neighbours((int)a, ptr, list, conn, nb.data());
+            for (int i = 0; i < 3; i++) {
+                int row = eqnos[3 * a + i];
+                if (row > 0) rowlen[row - 1] = columns((int)a, row, nb.data(), nnb, cols.data());
+            }
+        }
+    });
+    long total = 0;
+    for (int r = 0; r < n_eq; r++) total += rowlen[r];
+    if ((long)n_eq + 1 + total > 0x7fffffffL) fail(TG_OUT_OF_RANGE, "MSR pattern exceeds 32-bit indexing");
+    bindx.assign((size_t)(n_eq + 1 + total), 0);
+    bindx[0] = n_eq + 1;
+    for (int r = 0; r < n_eq; r++) bindx[r + 1] = bindx[r] + rowlen[r];
+    parallel_for(nnd, [&](long b, long e) {
+        std::vector<int> nb(MAXNB), cols(3 * MAXNB);
+        for (long a = b; a < e; a++) {
+            int nnb = node_neighbours((int)a, ptr, list, conn, nb.data());
+            for (int i = 0; i < 3; i++) {
+                int row = eqnos[3 * a + i];
+                if (row <= 0) continue;
+                int m = columns((int)a, row, nb.data(), nnb, cols.data());
+                std::memcpy(&bindx[(size_t)bindx[row - 1]], cols.data(), sizeof(int) * (size_t)m);
+            }
+        }
+    });
+}
+
+// First-fit greedy colouring: no two elements of one colour share a node, so
+// a colour's scatter into node-space vectors and MSR values needs no atomics.
+// Elements keep their mesh order inside a colour.
+int color_elements(int nnd, int nel, const int* conn, std::vector<int>& cptr, std::vector<int>& elems)
+{
+    std::vector<uint64_t> used((size_t)nnd, 0);
+    std::vector<int> color((size_t)nel);
+    int ncol = 0;
+    for (int e = 0; e < nel; e++) {
+        uint64_t forbid = 0;
+        for (int a = 0; a < 8; a++) forbid |= used[conn[8 * e + a]];
+        if (~forbid == 0) fail(TG_GENERAL_FAIL, "element colouring needs more than 64 colours");
+        int c = __builtin_ctzll(~forbid);
+        color[e] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int a = 0; a < 8; a++) used[conn[8 * e + a]] |= (1ull << c);
+    }
+    cptr.assign((size_t)ncol + 1, 0);
+    for (int e = 0; e < nel; e++) cptr[color[e] + 1]++;
+    for (int c = 0; c < ncol; c++) cptr[c + 1] += cptr[c];
+    elems.resize((size_t)nel);
+    std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+    for (int e = 0; e < nel; e++) elems[(size_t)fill[color[e]]++] = e;
+    return ncol;
+}
+
+static long msr_find(const std::vector<int>& bindx, int row, int col)
+{
+    if (row == col) return row;
+    const int* b = bindx.data();
+    const int* lo = b + b[row];
+    const int* hi = b + b[row + 1];
+    const int* p = std::lower_bound(lo, hi, col);
+    if (p == hi || *p != col) return -1;
+    return (long)(p - b);
+}
+
+// Solver operator structure: block row = node, block columns = neighbour
+// nodes; 64 block rows per slice, each slice padded to its widest row.
+// d_src maps each stored value to the MSR value it mirrors (upper entry
+// (min(r,c), max(r,c))), -1 for inactive dofs and padding.
+void build_bsell(tg_ctx* c)
+{
+    Bsell& B = c->op;
+    const int nnd = c->nnd;
+    std::vector<int> ptr, list;
+    node_elements(nnd, c->nel, c->conn.data(), ptr, list);
+    std::vector<int> deg((size_t)nnd);
+    const int MAXNB = 8 * 64;
+    parallel_for(nnd, [&](long b, long e) {
+        std::vector<int> nb(MAXNB);
+        for (long a = b; a < e; a++) deg[a] = node_neighbours((int)a, ptr, list, c->conn.data(), nb.data());
+    });
+    B.nslices = (nnd + 63) / 64;
+    B.slice_base.assign((size_t)B.nslices, 0);
+    B.slice_width.assign((size_t)B.nslices, 0);
+    long base = 0;
+    for (int s = 0; s < B.nslices; s++) {
+        int w = 0;
+        for (int l = 0; l < 64 && 64 * s + l < nnd; l++) w = std::max(w, deg[64 * s + l]);
+        B.slice_base[s] = (int)base;
+        B.slice_width[s] = w;
+        base += w;
+    }
+    B.nblocks_padded = base * 64;
+    if (B.nblocks_padded * 9 > 0x7fffffffL) fail(TG_OUT_OF_RANGE, "block operator exceeds 32-bit indexing");
+    std::vector<int> col((size_t)B.nblocks_padded, -1);
+    std::vector<int> src((size_t)B.nblocks_padded * 9, -1);
+    const int* eq = c->eqnos.data();
+    parallel_for(B.nslices, [&](long sb, long se) {
+        std::vector<int> nb(MAXNB);
+        for (long s = sb; s < se; s++)
+            for (int l = 0; l < 64; l++) {
+                long a = 64 * s + l;
+                if (a >= nnd) break;
+                int nnb = node_neighbours((int)a, ptr, list, c->conn.data(), nb.data());
+                for (int k = 0; k < nnb; k++) {
+                    long slot = ((long)B.slice_base[s] + k) * 64 + l;
+                    col[(size_t)slot] = nb[k];
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) {
+                            int r = eq[3 * a + i], cc = eq[3L * nb[k] + j];
+                            long v = -1;
+                            if (r > 0 && cc > 0) v = msr_find(c->bindx, std::min(r, cc) - 1, std::max(r, cc) - 1);
+                            src[(size_t)((((long)B.slice_base[s] + k) * 9 + 3 * i + j) * 64 + l)] = (int)v;
+                        }
+                }
+            }
+    });
+    B.d_slice_base.alloc(B.nslices);
+    B.d_slice_width.alloc(B.nslices);
+    B.d_col.alloc((size_t)B.nblocks_padded);
+    B.d_src.alloc((size_t)B.nblocks_padded * 9);
+    B.d_val.alloc((size_t)B.nblocks_padded * 9);
+    TG_HIP(hipMemcpy(B.d_slice_base.p, B.slice_base.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpy(B.d_slice_width.p, B.slice_width.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpy(B.d_col.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpy(B.d_src.p, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice));
+}
+
+}  // namespace tg

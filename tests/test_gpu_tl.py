@@ -1,0 +1,169 @@
+"""GPU parity of the total-Lagrangian path through the C ABI.
+
+Reference fixtures (tests/golden/*.npz, written by the reference CPU build):
+equation numbers and the MSR pattern must match bit-exactly; the residual
+and the assembled MSR stiffness at every dumped Newton iterate within 1e-12
+relative (to the largest entry of the iteration-0 vector/matrix); converged
+displacements within 1e-9 relative.  Larger meshes are checked against the
+C oracle and through size-independent properties.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden_K_dense, golden_R_eq, load_golden, msr_dense, msr_pattern_dense
+
+pytestmark = pytest.mark.gpu
+
+TOL_RK = 1e-12
+TOL_U = 1e-9
+
+
+def make_solid(g, formulation=None):
+    from tahoe_amd import api
+    s = api.Solid(0)
+    s.set_mesh(g["X"], g["conn"], g["mat"])
+    for m, (E, nu) in enumerate(zip(g["E"], g["nu"])):
+        s.set_material(m, *api.mu_kappa(E, nu))
+    s.set_formulation(formulation if formulation is not None else (1 if "Q1P0" in str(g["element"]) else 0))
+    s.set_kbc(g["kbc_nodes"], g["kbc_dofs"], g["kbc_values"])
+    s.initialize()
+    return s
+
+
+def test_mfma_layout_selftest():
+    from tahoe_amd import api
+    rc, err = api.selftest_mfma(0)
+    assert rc == 0 and err == 0.0
+
+
+TL_CASES = ["tl_n2", "tl_n4", "rve_n3", "rve_n4"]
+
+
+@pytest.mark.parametrize("name", TL_CASES)
+def test_eqnos_and_pattern(name):
+    from oracle import oracle as O
+    g = load_golden(name)
+    s = make_solid(g)
+    assert np.array_equal(s.eqnos(), g["eqnos"])
+    n = s.n_eq
+    b = s.msr_bindx()
+    ob = O.msr_pattern(n, g["conn"], g["eqnos"])
+    assert np.array_equal(b, ob)
+    P = msr_pattern_dense(b, n)
+    G = golden_K_dense(g, 0, n) != 0
+    assert not np.any(G & ~P)
+    if not name.startswith("rve"):
+        # perturbed meshes: every pattern entry is non-zero in the reference dump
+        assert np.array_equal(P, G)
+
+
+@pytest.mark.parametrize("name", TL_CASES)
+def test_residual_and_stiffness_at_reference_iterates(name):
+    g = load_golden(name)
+    s = make_solid(g)
+    n = s.n_eq
+    b = s.msr_bindx()
+    R0 = np.abs(golden_R_eq(g, 0)).max()
+    nsolve = len(g["lhs_ptr"]) - 1
+    K0 = np.abs(golden_K_dense(g, 0, n)).max()
+    for k in range(g["it_u"].shape[0]):
+        s.set_displacement(g["it_u"][k])
+        R, nrm = s.form_rhs()
+        Rg = golden_R_eq(g, k)
+        assert np.abs(R - Rg).max() <= TOL_RK * R0, (name, k)
+        assert abs(nrm - np.linalg.norm(R)) <= 1e-12 * max(nrm, 1e-300)
+        if k < nsolve:
+            s.form_lhs()
+            A = msr_dense(b, s.msr_values(), n)
+            Gk = golden_K_dense(g, k, n)
+            assert np.abs(A - Gk).max() <= TOL_RK * K0, (name, k)
+
+
+@pytest.mark.parametrize("name", ["tl_n2", "tl_n4", "tl_n8", "tl_n4_s20", "rve_n3", "rve_n4", "tl_n16"])
+def test_converged_displacements(name):
+    from tahoe_amd import api
+    g = load_golden(name)
+    s = make_solid(g)
+    step_u, iters = api.static_solve(s, int(g["num_steps"]))
+    for k in range(step_u.shape[0]):
+        ref = g["step_u"][k]
+        err = np.abs(step_u[k] - ref).max() / np.abs(ref).max()
+        assert err <= TOL_U, (name, k, err)
+    assert list(iters) == g["step_iters"].tolist()
+
+
+def test_multx_copy_diagonal_and_solve_match_oracle():
+    from oracle import oracle as O
+    g = load_golden("tl_n8")
+    s = make_solid(g)
+    n = s.n_eq
+    u = g["step_u"][0]
+    s.set_displacement(u)
+    s.form_lhs()
+    val = s.msr_values()
+    b = s.msr_bindx()
+    mu, ka = O.mu_kappa(1000.0, 0.3)
+    oval = O.stiffness(O.TL, g["X"], g["conn"], g["mat"], mu, ka, u, g["eqnos"], n, b)
+    assert np.abs(val - oval).max() <= TOL_RK * np.abs(oval).max()
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(n)
+    y = s.multx(x)
+    yo = O.msr_multx(b, oval, x)
+    assert np.abs(y - yo).max() <= 1e-12 * np.abs(yo).max()
+    assert np.array_equal(s.copy_diagonal(), val[:n])
+    rhs = rng.standard_normal(n)
+    xs, st = s.solve(rhs, rtol=1e-12)
+    assert st.converged
+    r = rhs - O.msr_multx(b, oval, xs)
+    assert np.linalg.norm(r) <= 1e-11 * np.linalg.norm(rhs)
+
+
+def test_bad_jacobian_is_reported():
+    """An inverted element gives status 7 (kBadJacobianDet), as
+    TotalLagrangianT::FormKd throws (TotalLagrangianT.cpp:127-130)."""
+    from tahoe_amd import api
+    g = load_golden("tl_n2")
+    s = make_solid(g)
+    u = np.zeros((s.nnd, 3))
+    u[13, 2] = -0.9  # push the centre node through the bottom layer
+    s.set_displacement(u)
+    with pytest.raises(api.TahoeGpuError) as e:
+        s.form_rhs()
+    assert e.value.code == 7
+    # the handle stays usable
+    s.set_displacement(np.zeros((s.nnd, 3)))
+    R, nrm = s.form_rhs()
+    assert nrm == 0.0
+
+
+def test_large_mesh_against_oracle_and_invariants():
+    """32^3 mesh: residual/stiffness against the oracle at a non-trivial state;
+    equilibrium of rigid translation (R independent of a uniform shift of u)
+    and symmetry of K through Multx."""
+    from oracle import oracle as O
+    from tahoe_amd import api, mesh as M
+    m = M.cube(32, seed=7)
+    kbc = M.kbc_uniaxial(m, 0.05)
+    s = api.Solid(0)
+    s.set_mesh(m.X, m.conn, m.mat)
+    mu, ka = api.mu_kappa(1000.0, 0.3)
+    s.set_material(0, mu, ka)
+    s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
+    n, L = s.initialize()
+    eq, n2 = O.equation_numbers(m.nnd, kbc.nodes, kbc.dofs)
+    assert n == n2 and np.array_equal(s.eqnos(), eq)
+    rng = np.random.default_rng(3)
+    u = 1e-3 * rng.standard_normal((m.nnd, 3))
+    s.set_displacement(u)
+    R, _ = s.form_rhs()
+    Ro = O.residual(O.TL, m.X, m.conn, m.mat, mu, ka, u, eq, n)
+    assert np.abs(R - Ro).max() <= 1e-12 * np.abs(Ro).max()
+    s.form_lhs()
+    val = s.msr_values()
+    b = s.msr_bindx()
+    assert np.array_equal(b, O.msr_pattern(n, m.conn, eq))
+    vo = O.stiffness(O.TL, m.X, m.conn, m.mat, mu, ka, u, eq, n, b)
+    assert np.abs(val - vo).max() <= 1e-12 * np.abs(vo).max()
+    x = rng.standard_normal(n)
+    y = rng.standard_normal(n)
+    assert abs(y @ s.multx(x) - x @ s.multx(y)) <= 1e-10 * np.linalg.norm(x) * np.linalg.norm(y) * np.abs(vo).max()
