@@ -39,6 +39,13 @@ enum {
  * (tahoe/src/elements/ElementListT.cpp:744-745) */
 enum { TG_TOTAL_LAGRANGIAN = 0, TG_Q1P0 = 1 };
 
+/* arithmetic of the stiffness and the inner linear solve:
+ * TG_FP64  everything in FP64;
+ * TG_MIXED element stiffness and Jacobi-PCG in FP32 (values, vectors and the
+ *          SpMV), dot products accumulated in FP64; residual, Newton test and
+ *          update stay FP64, so Newton acts as the iterative refinement. */
+enum { TG_FP64 = 0, TG_MIXED = 1 };
+
 typedef struct tg_ctx tg_ctx;
 
 typedef struct {
@@ -87,6 +94,9 @@ int tg_set_mesh(tg_ctx* ctx, int nnd, int nel, const double* X, const int* conn,
 int tg_set_material(tg_ctx* ctx, int id, double mu, double kappa);
 /* TG_TOTAL_LAGRANGIAN or TG_Q1P0 */
 int tg_set_formulation(tg_ctx* ctx, int formulation);
+/* TG_FP64 (default) or TG_MIXED; no counterpart in the reference, which is
+ * FP64 only.  TG_MIXED applies to the total-Lagrangian element. */
+int tg_set_precision(tg_ctx* ctx, int precision);
 /* Kinematic BC cards: prescribed value = values[k] * t (KBC_CardT::Value,
  * tahoe/src/primitives/KBC_CardT.cpp:53-64, linear schedule). */
 int tg_set_kbc(tg_ctx* ctx, int nkbc, const int* nodes, const int* dofs, const double* values);

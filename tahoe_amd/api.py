@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtahoe_gpu.so")
 
 TOTAL_LAGRANGIAN, Q1P0 = 0, 1
+FP64, MIXED = 0, 1
 
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
@@ -21,7 +22,7 @@ I = C.POINTER(C.c_int)
 # exported symbols, in declaration order of include/tahoe_gpu.h
 SYMBOLS = [
     "tg_create", "tg_destroy", "tg_last_error", "tg_set_mesh", "tg_set_material", "tg_set_formulation",
-    "tg_set_kbc", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
+    "tg_set_precision", "tg_set_kbc", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
     "tg_get_displacement", "tg_apply_kbc", "tg_close_step", "tg_reset_step", "tg_form_rhs", "tg_form_lhs",
     "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_correction",
     "tg_synchronize", "tg_timing_enable", "tg_timing_reset", "tg_timing_get", "tg_selftest_mfma",
@@ -79,6 +80,7 @@ def lib():
         L.tg_set_mesh.argtypes = [vp, C.c_int, C.c_int, D, I, I]
         L.tg_set_material.argtypes = [vp, C.c_int, C.c_double, C.c_double]
         L.tg_set_formulation.argtypes = [vp, C.c_int]
+        L.tg_set_precision.argtypes = [vp, C.c_int]
         L.tg_set_kbc.argtypes = [vp, C.c_int, I, I, D]
         L.tg_initialize.argtypes = [vp, I, C.POINTER(C.c_long)]
         L.tg_get_eqnos.argtypes = [vp, I]
@@ -159,6 +161,9 @@ class Solid:
 
     def set_formulation(self, f):
         self._ok(lib().tg_set_formulation(self._h, f))
+
+    def set_precision(self, p):
+        self._ok(lib().tg_set_precision(self._h, p))
 
     def set_kbc(self, nodes, dofs, values):
         n = np.ascontiguousarray(nodes, dtype=np.int32)

@@ -148,48 +148,70 @@ void launch_residual(tg_ctx* c, const double* u, double* R_node)
 // Position of (m, n), m <= n < 24, in the packed upper triangle.
 __device__ __forceinline__ int upper_index(int m, int n) { return m * 24 - (m * (m - 1)) / 2 + (n - m); }
 
-// Element stiffness (TotalLagrangianT::FormStiffness, TotalLagrangianT.cpp:40-104)
-//   K = sum_q B_q^T (s_q c_q) B_q  +  Expand(sum_q gradN^T (s_q sigma_q) gradN, 3)
-// with s_q = det0 w J, B from dN/dx.  The material part is one 24 x 48 x 24
-// product per element, Bs^T * DBs, run as 16x16x4 FP64 MFMA tiles (rows and
-// columns 0-15 / 16-31, the lower-left tile is skipped: only the upper
-// triangle is assembled, MSRMatrixT.cpp:110-165).  A wave handles 8 elements:
-// all 64 lanes first evaluate the 8x8 integration points, then the wave runs
-// the MFMA tiles element by element and scatters through the slot map.
-__global__ void __launch_bounds__(64) k_stiffness_tl(const int* __restrict__ elems, int count,
-                                                     const int* __restrict__ conn, const int* __restrict__ emat,
-                                                     const double2* __restrict__ props,
-                                                     const double* __restrict__ X, const double* __restrict__ U,
-                                                     const int* __restrict__ slot, double* __restrict__ val,
-                                                     int* __restrict__ status)
+// Element stiffness (TotalLagrangianT::FormStiffness, TotalLagrangianT.cpp:40-104),
+// one lane per upper 3x3 node block (a <= b) of an element.  With
+// D_q = s_q c_q and T_q = s_q sigma_q (s_q = det0 w J, Voigt 11,22,33,23,13,12):
+//   K(ai,bj) = sum_q sum_kl g_q[k][a] g_q[l][b] ( D_q[V(i,k)][V(j,l)] + delta_ij T_q[V(k,l)] )
+// i.e. B^T D B plus Expand(gradN^T T gradN, 3), accumulated over integration
+// points in order.  A block of 576 lanes owns 16 elements: 8 lanes per
+// element first evaluate the integration points into LDS, then 36 lanes per
+// element form one 3x3 block each (no divergence, 18 accumulators) and
+// scatter its upper entries through the slot map.  FP64 MFMA peaks at the
+// FP64 VALU rate on gfx950 while the dense 24x48x24 tile product does about
+// twice the arithmetic of this sparse form, so the FP64 path uses the VALU.
+constexpr int SK_EPB = 16;  // elements per block
+constexpr int SK_THREADS = 36 * SK_EPB;
+
+__device__ __forceinline__ constexpr int voigt(int i, int k) { return i == k ? i : 6 - i - k; }
+
+// packed upper-triangle index of a symmetric 6x6 (row-major, v <= w)
+__device__ __forceinline__ constexpr int sym6(int v, int w)
 {
-    __shared__ double sxu[8][8][6];
-    __shared__ double sg[8][8][3][8];   // dN/dx [element][ip][dir][node]
-    __shared__ double sd[8][8][36];     // s_q c_q [element][ip][v*6+w]
-    __shared__ double ss[8][8][6];      // s_q sigma_q
-    __shared__ double sgeo[8][64];      // geometric stiffness G[a*8+b]
-    const int t = threadIdx.x, le = t >> 3, q = t & 7;
-    const long base = (long)blockIdx.x * 8;
-    const bool valid = base + le < count;
-    int e = 0;
+    return (v < w ? v : w) * 6 - ((v < w ? v : w) * ((v < w ? v : w) - 1)) / 2 + ((v < w ? w : v) - (v < w ? v : w));
+}
+
+// VT is the arithmetic of the 3x3 block products and of the assembled
+// values: double (FP64 path) or float (mixed path, integration-point
+// kinematics and material stay FP64 and are rounded once to FP32).
+template <class VT>
+__global__ void __launch_bounds__(SK_THREADS) k_stiffness_tl(const int* __restrict__ elems, int count,
+                                                             const int* __restrict__ conn,
+                                                             const int* __restrict__ emat,
+                                                             const double2* __restrict__ props,
+                                                             const double* __restrict__ X,
+                                                             const double* __restrict__ U,
+                                                             const int* __restrict__ slot, VT* __restrict__ val,
+                                                             int* __restrict__ status)
+{
+    __shared__ double sxu[SK_EPB][8][6];
+    __shared__ VT sg[SK_EPB][8][3][8];  // dN/dx [element][ip][dir][node]
+    __shared__ VT sd[SK_EPB][8][21];    // s c, packed symmetric Voigt
+    __shared__ VT st[SK_EPB][8][6];     // s sigma, Voigt
+    const int t = threadIdx.x;
+    const long base = (long)blockIdx.x * SK_EPB;
     bool bad = false;
-    if (valid) {
-        e = elems[base + le];
-        const int node = conn[8L * e + q];
+    if (t < 8 * SK_EPB) {
+        const int le = t >> 3, q = t & 7;
+        if (base + le < count) {
+            const int node = conn[8L * elems[base + le] + q];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            sxu[le][q][c] = X[3L * node + c];
-            sxu[le][q][3 + c] = U[3L * node + c];
+            for (int c = 0; c < 3; c++) {
+                sxu[le][q][c] = X[3L * node + c];
+                sxu[le][q][3 + c] = U[3L * node + c];
+            }
         }
     }
     __syncthreads();
-    // ---- phase 1: integration point data ----
-    {
-        double d[3][8], j0[9], ji[9], g0[3][8], F[9], Fi[9];
+    // ---- phase 1: integration-point data ----
+    if (t < 8 * SK_EPB) {
+        const int le = t >> 3, q = t & 7;
+        const bool valid = base + le < count;
         double scale = 0.0, Jb = 1.0, bb[6] = {1, 1, 1, 0, 0, 0}, sig[6] = {0, 0, 0, 0, 0, 0};
         double2 mk = make_double2(0.0, 0.0);
+        double g[3][8];
         if (valid) {
-            mk = props[emat[e]];
+            mk = props[emat[elems[base + le]]];
+            double d[3][8], j0[9], ji[9], g0[3][8], F[9], Fi[9];
             parent_derivs(q, d);
             jacobian([&](int a, int c) { return sxu[le][a][c]; }, d, j0);
             const double det0 = det3(j0);
@@ -202,110 +224,91 @@ __global__ void __launch_bounds__(64) k_stiffness_tl(const int* __restrict__ ele
             bad |= !simo_kinematics(F, Jb, bb);
             simo_cauchy(mk.x, mk.y, Jb, bb, sig);
             inv3(F, J, Fi);
-            scale = det0 * J;  // weight 1
+            scale = bad ? 0.0 : det0 * J;  // weight 1
             // dN/dx_j = sum_k Fi(k,j) dN/dX_k (ShapeFunctionT::TransformDerivatives)
 #pragma unroll
             for (int a = 0; a < 8; a++)
 #pragma unroll
                 for (int j = 0; j < 3; j++)
-                    sg[le][q][j][a] = Fi[0 + 3 * j] * g0[0][a] + Fi[1 + 3 * j] * g0[1][a] + Fi[2 + 3 * j] * g0[2][a];
-            if (bad) scale = 0.0;
+                    g[j][a] = Fi[0 + 3 * j] * g0[0][a] + Fi[1 + 3 * j] * g0[1][a] + Fi[2 + 3 * j] * g0[2][a];
         } else {
 #pragma unroll
             for (int a = 0; a < 8; a++)
 #pragma unroll
-                for (int j = 0; j < 3; j++) sg[le][q][j][a] = 0.0;
+                for (int j = 0; j < 3; j++) g[j][a] = 0.0;
         }
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) sg[le][q][j][a] = (VT)g[j][a];
         double cm[6][6];
         simo_moduli(mk.x, mk.y, Jb, bb, cm);
 #pragma unroll
         for (int v = 0; v < 6; v++)
 #pragma unroll
-            for (int w = 0; w < 6; w++) sd[le][q][v * 6 + w] = scale * cm[v][w];
+            for (int w = v; w < 6; w++) sd[le][q][sym6(v, w)] = (VT)(scale * cm[v][w]);
 #pragma unroll
-        for (int k = 0; k < 6; k++) ss[le][q][k] = scale * sig[k];
+        for (int k = 0; k < 6; k++) st[le][q][k] = (VT)(scale * sig[k]);
     }
     __syncthreads();
-    // ---- geometric stiffness G_ab = sum_q gradN_a . (s sigma) gradN_b ----
-    for (int task = t; task < 8 * 64; task += 64) {
-        const int el = task >> 6, ab = task & 63, a = ab >> 3, b = ab & 7;
-        double s = 0.0;
-        for (int qq = 0; qq < 8; qq++) {
-            const double* sq = ss[el][qq];
-            const double ga0 = sg[el][qq][0][a], ga1 = sg[el][qq][1][a], ga2 = sg[el][qq][2][a];
-            const double gb0 = sg[el][qq][0][b], gb1 = sg[el][qq][1][b], gb2 = sg[el][qq][2][b];
-            // sigma Voigt (11,22,33,23,13,12)
-            s += ga0 * (sq[0] * gb0 + sq[5] * gb1 + sq[4] * gb2) + ga1 * (sq[5] * gb0 + sq[1] * gb1 + sq[3] * gb2) +
-                 ga2 * (sq[4] * gb0 + sq[3] * gb1 + sq[2] * gb2);
+    // ---- phase 2: one 3x3 block per lane ----
+    const int le = t / 36, p = t % 36;
+    if (base + le < count) {
+        // p -> (a, b), a <= b, row-major over the upper triangle of 8x8
+        int a = 0, rem = p;
+        while (rem >= 8 - a) {
+            rem -= 8 - a;
+            a++;
         }
-        sgeo[el][ab] = s;
-    }
-    __syncthreads();
-    // ---- phase 2: MFMA tiles per element ----
-    const int lr = t & 15, lk = t >> 4;  // operand row/col within tile, k within step
-    for (int el = 0; el < 8; el++) {
-        if (base + el >= count) break;
-        double4_t acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
-        // A operand rows m0 = lr (tile 0), m1 = 16 + lr (tile 1); B operand columns n likewise
-        const int m0 = lr, a0 = m0 / 3, i0 = m0 % 3;
-        const int m1 = 16 + lr, a1 = m1 / 3, i1 = m1 % 3;
-        const bool m1ok = m1 < 24;
-#pragma unroll 4
-        for (int ks = 0; ks < 12; ks++) {
-            const int kk = 4 * ks + lk, qq = kk / 6, v = kk % 6;
-            const double* D = sd[el][qq] + v * 6;
-            // A = Bs^T(m, kk) = B_q[v][m]
-            const int c0 = bcomp(v, i0);
-            const double A0 = c0 >= 0 ? sg[el][qq][c0][a0] : 0.0;
-            const int c1 = m1ok ? bcomp(v, i1) : -1;
-            const double A1 = c1 >= 0 ? sg[el][qq][c1][a1] : 0.0;
-            // B = DBs(kk, n) = sum_w D[v][w] B_q[w][n]
-            double B0, B1;
-            {
-                const int n = m0, b = a0, j = i0;
-                double s = 0.0;
+        const int b = a + rem;
+        VT K[3][3];
 #pragma unroll
-                for (int w = 0; w < 6; w++) {
-                    const int cw = bcomp(w, j);
-                    if (cw >= 0) s += D[w] * sg[el][qq][cw][b];
-                }
-                B0 = s;
-                (void)n;
-            }
-            {
-                const int b = a1, j = i1;
-                double s = 0.0;
-                if (m1ok) {
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-                    for (int w = 0; w < 6; w++) {
-                        const int cw = bcomp(w, j);
-                        if (cw >= 0) s += D[w] * sg[el][qq][cw][b];
-                    }
-                }
-                B1 = s;
+            for (int j = 0; j < 3; j++) K[i][j] = 0;
+        for (int q = 0; q < 8; q++) {
+            VT D[21], T[6], ga[3], gb[3];
+#pragma unroll
+            for (int k = 0; k < 21; k++) D[k] = sd[le][q][k];
+#pragma unroll
+            for (int k = 0; k < 6; k++) T[k] = st[le][q][k];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                ga[k] = sg[le][q][k][a];
+                gb[k] = sg[le][q][k][b];
             }
-            acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(A0, B0, acc00, 0, 0, 0);
-            acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(A0, B1, acc01, 0, 0, 0);
-            acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(A1, B1, acc11, 0, 0, 0);
+            VT h[3][3], geo = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int l = 0; l < 3; l++) {
+                    h[k][l] = ga[k] * gb[l];
+                    geo += h[k][l] * T[voigt(k, l)];
+                }
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    VT x = 0;
+#pragma unroll
+                    for (int k = 0; k < 3; k++)
+#pragma unroll
+                        for (int l = 0; l < 3; l++) x += h[k][l] * D[sym6(voigt(i, k), voigt(j, l))];
+                    K[i][j] += x;
+                }
+#pragma unroll
+            for (int i = 0; i < 3; i++) K[i][i] += geo;
         }
-        // ---- geometric part and scatter ----
-        const int ee = elems[base + el];
-        const int* es = slot + 300L * ee;
-        const double* G = sgeo[el];
+        const int* es = slot + 300L * elems[base + le];
 #pragma unroll
-        for (int tile = 0; tile < 3; tile++) {
-            const double4_t acc = tile == 0 ? acc00 : (tile == 1 ? acc01 : acc11);
-            const int row0 = tile == 2 ? 16 : 0, col0 = tile == 0 ? 0 : 16;
+        for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m = row0 + 4 * r + lk, n = col0 + lr;
-                if (m > n || n >= 24) continue;
-                double x = acc[r];
-                if (m % 3 == n % 3) x += G[(m / 3) * 8 + n / 3];
+            for (int j = 0; j < 3; j++) {
+                const int m = 3 * a + i, n = 3 * b + j;
+                if (n < m) continue;  // lower half of a diagonal block
                 const int s = es[upper_index(m, n)];
-                if (s >= 0) val[s] += x;
+                if (s >= 0) val[s] += K[i][j];
             }
-        }
     }
     if (bad) atomicOr(status, 7);
 }
@@ -313,7 +316,11 @@ __global__ void __launch_bounds__(64) k_stiffness_tl(const int* __restrict__ ele
 void launch_stiffness(tg_ctx* c, const double* u)
 {
     TimedRegion tr(c, T_STIFFNESS);
-    TG_HIP(hipMemsetAsync(c->dval.p, 0, sizeof(double) * (size_t)c->msr_len, c->stream));
+    const bool mixed = c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN;
+    if (mixed)
+        TG_HIP(hipMemsetAsync(c->dval32.p, 0, sizeof(float) * (size_t)c->msr_len, c->stream));
+    else
+        TG_HIP(hipMemsetAsync(c->dval.p, 0, sizeof(double) * (size_t)c->msr_len, c->stream));
     if (c->formulation == TG_Q1P0) {
         launch_q1p0(c, u, c->dR.p, true);
         return;
@@ -321,11 +328,15 @@ void launch_stiffness(tg_ctx* c, const double* u)
     for (int col = 0; col < c->ncolors; col++) {
         const int first = c->color_ptr[col], count = c->color_ptr[col + 1] - first;
         if (count == 0) continue;
-        const int blocks = (count + 7) / 8;
-        if (c->formulation == TG_TOTAL_LAGRANGIAN)
-            hipLaunchKernelGGL(k_stiffness_tl, dim3(blocks), dim3(64), 0, c->stream, c->dcolor_elems.p + first, count,
-                               c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->dslot.p, c->dval.p,
-                               c->dstatus.p);
+        const int blocks = (count + SK_EPB - 1) / SK_EPB;
+        if (mixed)
+            hipLaunchKernelGGL(k_stiffness_tl<float>, dim3(blocks), dim3(SK_THREADS), 0, c->stream,
+                               c->dcolor_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                               c->dslot.p, c->dval32.p, c->dstatus.p);
+        else
+            hipLaunchKernelGGL(k_stiffness_tl<double>, dim3(blocks), dim3(SK_THREADS), 0, c->stream,
+                               c->dcolor_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                               c->dslot.p, c->dval.p, c->dstatus.p);
     }
     TG_HIP(hipGetLastError());
 }

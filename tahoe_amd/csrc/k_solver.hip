@@ -3,11 +3,10 @@
 //
 // All vectors live in node space (3 entries per node, prescribed dofs held at
 // zero), so the SpMV needs no equation lookups.  Reductions are two-level:
-// every block writes a partial sum, the last block to finish adds the
-// partials in block order (deterministic, no floating-point atomics) and
-// updates the scalars of the iteration on the device.  The host only polls a
-// "done" flag every few iterations; kernels enqueued after convergence
-// return immediately.
+// every block writes a partial sum and a one-block finalize kernel adds the
+// partials in block order (deterministic, no atomics) and updates the
+// scalars of the iteration on the device.  The host only polls a "done" flag
+// every few iterations; kernels enqueued after convergence return at once.
 #include "tg_internal.h"
 
 namespace tg {
@@ -47,78 +46,71 @@ __device__ __forceinline__ void block_sum(double (&v)[NV])
     }
 }
 
-// Writes this block's partials; returns true in thread 0 of the last block,
-// whose tot[] then holds the ordered sum over all blocks.
+// Writes this block's NV partial sums to part[k * gridDim.x + blockIdx.x].
 template <int NV>
-__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* part, unsigned* counter, double (&tot)[NV])
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* part)
 {
-    __shared__ bool last;
     block_sum<NV>(v);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < NV; k++) part[(long)k * gridDim.x + blockIdx.x] = v[k];
-        __threadfence();
-        const unsigned ticket = atomicAdd(counter, 1u);
-        last = (ticket == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double s[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) s[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-#pragma unroll
-        for (int k = 0; k < NV; k++) s[k] += ((volatile double*)part)[(long)k * gridDim.x + b];
-    // ordered combine: per-thread strided sums are added in thread order
+}
+
+// Ordered sum of nparts partials per value, by one block of VB threads:
+// thread i sums partials i, i+VB, ... in order, then thread 0 adds the VB
+// thread sums in order.  Deterministic for a fixed grid.
+template <int NV>
+__device__ __forceinline__ void ordered_total(const double* part, int nparts, double (&tot)[NV])
+{
     __shared__ double red[NV][VB];
 #pragma unroll
-    for (int k = 0; k < NV; k++) red[k][threadIdx.x] = s[k];
+    for (int k = 0; k < NV; k++) {
+        double x = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += VB) x += part[(long)k * nparts + b];
+        red[k][threadIdx.x] = x;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             double x = 0.0;
-            for (int i = 0; i < (int)blockDim.x; i++) x += red[k][i];
+            for (int i = 0; i < VB; i++) x += red[k][i];
             tot[k] = x;
         }
-        *counter = 0;
-        return true;
-    }
-    return false;
 }
 
 // ---------------------------------------------------------------------------
 // operator
 // ---------------------------------------------------------------------------
 
-__global__ void k_expand(long n, const int* __restrict__ src, const double* __restrict__ val, double* __restrict__ out)
+template <class VT>
+__global__ void k_expand(long n, const int* __restrict__ src, const VT* __restrict__ val, VT* __restrict__ out)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const int s = src[i];
-        out[i] = s >= 0 ? val[s] : 0.0;
+        out[i] = s >= 0 ? val[s] : VT(0);
     }
 }
 
 // inverse diagonal in node space (Jacobi; DiagonalMatrixT::Factorize/BackSubstitute,
 // DiagonalMatrixT.cpp:267-324)
-__global__ void k_dinv(long n, const int* __restrict__ eqn, const double* __restrict__ val, double* __restrict__ dinv)
+template <class VT>
+__global__ void k_dinv(long n, const int* __restrict__ eqn, const VT* __restrict__ val, VT* __restrict__ dinv)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const int e = eqn[i];
-        const double d = e >= 0 ? val[e] : 0.0;
-        dinv[i] = d != 0.0 ? 1.0 / d : 0.0;
+        const VT d = e >= 0 ? val[e] : VT(0);
+        dinv[i] = d != VT(0) ? VT(1) / d : VT(0);
     }
 }
 
-// y = A x for block rows of one thread each; optionally partial dot(y, w).
-// The last block then sets alpha = rz / (p.q) when `pcg` is set.
-template <bool PCG>
+// y = A x for block rows of one thread each; with PCG also the block
+// partials of p.q (x = p, y = q) for k_fin_alpha.
+template <bool PCG, class VT>
 __global__ void __launch_bounds__(VB) k_spmv(int nnd, const int* __restrict__ sbase, const int* __restrict__ swidth,
-                                             const int* __restrict__ bcol, const double* __restrict__ bval,
-                                             const double* __restrict__ x, double* __restrict__ y,
-                                             double* __restrict__ part, unsigned* __restrict__ counter,
-                                             double* __restrict__ scal)
+                                             const int* __restrict__ bcol, const VT* __restrict__ bval,
+                                             const VT* __restrict__ x, VT* __restrict__ y,
+                                             double* __restrict__ part, const double* __restrict__ scal)
 {
     if (PCG && scal[S_DONE] != 0.0) return;
     const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -128,13 +120,13 @@ __global__ void __launch_bounds__(VB) k_spmv(int nnd, const int* __restrict__ sb
     if (a < nnd) {
         const long b0 = sbase[s];
         const int w = swidth[s];
-        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        VT y0 = 0, y1 = 0, y2 = 0;
         for (int k = 0; k < w; k++) {
             const long blk = b0 + k;
             const int c = bcol[blk * 64 + l];
             if (c < 0) continue;
-            const double x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
-            const double* v = bval + blk * 9 * 64 + l;
+            const VT x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
+            const VT* v = bval + blk * 9 * 64 + l;
             y0 += v[0 * 64] * x0 + v[1 * 64] * x1 + v[2 * 64] * x2;
             y1 += v[3 * 64] * x0 + v[4 * 64] * x1 + v[5 * 64] * x2;
             y2 += v[6 * 64] * x0 + v[7 * 64] * x1 + v[8 * 64] * x2;
@@ -142,14 +134,22 @@ __global__ void __launch_bounds__(VB) k_spmv(int nnd, const int* __restrict__ sb
         y[3L * a] = y0;
         y[3L * a + 1] = y1;
         y[3L * a + 2] = y2;
-        if (PCG) dotv[0] = y0 * x[3L * a] + y1 * x[3L * a + 1] + y2 * x[3L * a + 2];
+        if (PCG)
+            dotv[0] = (double)y0 * (double)x[3L * a] + (double)y1 * (double)x[3L * a + 1] +
+                      (double)y2 * (double)x[3L * a + 2];
     }
-    if (PCG) {
-        double tot[1];
-        if (grid_sum<1>(dotv, part, counter, tot)) {
-            scal[S_PQ] = tot[0];
-            scal[S_ALPHA] = scal[S_RZ] / tot[0];
-        }
+    if (PCG) block_partials<1>(dotv, part);
+}
+
+// alpha = rz / (p.q)
+__global__ void __launch_bounds__(VB) k_fin_alpha(const double* __restrict__ part, int nparts, double* __restrict__ scal)
+{
+    if (scal[S_DONE] != 0.0) return;
+    double tot[1];
+    ordered_total<1>(part, nparts, tot);
+    if (threadIdx.x == 0) {
+        scal[S_PQ] = tot[0];
+        scal[S_ALPHA] = scal[S_RZ] / tot[0];
     }
 }
 
@@ -158,22 +158,29 @@ __global__ void __launch_bounds__(VB) k_spmv(int nnd, const int* __restrict__ sb
 // ---------------------------------------------------------------------------
 
 // x = 0, r = b, p = D^-1 r; rz = r.D^-1 r, bb = b.b
-__global__ void __launch_bounds__(VB) k_pcg_init(long n, const double* __restrict__ b, const double* __restrict__ dinv,
-                                                 double* __restrict__ x, double* __restrict__ r, double* __restrict__ p,
-                                                 double* __restrict__ part, unsigned* __restrict__ counter,
-                                                 double* __restrict__ scal, double rtol, int maxit)
+template <class VT>
+__global__ void __launch_bounds__(VB) k_pcg_init(long n, const VT* __restrict__ b, const VT* __restrict__ dinv,
+                                                 VT* __restrict__ x, VT* __restrict__ r, VT* __restrict__ p,
+                                                 double* __restrict__ part)
 {
     double v[2] = {0.0, 0.0};
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double bi = b[i], zi = dinv[i] * bi;
-        x[i] = 0.0;
+        const VT bi = b[i], zi = dinv[i] * bi;
+        x[i] = VT(0);
         r[i] = bi;
         p[i] = zi;
-        v[0] += bi * zi;
-        v[1] += bi * bi;
+        v[0] += (double)bi * (double)zi;
+        v[1] += (double)bi * (double)bi;
     }
+    block_partials<2>(v, part);
+}
+
+__global__ void __launch_bounds__(VB) k_fin_init(const double* __restrict__ part, int nparts,
+                                                 double* __restrict__ scal, double rtol, int maxit)
+{
     double tot[2];
-    if (grid_sum<2>(v, part, counter, tot)) {
+    ordered_total<2>(part, nparts, tot);
+    if (threadIdx.x == 0) {
         scal[S_RZ] = tot[0];
         scal[S_BB] = tot[1];
         scal[S_RR] = tot[1];
@@ -185,23 +192,33 @@ __global__ void __launch_bounds__(VB) k_pcg_init(long n, const double* __restric
 }
 
 // x += alpha p, r -= alpha q; rz' = r.D^-1 r, rr = r.r; beta = rz'/rz
-__global__ void __launch_bounds__(VB) k_pcg_update(long n, const double* __restrict__ p, const double* __restrict__ q,
-                                                   const double* __restrict__ dinv, double* __restrict__ x,
-                                                   double* __restrict__ r, double* __restrict__ part,
-                                                   unsigned* __restrict__ counter, double* __restrict__ scal)
+template <class VT>
+__global__ void __launch_bounds__(VB) k_pcg_update(long n, const VT* __restrict__ p, const VT* __restrict__ q,
+                                                   const VT* __restrict__ dinv, VT* __restrict__ x,
+                                                   VT* __restrict__ r, double* __restrict__ part,
+                                                   const double* __restrict__ scal)
 {
     if (scal[S_DONE] != 0.0) return;
-    const double alpha = scal[S_ALPHA];
+    const VT alpha = (VT)scal[S_ALPHA];
     double v[2] = {0.0, 0.0};
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
-        const double ri = r[i] - alpha * q[i];
+        const VT ri = r[i] - alpha * q[i];
         r[i] = ri;
-        v[0] += ri * dinv[i] * ri;
-        v[1] += ri * ri;
+        const double rd = (double)ri;
+        v[0] += rd * (double)dinv[i] * rd;
+        v[1] += rd * rd;
     }
+    block_partials<2>(v, part);
+}
+
+// beta = rz'/rz; convergence test ||r||^2 <= rtol^2 ||b||^2
+__global__ void __launch_bounds__(VB) k_fin_beta(const double* __restrict__ part, int nparts, double* __restrict__ scal)
+{
+    if (scal[S_DONE] != 0.0) return;
     double tot[2];
-    if (grid_sum<2>(v, part, counter, tot)) {
+    ordered_total<2>(part, nparts, tot);
+    if (threadIdx.x == 0) {
         const double it = scal[S_ITER] + 1.0;
         scal[S_ITER] = it;
         scal[S_BETA] = tot[0] / scal[S_RZ];
@@ -212,25 +229,37 @@ __global__ void __launch_bounds__(VB) k_pcg_update(long n, const double* __restr
 }
 
 // p = D^-1 r + beta p
-__global__ void __launch_bounds__(VB) k_pcg_direction(long n, const double* __restrict__ r,
-                                                      const double* __restrict__ dinv, double* __restrict__ p,
-                                                      const double* __restrict__ scal)
+template <class VT>
+__global__ void __launch_bounds__(VB) k_pcg_direction(long n, const VT* __restrict__ r, const VT* __restrict__ dinv,
+                                                      VT* __restrict__ p, const double* __restrict__ scal)
 {
     if (scal[S_DONE] != 0.0) return;
-    const double beta = scal[S_BETA];
+    const VT beta = (VT)scal[S_BETA];
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
 __global__ void __launch_bounds__(VB) k_dot(long n, const double* __restrict__ a, const double* __restrict__ b,
-                                            double* __restrict__ part, unsigned* __restrict__ counter,
-                                            double* __restrict__ scal)
+                                            double* __restrict__ part)
 {
     double v[1] = {0.0};
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         v[0] += a[i] * b[i];
+    block_partials<1>(v, part);
+}
+
+__global__ void __launch_bounds__(VB) k_fin_dot(const double* __restrict__ part, int nparts, double* __restrict__ scal)
+{
     double tot[1];
-    if (grid_sum<1>(v, part, counter, tot)) scal[S_DOT] = tot[0];
+    ordered_total<1>(part, nparts, tot);
+    if (threadIdx.x == 0) scal[S_DOT] = tot[0];
+}
+
+template <class TO, class TI>
+__global__ void k_convert(long n, const TI* __restrict__ src, TO* __restrict__ dst)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = (TO)src[i];
 }
 
 __global__ void k_axpy(long n, double* __restrict__ y, const double* __restrict__ x, double a)
@@ -274,26 +303,44 @@ static unsigned grid_for(long n)
     return (unsigned)std::max(1L, std::min(g, (long)VGRID));
 }
 
-static unsigned* counter_of(tg_ctx* c) { return reinterpret_cast<unsigned*>(c->dstatus.p + 1); }
-
 void launch_expand(tg_ctx* c)
 {
     TimedRegion tr(c, T_EXPAND);
     const long n = c->op.nblocks_padded * 9;
-    hipLaunchKernelGGL(k_expand, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p, c->dval.p,
+    hipLaunchKernelGGL(k_expand<double>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p, c->dval.p,
                        c->op.d_val.p);
     const long nn = 3L * c->nnd;
-    hipLaunchKernelGGL(k_dinv, dim3(grid_for(nn)), dim3(VB), 0, c->stream, nn, c->deqn.p, c->dval.p, c->vdinv.p);
+    hipLaunchKernelGGL(k_dinv<double>, dim3(grid_for(nn)), dim3(VB), 0, c->stream, nn, c->deqn.p, c->dval.p,
+                       c->vdinv.p);
     TG_HIP(hipGetLastError());
 }
 
-void launch_spmv(tg_ctx* c, const double* x, double* y, double*, const double*)
+void launch_expand32(tg_ctx* c)
+{
+    TimedRegion tr(c, T_EXPAND);
+    const long n = c->op.nblocks_padded * 9;
+    hipLaunchKernelGGL(k_expand<float>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p,
+                       c->dval32.p, c->op.d_val32.p);
+    const long nn = 3L * c->nnd;
+    hipLaunchKernelGGL(k_dinv<float>, dim3(grid_for(nn)), dim3(VB), 0, c->stream, nn, c->deqn.p, c->dval32.p,
+                       c->fdinv.p);
+    TG_HIP(hipGetLastError());
+}
+
+void widen_values(tg_ctx* c)
+{
+    const long n = c->msr_len;
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->dval32.p,
+                       c->dval.p);
+    TG_HIP(hipGetLastError());
+}
+
+void launch_spmv(tg_ctx* c, const double* x, double* y)
 {
     TimedRegion tr(c, T_SPMV);
     const unsigned blocks = (unsigned)((c->nnd + VB - 1) / VB);
-    hipLaunchKernelGGL(k_spmv<false>, dim3(blocks), dim3(VB), 0, c->stream, c->nnd, c->op.d_slice_base.p,
-                       c->op.d_slice_width.p, c->op.d_col.p, c->op.d_val.p, x, y, c->dpart.p, counter_of(c),
-                       c->dscal.p);
+    hipLaunchKernelGGL((k_spmv<false, double>), dim3(blocks), dim3(VB), 0, c->stream, c->nnd, c->op.d_slice_base.p,
+                       c->op.d_slice_width.p, c->op.d_col.p, c->op.d_val.p, x, y, c->dpart.p, c->dscal.p);
     TG_HIP(hipGetLastError());
 }
 
@@ -301,8 +348,9 @@ double device_dot(tg_ctx* c, const double* a, const double* b, long n)
 {
     {
         TimedRegion tr(c, T_VECTOR);
-        hipLaunchKernelGGL(k_dot, dim3(grid_for(n)), dim3(VB), 0, c->stream, n, a, b, c->dpart.p, counter_of(c),
-                           c->dscal.p);
+        const unsigned g = grid_for(n);
+        hipLaunchKernelGGL(k_dot, dim3(g), dim3(VB), 0, c->stream, n, a, b, c->dpart.p);
+        hipLaunchKernelGGL(k_fin_dot, dim3(1), dim3(VB), 0, c->stream, c->dpart.p, (int)g, c->dscal.p);
         TG_HIP(hipGetLastError());
     }
     TG_HIP(hipMemcpyAsync(c->hscal, c->dscal.p + S_DOT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -340,23 +388,23 @@ void launch_apply_kbc(tg_ctx* c, double t)
     TG_HIP(hipGetLastError());
 }
 
-// Jacobi-PCG on the expanded operator: b, x in node space.  The reference has
-// no linear CG; this is the textbook method (Saad, Iterative Methods for
-// Sparse Linear Systems, Alg. 9.1) on the reference's building blocks:
-// Multx (MSRMatrixT.cpp:385-420), Jacobi scaling (DiagonalMatrixT.cpp:267-324),
-// InnerProduct (SolverT.cpp:854-860).  Stops at ||r|| <= rtol ||b||.
-void pcg_solve(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
+// Jacobi-PCG on the expanded operator in arithmetic VT; b, x in node space.
+// The reference has no linear CG; this is the textbook method (Saad,
+// Iterative Methods for Sparse Linear Systems, Alg. 9.1) on the reference's
+// building blocks: Multx (MSRMatrixT.cpp:385-420), Jacobi scaling
+// (DiagonalMatrixT.cpp:267-324), InnerProduct (SolverT.cpp:854-860).  Stops at
+// ||r|| <= rtol ||b|| (recurrence residual) or maxit.
+template <class VT>
+static void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r, VT* p, VT* q, double rtol,
+                    int maxit, tg_lin_stats* st)
 {
     const long n = 3L * c->nnd;
     const unsigned g = grid_for(n);
     const unsigned sblocks = (unsigned)((c->nnd + VB - 1) / VB);
-    double* r = c->vr.p;
-    double* p = c->vp.p;
-    double* q = c->vq.p;
     {
         TimedRegion tr(c, T_VECTOR);
-        hipLaunchKernelGGL(k_pcg_init, dim3(g), dim3(VB), 0, c->stream, n, b, c->vdinv.p, x, r, p, c->dpart.p,
-                           counter_of(c), c->dscal.p, rtol, maxit);
+        hipLaunchKernelGGL(k_pcg_init<VT>, dim3(g), dim3(VB), 0, c->stream, n, b, dinv, x, r, p, c->dpart.p);
+        hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(VB), 0, c->stream, c->dpart.p, (int)g, c->dscal.p, rtol, maxit);
     }
     const int CHUNK = 16;
     int issued = 0;
@@ -364,15 +412,18 @@ void pcg_solve(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg
         for (int k = 0; k < CHUNK; k++) {
             {
                 TimedRegion tr(c, T_SPMV);
-                hipLaunchKernelGGL(k_spmv<true>, dim3(sblocks), dim3(VB), 0, c->stream, c->nnd, c->op.d_slice_base.p,
-                                   c->op.d_slice_width.p, c->op.d_col.p, c->op.d_val.p, p, q, c->dpart.p,
-                                   counter_of(c), c->dscal.p);
+                hipLaunchKernelGGL((k_spmv<true, VT>), dim3(sblocks), dim3(VB), 0, c->stream, c->nnd,
+                                   c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, p, q, c->dpart.p,
+                                   c->dscal.p);
             }
             {
                 TimedRegion tr(c, T_VECTOR);
-                hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(VB), 0, c->stream, n, p, q, c->vdinv.p, x, r,
-                                   c->dpart.p, counter_of(c), c->dscal.p);
-                hipLaunchKernelGGL(k_pcg_direction, dim3(g), dim3(VB), 0, c->stream, n, r, c->vdinv.p, p, c->dscal.p);
+                hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(VB), 0, c->stream, c->dpart.p, (int)sblocks,
+                                   c->dscal.p);
+                hipLaunchKernelGGL(k_pcg_update<VT>, dim3(g), dim3(VB), 0, c->stream, n, p, q, dinv, x, r, c->dpart.p,
+                                   c->dscal.p);
+                hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(VB), 0, c->stream, c->dpart.p, (int)g, c->dscal.p);
+                hipLaunchKernelGGL(k_pcg_direction<VT>, dim3(g), dim3(VB), 0, c->stream, n, r, dinv, p, c->dscal.p);
             }
         }
         TG_HIP(hipGetLastError());
@@ -387,6 +438,28 @@ void pcg_solve(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg
         st->rel_residual = bb > 0.0 ? sqrt(c->hscal[S_RR] / bb) : 0.0;
         st->converged = (c->hscal[S_RR] <= c->hscal[S_THRESH]) ? 1 : 0;
     }
+}
+
+void pcg_solve(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
+{
+    pcg_run<double>(c, c->op.d_val.p, c->vdinv.p, b, x, c->vr.p, c->vp.p, c->vq.p, rtol, maxit, st);
+}
+
+// Mixed precision inner solve: b (FP64) is rounded to FP32, PCG runs in FP32
+// with FP64 dot products, the correction is widened back to FP64.
+void pcg_solve_mixed(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
+{
+    const long n = 3L * c->nnd;
+    {
+        TimedRegion tr(c, T_VECTOR);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(VB), 0, c->stream, n, b, c->fb.p);
+    }
+    pcg_run<float>(c, c->op.d_val32.p, c->fdinv.p, c->fb.p, c->fx.p, c->fr.p, c->fp.p, c->fq.p, rtol, maxit, st);
+    {
+        TimedRegion tr(c, T_VECTOR);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(VB), 0, c->stream, n, c->fx.p, x);
+    }
+    TG_HIP(hipGetLastError());
 }
 
 }  // namespace tg

@@ -232,6 +232,16 @@ int tg_set_formulation(tg_ctx* c, int f)
     });
 }
 
+int tg_set_precision(tg_ctx* c, int prec)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (prec != TG_FP64 && prec != TG_MIXED) fail(TG_BAD_INPUT, "unknown precision");
+        c->precision = prec;
+        c->initialized = false;
+    });
+}
+
 int tg_set_kbc(tg_ctx* c, int nkbc, const int* nodes, const int* dofs, const double* values)
 {
     if (!c) return TG_BAD_INPUT;
@@ -279,6 +289,9 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         c->dval.alloc((size_t)c->msr_len);
         c->dslot.alloc(300 * (size_t)c->nel);
         for (auto* v : {&c->vx, &c->vr, &c->vz, &c->vp, &c->vq, &c->vdinv, &c->vwork}) v->alloc(nn);
+        const bool mixed = c->precision == TG_MIXED;
+        c->dval32.alloc(mixed ? (size_t)c->msr_len : 0);
+        for (auto* v : {&c->fx, &c->fr, &c->fp, &c->fq, &c->fdinv, &c->fb}) v->alloc(mixed ? nn : 0);
         const size_t sblocks = ((size_t)c->nnd + 255) / 256;
         c->dpart.alloc(2 * std::max<size_t>(sblocks, 1024) + 16);
         std::vector<int> eqn0(nn);
@@ -304,6 +317,7 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         }
         launch_build_slots(c);
         build_bsell(c);
+        c->op.d_val32.alloc(mixed ? (size_t)c->op.nblocks_padded * 9 : 0);
         TG_HIP(hipStreamSynchronize(c->stream));
         c->initialized = true;
         if (n_eq) *n_eq = c->n_eq;
@@ -406,12 +420,39 @@ int tg_form_lhs(tg_ctx* c)
     });
 }
 
+static bool mixed_values(const tg_ctx* c) { return c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN; }
+
+// FP64 copy of the assembled values: in mixed precision the stiffness is
+// assembled in FP32 and widened here for the GlobalMatrixT queries.
+static void fp64_values(tg_ctx* c)
+{
+    if (mixed_values(c)) widen_values(c);
+}
+
+// operator of the inner linear solve, in the solve's precision
+static void prepare_operator(tg_ctx* c)
+{
+    if (mixed_values(c))
+        launch_expand32(c);
+    else
+        launch_expand(c);
+}
+
+static void solve_linear(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
+{
+    if (mixed_values(c))
+        pcg_solve_mixed(c, b, x, rtol, maxit, st);
+    else
+        pcg_solve(c, b, x, rtol, maxit, st);
+}
+
 int tg_get_msr_values(tg_ctx* c, double* val)
 {
     if (!c || !val) return TG_BAD_INPUT;
     return guarded(c, [&] {
         require_init(c);
         check_status(c);
+        fp64_values(c);
         TG_HIP(hipMemcpyAsync(val, c->dval.p, sizeof(double) * c->msr_len, hipMemcpyDeviceToHost, c->stream));
         TG_HIP(hipStreamSynchronize(c->stream));
     });
@@ -423,10 +464,11 @@ int tg_multx(tg_ctx* c, const double* x, double* b)
     return guarded(c, [&] {
         require_init(c);
         check_status(c);
+        fp64_values(c);
         TG_HIP(hipMemcpyAsync(c->vwork.p, x, sizeof(double) * c->n_eq, hipMemcpyHostToDevice, c->stream));
         eq_to_node(c, c->vwork.p, c->vx.p);
         launch_expand(c);
-        launch_spmv(c, c->vx.p, c->vq.p, nullptr, nullptr);
+        launch_spmv(c, c->vx.p, c->vq.p);
         node_to_eq(c, c->vq.p, c->vwork.p);
         TG_HIP(hipMemcpyAsync(b, c->vwork.p, sizeof(double) * c->n_eq, hipMemcpyDeviceToHost, c->stream));
         TG_HIP(hipStreamSynchronize(c->stream));
@@ -438,6 +480,7 @@ int tg_copy_diagonal(tg_ctx* c, double* diag)
     if (!c || !diag) return TG_BAD_INPUT;
     return guarded(c, [&] {
         require_init(c);
+        fp64_values(c);
         TG_HIP(hipMemcpyAsync(diag, c->dval.p, sizeof(double) * c->n_eq, hipMemcpyDeviceToHost, c->stream));
         TG_HIP(hipStreamSynchronize(c->stream));
     });
@@ -451,8 +494,8 @@ int tg_solve(tg_ctx* c, double* rhs, double rtol, int maxit, tg_lin_stats* st)
         check_status(c);
         TG_HIP(hipMemcpyAsync(c->vwork.p, rhs, sizeof(double) * c->n_eq, hipMemcpyHostToDevice, c->stream));
         eq_to_node(c, c->vwork.p, c->vz.p);
-        launch_expand(c);
-        pcg_solve(c, c->vz.p, c->vx.p, rtol, maxit, st);
+        prepare_operator(c);
+        solve_linear(c, c->vz.p, c->vx.p, rtol, maxit, st);
         node_to_eq(c, c->vx.p, c->vwork.p);
         TG_HIP(hipMemcpyAsync(rhs, c->vwork.p, sizeof(double) * c->n_eq, hipMemcpyDeviceToHost, c->stream));
         TG_HIP(hipStreamSynchronize(c->stream));
@@ -467,9 +510,9 @@ static void correction(tg_ctx* c, const tg_newton_opts* o, double* err_before, l
     check_status(c);
     *err_before = std::sqrt(device_dot(c, c->dR.p, c->dR.p, 3L * c->nnd));
     launch_stiffness(c, c->du.p);
-    launch_expand(c);
+    prepare_operator(c);
     tg_lin_stats st{};
-    pcg_solve(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
+    solve_linear(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
     check_status(c);
     *lin_its += st.iterations;
     launch_axpy(c, c->du.p, c->vx.p, 1.0, 3L * c->nnd);
@@ -514,9 +557,9 @@ int tg_newton(tg_ctx* c, const tg_newton_opts* o, tg_newton_stats* s)
         long lin = 0;
         for (;;) {
             launch_stiffness(c, c->du.p);
-            launch_expand(c);
+            prepare_operator(c);
             tg_lin_stats st{};
-            pcg_solve(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
+            solve_linear(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
             check_status(c);
             lin += st.iterations;
             launch_axpy(c, c->du.p, c->vx.p, 1.0, 3L * c->nnd);

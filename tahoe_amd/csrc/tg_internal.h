@@ -57,6 +57,7 @@ struct Bsell {
     DBuf<int> d_col;                       // [nblocks_padded] column node, -1 = padding
     DBuf<int> d_src;                       // [nblocks_padded * 9] MSR val index, -1 = zero
     DBuf<double> d_val;                    // [nblocks_padded * 9]
+    DBuf<float> d_val32;                   // FP32 copy for the mixed-precision inner solve
 };
 
 struct Timing {
@@ -77,6 +78,7 @@ struct tg_ctx {
     std::vector<int> conn, mat;
     std::vector<double> mu, kappa;
     int formulation = TG_TOTAL_LAGRANGIAN;
+    int precision = TG_FP64;
     std::vector<int> kbc_nodes, kbc_dofs;
     std::vector<double> kbc_vals;
 
@@ -99,12 +101,14 @@ struct tg_ctx {
     tg::DBuf<double> dkbc_val;
     tg::DBuf<int> dbindx;
     tg::DBuf<double> dval;
+    tg::DBuf<float> dval32;             // mixed precision: FP32 stiffness values
     tg::DBuf<int> dslot;                // [nel*300] element upper entry -> val index (-1 none)
     tg::DBuf<int> dperm;                // [n_eq] equation -> node-space index
     tg::Bsell op;
 
     // PCG work vectors in node space
     tg::DBuf<double> vx, vr, vz, vp, vq, vdinv, vwork;
+    tg::DBuf<float> fx, fr, fp, fq, fdinv, fb;   // mixed precision inner PCG vectors
     tg::DBuf<double> dpart;             // partial reduction sums
     tg::DBuf<double> dscal;             // device scalars
     tg::DBuf<int> dstatus;              // device error flag (7 = bad Jacobian)
@@ -130,7 +134,10 @@ void launch_residual(tg_ctx* c, const double* u, double* R_node);
 void launch_stiffness(tg_ctx* c, const double* u);
 void launch_build_slots(tg_ctx* c);
 void launch_expand(tg_ctx* c);
-void launch_spmv(tg_ctx* c, const double* x, double* y, double* dot_out_partial, const double* dot_with);
+void launch_spmv(tg_ctx* c, const double* x, double* y);
+void launch_expand32(tg_ctx* c);
+void widen_values(tg_ctx* c);
+void pcg_solve_mixed(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
 double device_dot(tg_ctx* c, const double* a, const double* b, long n);
 void pcg_solve(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
 void eq_to_node(tg_ctx* c, const double* src_eq, double* dst_node);
