@@ -37,7 +37,6 @@ FP64_PEAK = 78.6e12      # flop/s, MI355X FP64 vector and matrix (dense)
 # algorithmic work per unit (see DESIGN.md "Measurement")
 RES_FLOP_PER_EL = 6.2e3          # residual: shape derivatives + kinematics + stress + B^T P
 STIFF_FLOP_PER_EL = 83.0e3       # stiffness: 8 x (2*24*6*24 + 2*6*6*24) material + geometric + kinematics
-SPMV_BYTES_PER_BLOCK = 9 * 8 + 4  # 3x3 FP64 block + its column index
 
 
 def parse_args():
@@ -48,6 +47,13 @@ def parse_args():
     ap.add_argument("--n", type=int, default=128, help="elements per cube edge")
     ap.add_argument("--lin-rtol", type=float, default=1e-10)
     ap.add_argument("--cpu-n", type=int, default=64, help="edge of the CPU-baseline sample cube (0: skip)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for profilers: 2 Newton corrections with 48 PCG iterations each")
+    ap.add_argument("--stiff-chunk", type=int, default=None,
+                    help="elements per range of the colour-ordered stiffness launches (default: autotuned)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only the plain copy kernel (1 GiB) used to calibrate HBM counters")
     return ap.parse_args()
 
 
@@ -85,6 +91,13 @@ def cpu_baseline(n):
 
 def main():
     a = parse_args()
+    if a.calibrate_copy:
+        from tahoe_amd import api
+        nbytes = 1 << 30
+        ms = api.calib_copy(nbytes, reps=10)
+        print(json.dumps({"copy_kernel": "k_copy_calib", "bytes_read": nbytes, "bytes_written": nbytes,
+                          "launches": 11, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU layout is "
               "described in DESIGN.md and is not measured here.", file=sys.stderr)
@@ -99,17 +112,35 @@ def main():
     s.set_mesh(m.X, m.conn, m.mat)
     s.set_material(0, *api.mu_kappa(1000.0, 0.3))
     s.set_formulation(api.TOTAL_LAGRANGIAN)
+    s.set_precision(api.MIXED if a.precision == "mixed" else api.FP64)
     s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
     n_eq, msr_len = s.initialize()
     print(f"# setup {time.perf_counter() - t_setup:.1f} s: nel={m.nel} nnd={m.nnd} n_eq={n_eq} "
           f"msr_upper_nnz={msr_len - 1}", file=sys.stderr, flush=True)
-    opts = s.opts(lin_rtol=a.lin_rtol, lin_maxit=200000)
+    if a.stiff_chunk is not None:
+        s.set_tuning(api.TUNE_STIFF_CHUNK, a.stiff_chunk)
+        tuned = {"best": a.stiff_chunk, "ms": {}}
+    else:
+        from tahoe_amd import autotune
+        tuned = autotune.tune_stiffness_chunk(s, reps=1)
+    print(f"# stiffness chunk {tuned['best']} (autotune ms: {tuned['ms']})", file=sys.stderr, flush=True)
+    opts = s.opts(lin_rtol=a.lin_rtol, lin_maxit=48 if a.profile else 200000)
     t_load = 0.2
 
     def step():
         s.reset_step()
         s.apply_kbc(t_load)
         return s.newton_correction(opts)
+
+    if a.profile:
+        s.timing_enable(True)
+        for _ in range(2):
+            s.reset_step()
+            s.apply_kbc(t_load)
+            s.newton_correction(opts)
+        s.synchronize()
+        print(json.dumps({"profile_run": s.timing()}), flush=True)
+        return
 
     for w in range(a.warmup):
         st = step()
@@ -138,7 +169,8 @@ def main():
     value = m.nel / ((r_ms + k_ms) * 1e-3)
     spmv_avg_ms = tm["spmv_ms"] / max(1, tm["spmv_launches"])
     nblocks = (n + 1 + 2 * n) ** 3  # (3n+1)^3 node-neighbour blocks of the structured cube
-    spmv_bytes = nblocks * SPMV_BYTES_PER_BLOCK + 2 * 24 * m.nnd
+    vbytes = 8 if a.precision == "fp64" else 4
+    spmv_bytes = nblocks * (9 * vbytes + 4) + 2 * 3 * vbytes * m.nnd
     spmv_gbs = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9
     res_gflops = RES_FLOP_PER_EL * m.nel / (r_ms * 1e-3) / 1e9
     stf_gflops = STIFF_FLOP_PER_EL * m.nel / (k_ms * 1e-3) / 1e9
@@ -155,12 +187,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if a.precision == "fp64" else "f32/f64",
         "data": "synthetic",
         "config": {"workload": f"{n}^3 Q1 hex TL neo-Hookean (Simo_isotropic E=1000 nu=0.3), +-5% perturbed, "
                                "uniaxial 5% load step 1/5, one Newton correction: R + K into MSR + Jacobi-PCG "
                                f"to {a.lin_rtol:g}",
-                   "elements": m.nel, "nodes": m.nnd, "equations": n_eq, "msr_upper_nnz": msr_len - 1},
+                   "elements": m.nel, "nodes": m.nnd, "equations": n_eq, "msr_upper_nnz": msr_len - 1,
+                   "stiffness_chunk": tuned["best"]},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": spmv_gbs / (HBM_PEAK / 1e9), "traffic": traffic_from_profiles(),
                      "kernel": "k_spmv<true> (block sliced-ELL SpMV + p.q)",
@@ -170,11 +203,12 @@ def main():
         "residual": {"ms": r_ms, "elements_per_s": m.nel / (r_ms * 1e-3), "gflops": res_gflops,
                      "frac_fp64_peak": res_gflops * 1e9 / FP64_PEAK},
         "stiffness": {"ms": k_ms, "elements_per_s": m.nel / (k_ms * 1e-3), "gflops": stf_gflops,
-                      "frac_fp64_mfma_peak": stf_gflops * 1e9 / FP64_PEAK},
+                      "frac_fp64_peak": stf_gflops * 1e9 / FP64_PEAK},
         "pcg": {"iterations_per_step": its, "ms_per_step": pcg_ms,
                 "dof_iterations_per_s": n_eq * its / (pcg_ms * 1e-3) if pcg_ms > 0 else None,
                 "expand_ms": tm["expand_ms"] / K},
         "timing_ms": tm,
+        "autotune_stiffness_ms": {str(k): v for k, v in tuned["ms"].items()},
     }
     print(json.dumps(line), flush=True)
 

@@ -101,6 +101,17 @@ int tg_set_precision(tg_ctx* ctx, int precision);
  * tahoe/src/primitives/KBC_CardT.cpp:53-64, linear schedule). */
 int tg_set_kbc(tg_ctx* ctx, int nkbc, const int* nodes, const int* dofs, const double* values);
 
+/* Batched problems: the mesh of tg_set_mesh is the disjoint union of nseg
+ * independent sub-meshes; sub-mesh g owns nodes [node_begin[g],
+ * node_begin[g+1]) and the elements on them.  node_begin[0] = 0,
+ * node_begin[nseg] = nnd, and every other begin is a multiple of 64 (pad
+ * with nodes no element uses, all three dofs carrying kinematic cards).
+ * Each sub-problem gets its own Newton and PCG control; its arithmetic, and
+ * so its result, is the same alone and in any batch.  No counterpart in the
+ * reference (one problem per run); for callers that solve many small
+ * problems (FE^2 micro-scale RVEs). */
+int tg_set_segments(tg_ctx* ctx, int nseg, const int* node_begin);
+
 /* Equation numbering, MSR pattern, element colouring and device storage.
  * Replaces FEManagerT::SetEquationSystem (FEManagerT.cpp:2248-2313) ->
  * NodeManagerT::SetEquationNumbers (NodeManagerT.cpp:712-760) and
@@ -147,6 +158,10 @@ int tg_solve(tg_ctx* ctx, double* rhs, double rtol, int maxit, tg_lin_stats* sta
 /* One load step's Newton loop, device resident: R, ||R||, K, PCG, u += du
  * until ExitIteration's tests hold (NLSolver::Solve, NLSolver.cpp:57-264). */
 int tg_newton(tg_ctx* ctx, const tg_newton_opts* opts, tg_newton_stats* stats);
+/* tg_newton for every segment at once: per-segment ||R||, ExitIteration
+ * tests and PCG; converged or failed segments stop updating while the rest
+ * continue.  stats[nseg] receives each segment's outcome. */
+int tg_newton_batch(tg_ctx* ctx, const tg_newton_opts* opts, tg_newton_stats* stats);
 /* One Newton correction without the convergence loop: R, K, PCG, u += du.
  * stats->error0 gets ||R|| before the update (used by benchmarks). */
 int tg_newton_correction(tg_ctx* ctx, const tg_newton_opts* opts, tg_newton_stats* stats);
@@ -156,9 +171,20 @@ int tg_synchronize(tg_ctx* ctx);
 int tg_timing_enable(tg_ctx* ctx, int enable);
 int tg_timing_reset(tg_ctx* ctx);
 int tg_timing_get(tg_ctx* ctx, tg_timing* out);
-/* layout self-test of the FP64 MFMA tile the stiffness kernel relies on:
- * returns 0 and max|error| in *err when D = A B matches on the device. */
-int tg_selftest_mfma(int device, double* err);
+/* plain device-to-device copy kernel (k_copy_calib) over `bytes` bytes, run
+ * `reps` times; *ms gets the mean kernel time.  Calibrates the HBM counters
+ * of a profiler against a kernel of known traffic (bytes read + written). */
+int tg_calib_copy(int device, long bytes, int reps, double* ms);
+
+/* ---- tuning ------------------------------------------------------------ */
+/* Launch-shape knobs (no counterpart in the reference; set by the
+ * autotuner in tahoe_amd/autotune.py or by hand):
+ * TG_TUNE_STIFF_CHUNK  elements per range of the colour-ordered stiffness
+ *                      launches (0: one launch per colour, -1: derived from
+ *                      the matrix footprint). */
+enum { TG_TUNE_STIFF_CHUNK = 1 };
+int tg_set_tuning(tg_ctx* ctx, int knob, long value);
+int tg_get_tuning(tg_ctx* ctx, int knob, long* value);
 
 #ifdef __cplusplus
 }

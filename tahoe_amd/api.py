@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libtahoe_gpu.so")
 
 TOTAL_LAGRANGIAN, Q1P0 = 0, 1
 FP64, MIXED = 0, 1
+TUNE_STIFF_CHUNK = 1
 
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
@@ -22,10 +23,11 @@ I = C.POINTER(C.c_int)
 # exported symbols, in declaration order of include/tahoe_gpu.h
 SYMBOLS = [
     "tg_create", "tg_destroy", "tg_last_error", "tg_set_mesh", "tg_set_material", "tg_set_formulation",
-    "tg_set_precision", "tg_set_kbc", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
+    "tg_set_precision", "tg_set_kbc", "tg_set_segments", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
     "tg_get_displacement", "tg_apply_kbc", "tg_close_step", "tg_reset_step", "tg_form_rhs", "tg_form_lhs",
-    "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_correction",
-    "tg_synchronize", "tg_timing_enable", "tg_timing_reset", "tg_timing_get", "tg_selftest_mfma",
+    "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_batch", "tg_newton_correction",
+    "tg_synchronize", "tg_timing_enable", "tg_timing_reset", "tg_timing_get", "tg_calib_copy", "tg_set_tuning",
+    "tg_get_tuning",
 ]
 
 
@@ -82,6 +84,7 @@ def lib():
         L.tg_set_formulation.argtypes = [vp, C.c_int]
         L.tg_set_precision.argtypes = [vp, C.c_int]
         L.tg_set_kbc.argtypes = [vp, C.c_int, I, I, D]
+        L.tg_set_segments.argtypes = [vp, C.c_int, I]
         L.tg_initialize.argtypes = [vp, I, C.POINTER(C.c_long)]
         L.tg_get_eqnos.argtypes = [vp, I]
         L.tg_get_msr_bindx.argtypes = [vp, I]
@@ -98,11 +101,14 @@ def lib():
         L.tg_solve.argtypes = [vp, D, C.c_double, C.c_int, C.POINTER(LinStats)]
         L.tg_newton.argtypes = [vp, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)]
         L.tg_newton_correction.argtypes = [vp, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)]
+        L.tg_newton_batch.argtypes = [vp, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)]
         L.tg_synchronize.argtypes = [vp]
         L.tg_timing_enable.argtypes = [vp, C.c_int]
         L.tg_timing_reset.argtypes = [vp]
         L.tg_timing_get.argtypes = [vp, C.POINTER(Timing)]
-        L.tg_selftest_mfma.argtypes = [C.c_int, D]
+        L.tg_calib_copy.argtypes = [C.c_int, C.c_long, C.c_int, D]
+        L.tg_set_tuning.argtypes = [vp, C.c_int, C.c_long]
+        L.tg_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_long)]
         _lib = L
     return _lib
 
@@ -115,10 +121,13 @@ def _i(a):
     return a.ctypes.data_as(I)
 
 
-def selftest_mfma(device=0):
-    err = C.c_double()
-    rc = lib().tg_selftest_mfma(device, C.byref(err))
-    return rc, err.value
+def calib_copy(nbytes, reps=10, device=0):
+    """Mean time (ms) of a plain copy kernel over nbytes (reads nbytes, writes nbytes)."""
+    ms = C.c_double()
+    rc = lib().tg_calib_copy(device, int(nbytes), reps, C.byref(ms))
+    if rc:
+        raise TahoeGpuError(rc, "tg_calib_copy failed")
+    return ms.value
 
 
 class Solid:
@@ -170,6 +179,11 @@ class Solid:
         d = np.ascontiguousarray(dofs, dtype=np.int32)
         v = np.ascontiguousarray(values, dtype=np.float64)
         self._ok(lib().tg_set_kbc(self._h, len(n), _i(n), _i(d), _d(v)))
+
+    def set_segments(self, node_begin):
+        nb = np.ascontiguousarray(node_begin, dtype=np.int32)
+        self.nseg = len(nb) - 1
+        self._ok(lib().tg_set_segments(self._h, self.nseg, _i(nb)))
 
     def initialize(self):
         n = C.c_int()
@@ -254,6 +268,12 @@ class Solid:
         self._ok(rc)
         return st
 
+    def newton_batch(self, opts=None):
+        o = opts or self.opts()
+        st = (NewtonStats * getattr(self, "nseg", 1))()
+        self._ok(lib().tg_newton_batch(self._h, C.byref(o), st))
+        return list(st)
+
     def newton_correction(self, opts=None):
         o = opts or self.opts()
         st = NewtonStats()
@@ -273,6 +293,14 @@ class Solid:
         t = Timing()
         self._ok(lib().tg_timing_get(self._h, C.byref(t)))
         return t.as_dict()
+
+    def set_tuning(self, knob, value):
+        self._ok(lib().tg_set_tuning(self._h, knob, int(value)))
+
+    def tuning(self, knob):
+        v = C.c_long()
+        self._ok(lib().tg_get_tuning(self._h, knob, C.byref(v)))
+        return v.value
 
 
 def static_solve(solid: Solid, num_steps: int, opts=None, on_step=None):

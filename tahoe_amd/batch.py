@@ -1,0 +1,109 @@
+"""Batches of independent problems (e.g. FE^2 micro-scale RVEs) in one
+device context.
+
+The meshes are packed as a disjoint union whose sub-meshes start at node
+multiples of 64 (``tg_set_segments``); padding nodes belong to no element and
+carry fixed kinematic cards, so they add no equations.  Every sub-problem
+keeps its own Newton/PCG control and the same arithmetic as when solved
+alone.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import api
+from .mesh import HexMesh, KBC
+
+
+@dataclass
+class Problem:
+    mesh: HexMesh
+    kbc: KBC
+    E: list      # Young's modulus per local material id
+    nu: list     # Poisson ratio per local material id
+
+
+@dataclass
+class Packed:
+    X: np.ndarray
+    conn: np.ndarray
+    mat: np.ndarray
+    kbc: KBC
+    node_begin: np.ndarray   # [nseg+1] first node of each sub-problem (padded)
+    node_count: np.ndarray   # [nseg] real nodes of each sub-problem
+    materials: list          # [(E, nu)] global table
+
+
+def pack(problems: list) -> Packed:
+    mats = []
+    index = {}
+    Xs, conns, matl, kn, kd, kv = [], [], [], [], [], []
+    begins = [0]
+    counts = []
+    off = 0
+    for p in problems:
+        gid = []
+        for E, nu in zip(p.E, p.nu):
+            key = (float(E), float(nu))
+            if key not in index:
+                index[key] = len(mats)
+                mats.append(key)
+            gid.append(index[key])
+        gid = np.asarray(gid, dtype=np.int32)
+        nnd = p.mesh.nnd
+        pad = (-nnd) % 64
+        X = np.zeros((nnd + pad, 3))
+        X[:nnd] = p.mesh.X
+        Xs.append(X)
+        conns.append(p.mesh.conn + off)
+        matl.append(gid[p.mesh.mat])
+        kn.append(p.kbc.nodes + off)
+        kd.append(p.kbc.dofs)
+        kv.append(p.kbc.values)
+        if pad:
+            padn = np.arange(off + nnd, off + nnd + pad, dtype=np.int32)
+            kn.append(np.repeat(padn, 3))
+            kd.append(np.tile(np.arange(3, dtype=np.int32), pad))
+            kv.append(np.zeros(3 * pad))
+        counts.append(nnd)
+        off += nnd + pad
+        begins.append(off)
+    # the last sub-problem needs no trailing padding, but keeping it is harmless
+    return Packed(np.concatenate(Xs), np.concatenate(conns).astype(np.int32), np.concatenate(matl).astype(np.int32),
+                  KBC(np.concatenate(kn), np.concatenate(kd), np.concatenate(kv)),
+                  np.asarray(begins, dtype=np.int32), np.asarray(counts, dtype=np.int32), mats)
+
+
+def make_solid(pk: Packed, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64, device=0) -> api.Solid:
+    s = api.Solid(device)
+    s.set_mesh(pk.X, pk.conn, pk.mat)
+    for m, (E, nu) in enumerate(pk.materials):
+        s.set_material(m, *api.mu_kappa(E, nu))
+    s.set_formulation(formulation)
+    s.set_precision(precision)
+    s.set_kbc(pk.kbc.nodes, pk.kbc.dofs, pk.kbc.values)
+    s.set_segments(pk.node_begin)
+    s.initialize()
+    return s
+
+
+def unpack(pk: Packed, u: np.ndarray) -> list:
+    return [u[b:b + n].copy() for b, n in zip(pk.node_begin[:-1], pk.node_count)]
+
+
+def static_solve_batch(s: api.Solid, pk: Packed, num_steps: int, opts=None):
+    """FEManagerT::Solve's static loop for every sub-problem at once; returns
+    per-problem converged displacements of the last step and the per-step
+    Newton statistics [step][problem]."""
+    stats = []
+    for k in range(1, num_steps + 1):
+        s.apply_kbc(k / num_steps)
+        st = s.newton_batch(opts)
+        if any(x.status != 0 for x in st):
+            bad = [i for i, x in enumerate(st) if x.status != 0]
+            raise api.TahoeGpuError(1, f"Newton failed at step {k} for problems {bad[:10]}")
+        s.close_step()
+        stats.append(st)
+    return unpack(pk, s.displacement()), stats

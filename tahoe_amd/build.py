@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtahoe_gpu.so")
 
-SOURCES = ["tg_setup.cpp", "tg_api.cpp", "k_element.hip", "k_solver.hip"]
+SOURCES = ["tg_setup.cpp", "tg_api.cpp", "k_element.hip", "k_solver.hip", "k_pcg.hip"]
 HEADERS = ["tg_internal.h", "tg_kinematics.h", "../../include/tahoe_gpu.h"]
 
 ARCH = os.environ.get("TAHOE_GPU_ARCH", "gfx950")

@@ -142,7 +142,7 @@ void launch_residual(tg_ctx* c, const double* u, double* R_node)
 }
 
 // ---------------------------------------------------------------------------
-// stiffness (FP64 MFMA)
+// stiffness
 // ---------------------------------------------------------------------------
 
 // Position of (m, n), m <= n < 24, in the packed upper triangle.
@@ -162,6 +162,7 @@ __device__ __forceinline__ int upper_index(int m, int n) { return m * 24 - (m * 
 constexpr int SK_EPB = 16;  // elements per block
 constexpr int SK_THREADS = 36 * SK_EPB;
 
+// Voigt index of (i,k): (11,22,33,23,13,12)
 __device__ __forceinline__ constexpr int voigt(int i, int k) { return i == k ? i : 6 - i - k; }
 
 // packed upper-triangle index of a symmetric 6x6 (row-major, v <= w)
@@ -313,6 +314,58 @@ __global__ void __launch_bounds__(SK_THREADS) k_stiffness_tl(const int* __restri
     if (bad) atomicOr(status, 7);
 }
 
+// Launch list of the colour passes.  Within one colour no two elements share
+// a node, so any split of a colour into launches is race-free.  Each
+// segment's elements are split by their rank inside the segment into ranges
+// of `chunk`, and all colours of range j run before range j+1: the matrix
+// rows a range touches then stay resident in the memory-side cache across
+// the colour passes instead of the whole matrix streaming once per colour.
+// An entry's contributions are summed in (range, colour) order, which
+// depends only on its own segment and the chunk, so a problem's result is
+// the same alone and inside any batch, run to run.
+static const std::vector<std::pair<int, int>>& stiffness_launches(tg_ctx* c)
+{
+    long chunk = c->stiff_chunk;
+    if (chunk < 0) {
+        const double vb = (c->precision == TG_MIXED ? 4.0 : 8.0) * (double)c->msr_len / std::max(1, c->nel);
+        chunk = (long)(64.0 * (1 << 20) / std::max(vb, 1.0));
+        chunk = std::max(chunk, 32768L);
+    }
+    if (chunk == 0 || chunk >= c->nel) chunk = c->nel;
+    if (c->stiff_chunk_built == chunk) return c->stiff_launches;
+    // range index of every element: rank inside its segment / chunk
+    const int nseg = c->seg.nseg;
+    std::vector<long> rank((size_t)nseg, 0);
+    std::vector<int> key((size_t)c->nel);
+    int nkeys = 1;
+    for (int e = 0; e < c->nel; e++) {
+        const int g = c->seg.elem_seg.empty() ? 0 : c->seg.elem_seg[e];
+        key[e] = (int)(rank[g]++ / chunk);
+        nkeys = std::max(nkeys, key[e] + 1);
+    }
+    // order: (range, colour, element); the colour lists are ascending
+    std::vector<int> cnt((size_t)nkeys * c->ncolors + 1, 0);
+    for (int col = 0; col < c->ncolors; col++)
+        for (int i = c->color_ptr[col]; i < c->color_ptr[col + 1]; i++)
+            cnt[(size_t)key[c->color_elems[i]] * c->ncolors + col + 1]++;
+    for (size_t k = 1; k < cnt.size(); k++) cnt[k] += cnt[k - 1];
+    std::vector<int> order((size_t)c->nel);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int col = 0; col < c->ncolors; col++)
+        for (int i = c->color_ptr[col]; i < c->color_ptr[col + 1]; i++) {
+            const int e = c->color_elems[i];
+            order[(size_t)fill[(size_t)key[e] * c->ncolors + col]++] = e;
+        }
+    c->stiff_launches.clear();
+    for (size_t k = 0; k + 1 < cnt.size(); k++)
+        if (cnt[k + 1] > cnt[k]) c->stiff_launches.emplace_back(cnt[k], cnt[k + 1] - cnt[k]);
+    if (c->dstiff_elems.n != (size_t)c->nel) c->dstiff_elems.alloc((size_t)c->nel);
+    TG_HIP(hipMemcpyAsync(c->dstiff_elems.p, order.data(), sizeof(int) * c->nel, hipMemcpyHostToDevice, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    c->stiff_chunk_built = chunk;
+    return c->stiff_launches;
+}
+
 void launch_stiffness(tg_ctx* c, const double* u)
 {
     TimedRegion tr(c, T_STIFFNESS);
@@ -325,17 +378,16 @@ void launch_stiffness(tg_ctx* c, const double* u)
         launch_q1p0(c, u, c->dR.p, true);
         return;
     }
-    for (int col = 0; col < c->ncolors; col++) {
-        const int first = c->color_ptr[col], count = c->color_ptr[col + 1] - first;
-        if (count == 0) continue;
+    for (const auto& fc : stiffness_launches(c)) {
+        const int first = fc.first, count = fc.second;
         const int blocks = (count + SK_EPB - 1) / SK_EPB;
         if (mixed)
             hipLaunchKernelGGL(k_stiffness_tl<float>, dim3(blocks), dim3(SK_THREADS), 0, c->stream,
-                               c->dcolor_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                               c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
                                c->dslot.p, c->dval32.p, c->dstatus.p);
         else
             hipLaunchKernelGGL(k_stiffness_tl<double>, dim3(blocks), dim3(SK_THREADS), 0, c->stream,
-                               c->dcolor_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                               c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
                                c->dslot.p, c->dval.p, c->dstatus.p);
     }
     TG_HIP(hipGetLastError());
@@ -397,63 +449,6 @@ void launch_build_slots(tg_ctx* c)
 }
 
 }  // namespace tg
-
-namespace tg {
-
-// One 16x16x4 FP64 MFMA on known operands; checks the operand/result lane
-// layout the stiffness kernel assumes: A[i][k] from lane k*16+i,
-// B[k][j] from lane k*16+j, D[4*r+l/16][l%16] in result register r.
-__global__ void k_mfma_selftest(const double* A, const double* B, double* D)
-{
-    const int l = threadIdx.x;
-    const double a = A[(l % 16) * 4 + l / 16];
-    const double b = B[(l / 16) * 16 + l % 16];
-    double4_t acc = {0, 0, 0, 0};
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-#pragma unroll
-    for (int r = 0; r < 4; r++) D[(4 * r + l / 16) * 16 + l % 16] = acc[r];
-}
-
-}  // namespace tg
-
-extern "C" int tg_selftest_mfma(int device, double* err)
-{
-    using namespace tg;
-    double A[64], B[64], D[256], R[256];
-    for (int i = 0; i < 64; i++) {
-        A[i] = 1.0 + 0.25 * i;
-        B[i] = 0.5 - 0.125 * i;
-    }
-    for (int i = 0; i < 16; i++)
-        for (int j = 0; j < 16; j++) {
-            double s = 0.0;
-            for (int k = 0; k < 4; k++) s += A[i * 4 + k] * B[k * 16 + j];
-            R[i * 16 + j] = s;
-        }
-    double *dA = nullptr, *dB = nullptr, *dD = nullptr;
-    if (hipSetDevice(device) != hipSuccess) return TG_OUT_OF_RANGE;
-    if (hipMalloc((void**)&dA, sizeof A) != hipSuccess) return TG_OUT_OF_MEMORY;
-    if (hipMalloc((void**)&dB, sizeof B) != hipSuccess) { (void)hipFree(dA); return TG_OUT_OF_MEMORY; }
-    if (hipMalloc((void**)&dD, sizeof D) != hipSuccess) { (void)hipFree(dA); (void)hipFree(dB); return TG_OUT_OF_MEMORY; }
-    int rc = TG_OK;
-    if (hipMemcpy(dA, A, sizeof A, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dB, B, sizeof B, hipMemcpyHostToDevice) != hipSuccess)
-        rc = TG_GENERAL_FAIL;
-    if (rc == TG_OK) {
-        hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, 0, dA, dB, dD);
-        if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(D, dD, sizeof D, hipMemcpyDeviceToHost) != hipSuccess)
-            rc = TG_GENERAL_FAIL;
-    }
-    double e = 0.0;
-    if (rc == TG_OK)
-        for (int i = 0; i < 256; i++) e = std::max(e, std::fabs(D[i] - R[i]));
-    if (err) *err = e;
-    (void)hipFree(dA);
-    (void)hipFree(dB);
-    (void)hipFree(dD);
-    if (rc == TG_OK && e > 1e-12) rc = TG_GENERAL_FAIL;
-    return rc;
-}
 
 // ---------------------------------------------------------------------------
 // Q1P0 mean-dilatation hexahedron (SimoQ1P0, tahoe/src/elements/continuum/solid/SimoQ1P0.cpp)

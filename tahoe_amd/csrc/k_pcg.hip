@@ -1,0 +1,352 @@
+// Segmented Jacobi-preconditioned conjugate gradients.
+//
+// A context holds one problem or a batch of independent problems packed as
+// a disjoint union ("segments"), each starting at a multiple of 64 nodes.
+// Every segment runs its own CG: its own dot products, alpha, beta and
+// stopping test, all kept on the device.  Reductions are per slice of 64
+// nodes (one wave, never shared by two segments) and then per segment in
+// slice order, so a segment's arithmetic -- and its result -- is the same
+// alone and inside any batch.  A single problem is the one-segment case.
+//
+// The reference has no linear CG; this is the textbook method (Saad,
+// Iterative Methods for Sparse Linear Systems, Alg. 9.1) on the reference's
+// building blocks: Multx (MSRMatrixT.cpp:385-420), Jacobi scaling
+// (DiagonalMatrixT.cpp:267-324) and InnerProduct (SolverT.cpp:854-860).
+#include "tg_internal.h"
+
+namespace tg {
+
+constexpr int PB = 256;  // threads (= nodes) per block, 4 slices
+constexpr int FB = 1024; // threads of the per-segment finalize blocks
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ double* sf(double* s, int field, int nseg, int seg) { return s + (long)field * nseg + seg; }
+
+// ---- per-slice partials -> per-segment totals ------------------------------
+
+// Sum of part[k][sb..se) by one block of FB threads: thread i adds slices
+// sb+i, sb+i+FB, ... in order, then a fixed tree (waves by butterfly, then
+// the FB/64 wave sums in order).  The order depends only on the segment's
+// own slice count, never on other segments.  The result is in thread 0.
+template <int NV>
+__device__ __forceinline__ void seg_total(const double* part, long nslices, int sb, int se, double (&tot)[NV])
+{
+    __shared__ double red[NV][FB / 64];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double x = 0.0;
+        for (int s = sb + t; s < se; s += FB) x += part[k * nslices + s];
+        x = wsum(x);
+        if ((t & 63) == 0) red[k][t >> 6] = x;
+    }
+    __syncthreads();
+    if (t == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double x = 0.0;
+            for (int w = 0; w < FB / 64; w++) x += red[k][w];
+            tot[k] = x;
+        }
+}
+
+// x = 0, r = b, p = D^-1 b; per segment rz = b.D^-1 b, bb = b.b
+template <class VT>
+__global__ void __launch_bounds__(PB) k_seg_init(int nnd, const VT* __restrict__ b, const VT* __restrict__ dinv,
+                                                 VT* __restrict__ x, VT* __restrict__ r, VT* __restrict__ p,
+                                                 double* __restrict__ part, long nslices)
+{
+    const int a = blockIdx.x * PB + threadIdx.x;
+    double v0 = 0.0, v1 = 0.0;
+    if (a < nnd) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const long k = 3L * a + i;
+            const VT bi = b[k], zi = dinv[k] * bi;
+            x[k] = VT(0);
+            r[k] = bi;
+            p[k] = zi;
+            v0 += (double)bi * (double)zi;
+            v1 += (double)bi * (double)bi;
+        }
+    }
+    v0 = wsum(v0);
+    v1 = wsum(v1);
+    if ((threadIdx.x & 63) == 0) {
+        const long s = a >> 6;
+        if (s < nslices) {
+            part[s] = v0;
+            part[nslices + s] = v1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(FB) k_seg_fin_init(const double* __restrict__ part, long nslices,
+                                                     const int* __restrict__ seg_slice, const int* __restrict__ active,
+                                                     int nseg, double* __restrict__ sc, double rtol, int maxit,
+                                                     int* __restrict__ ndone)
+{
+    const int g = blockIdx.x;
+    double tot[2];
+    seg_total<2>(part, nslices, seg_slice[g], seg_slice[g + 1], tot);
+    if (threadIdx.x == 0) {
+        *sf(sc, S_RZ, nseg, g) = tot[0];
+        *sf(sc, S_BB, nseg, g) = tot[1];
+        *sf(sc, S_RR, nseg, g) = tot[1];
+        *sf(sc, S_ITER, nseg, g) = 0.0;
+        *sf(sc, S_THRESH, nseg, g) = rtol * rtol * tot[1];
+        *sf(sc, S_MAXIT, nseg, g) = (double)maxit;
+        const bool done = !active[g] || tot[1] == 0.0 || maxit <= 0;
+        *sf(sc, S_DONE, nseg, g) = done ? 1.0 : 0.0;
+        if (done) atomicAdd(ndone, 1);
+    }
+}
+
+// q = A p on the block operator, with per-slice partials of p.q
+template <class VT>
+__global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict__ slice_seg,
+                                                 const int* __restrict__ sbase, const int* __restrict__ swidth,
+                                                 const int* __restrict__ bcol, const VT* __restrict__ bval,
+                                                 const VT* __restrict__ x, VT* __restrict__ y,
+                                                 double* __restrict__ part, long nslices, int nseg,
+                                                 const double* __restrict__ sc)
+{
+    const int a = blockIdx.x * PB + threadIdx.x;
+    const int s = a >> 6, l = a & 63;
+    if (a >= 64L * nslices) return;
+    const int g = slice_seg[s];  // wave-uniform
+    if (*sf((double*)sc, S_DONE, nseg, g) != 0.0) return;
+    double dv = 0.0;
+    if (a < nnd) {
+        const long b0 = sbase[s];
+        const int w = swidth[s];
+        VT y0 = 0, y1 = 0, y2 = 0;
+        for (int k = 0; k < w; k++) {
+            const long blk = b0 + k;
+            const int c = bcol[blk * 64 + l];
+            if (c < 0) continue;
+            const VT x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
+            const VT* v = bval + blk * 9 * 64 + l;
+            y0 += v[0 * 64] * x0 + v[1 * 64] * x1 + v[2 * 64] * x2;
+            y1 += v[3 * 64] * x0 + v[4 * 64] * x1 + v[5 * 64] * x2;
+            y2 += v[6 * 64] * x0 + v[7 * 64] * x1 + v[8 * 64] * x2;
+        }
+        y[3L * a] = y0;
+        y[3L * a + 1] = y1;
+        y[3L * a + 2] = y2;
+        dv = (double)y0 * (double)x[3L * a] + (double)y1 * (double)x[3L * a + 1] + (double)y2 * (double)x[3L * a + 2];
+    }
+    dv = wsum(dv);
+    if (l == 0) part[s] = dv;
+}
+
+__global__ void __launch_bounds__(FB) k_seg_fin_alpha(const double* __restrict__ part, long nslices,
+                                                      const int* __restrict__ seg_slice, int nseg,
+                                                      double* __restrict__ sc)
+{
+    const int g = blockIdx.x;
+    if (*sf(sc, S_DONE, nseg, g) != 0.0) return;
+    double tot[1];
+    seg_total<1>(part, nslices, seg_slice[g], seg_slice[g + 1], tot);
+    if (threadIdx.x == 0) {
+        *sf(sc, S_PQ, nseg, g) = tot[0];
+        *sf(sc, S_ALPHA, nseg, g) = *sf(sc, S_RZ, nseg, g) / tot[0];
+    }
+}
+
+// x += alpha p, r -= alpha q; per-slice partials of r.D^-1 r and r.r
+template <class VT>
+__global__ void __launch_bounds__(PB) k_seg_update(int nnd, const int* __restrict__ slice_seg,
+                                                   const VT* __restrict__ p, const VT* __restrict__ q,
+                                                   const VT* __restrict__ dinv, VT* __restrict__ x,
+                                                   VT* __restrict__ r, double* __restrict__ part, long nslices,
+                                                   int nseg, const double* __restrict__ sc)
+{
+    const int a = blockIdx.x * PB + threadIdx.x;
+    const int s = a >> 6;
+    if (a >= 64L * nslices) return;
+    const int g = slice_seg[s];
+    if (*sf((double*)sc, S_DONE, nseg, g) != 0.0) return;
+    const VT alpha = (VT)*sf((double*)sc, S_ALPHA, nseg, g);
+    double v0 = 0.0, v1 = 0.0;
+    if (a < nnd) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const long k = 3L * a + i;
+            x[k] += alpha * p[k];
+            const VT ri = r[k] - alpha * q[k];
+            r[k] = ri;
+            const double rd = (double)ri;
+            v0 += rd * (double)dinv[k] * rd;
+            v1 += rd * rd;
+        }
+    }
+    v0 = wsum(v0);
+    v1 = wsum(v1);
+    if ((a & 63) == 0) {
+        part[s] = v0;
+        part[nslices + s] = v1;
+    }
+}
+
+// beta = rz'/rz; stop on ||r||^2 <= rtol^2 ||b||^2 or the iteration cap
+__global__ void __launch_bounds__(FB) k_seg_fin_beta(const double* __restrict__ part, long nslices,
+                                                     const int* __restrict__ seg_slice, int nseg,
+                                                     double* __restrict__ sc, int* __restrict__ ndone)
+{
+    const int g = blockIdx.x;
+    if (*sf(sc, S_DONE, nseg, g) != 0.0) return;
+    double tot[2];
+    seg_total<2>(part, nslices, seg_slice[g], seg_slice[g + 1], tot);
+    if (threadIdx.x == 0) {
+        const double it = *sf(sc, S_ITER, nseg, g) + 1.0;
+        *sf(sc, S_ITER, nseg, g) = it;
+        *sf(sc, S_BETA, nseg, g) = tot[0] / *sf(sc, S_RZ, nseg, g);
+        *sf(sc, S_RZ, nseg, g) = tot[0];
+        *sf(sc, S_RR, nseg, g) = tot[1];
+        if (tot[1] <= *sf(sc, S_THRESH, nseg, g) || it >= *sf(sc, S_MAXIT, nseg, g)) {
+            *sf(sc, S_DONE, nseg, g) = 1.0;
+            atomicAdd(ndone, 1);
+        }
+    }
+}
+
+// p = D^-1 r + beta p
+template <class VT>
+__global__ void __launch_bounds__(PB) k_seg_direction(int nnd, const int* __restrict__ slice_seg,
+                                                      const VT* __restrict__ r, const VT* __restrict__ dinv,
+                                                      VT* __restrict__ p, long nslices, int nseg,
+                                                      const double* __restrict__ sc)
+{
+    const int a = blockIdx.x * PB + threadIdx.x;
+    if (a >= nnd) return;
+    const int g = slice_seg[a >> 6];
+    if (*sf((double*)sc, S_DONE, nseg, g) != 0.0) return;
+    const VT beta = (VT)*sf((double*)sc, S_BETA, nseg, g);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const long k = 3L * a + i;
+        p[k] = dinv[k] * r[k] + beta * p[k];
+    }
+}
+
+// per-segment dot product a.b into out[seg]
+__global__ void __launch_bounds__(PB) k_seg_dot(int nnd, const double* __restrict__ x, const double* __restrict__ y,
+                                                double* __restrict__ part, long nslices)
+{
+    const int a = blockIdx.x * PB + threadIdx.x;
+    double v = 0.0;
+    if (a < nnd)
+#pragma unroll
+        for (int i = 0; i < 3; i++) v += x[3L * a + i] * y[3L * a + i];
+    v = wsum(v);
+    if ((threadIdx.x & 63) == 0 && (a >> 6) < nslices) part[a >> 6] = v;
+}
+
+__global__ void __launch_bounds__(FB) k_seg_fin_dot(const double* __restrict__ part, long nslices,
+                                                    const int* __restrict__ seg_slice, double* __restrict__ out)
+{
+    const int g = blockIdx.x;
+    double tot[1];
+    seg_total<1>(part, nslices, seg_slice[g], seg_slice[g + 1], tot);
+    if (threadIdx.x == 0) out[g] = tot[0];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static unsigned node_blocks(const tg_ctx* c) { return (unsigned)((c->nnd + PB - 1) / PB); }
+
+void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out)
+{
+    const long ns = c->seg.nslices;
+    {
+        TimedRegion tr(c, T_VECTOR);
+        hipLaunchKernelGGL(k_seg_dot, dim3(node_blocks(c)), dim3(PB), 0, c->stream, c->nnd, a, b, c->seg.d_part.p, ns);
+        hipLaunchKernelGGL(k_seg_fin_dot, dim3(c->seg.nseg), dim3(FB), 0, c->stream, c->seg.d_part.p, ns,
+                           c->seg.d_seg_slice.p, c->seg.d_out.p);
+        TG_HIP(hipGetLastError());
+    }
+    TG_HIP(hipMemcpyAsync(host_out, c->seg.d_out.p, sizeof(double) * c->seg.nseg, hipMemcpyDeviceToHost, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+}
+
+template <class VT>
+void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r, VT* p, VT* q, double rtol,
+             int maxit, const int* d_active, int nactive, tg_lin_stats* st)
+{
+    Segments& S = c->seg;
+    const long ns = S.nslices;
+    const int nseg = S.nseg;
+    const unsigned nb = node_blocks(c);
+    TG_HIP(hipMemsetAsync(S.d_ndone.p, 0, sizeof(int), c->stream));
+    {
+        TimedRegion tr(c, T_VECTOR);
+        hipLaunchKernelGGL(k_seg_init<VT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, b, dinv, x, r, p, S.d_part.p, ns);
+        hipLaunchKernelGGL(k_seg_fin_init, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns, S.d_seg_slice.p,
+                           d_active, nseg, S.d_scal.p, rtol, maxit, S.d_ndone.p);
+    }
+    const int CHUNK = 16;
+    int issued = 0;
+    for (;;) {
+        for (int k = 0; k < CHUNK; k++) {
+            {
+                TimedRegion tr(c, T_SPMV);
+                hipLaunchKernelGGL(k_seg_spmv<VT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p,
+                                   c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, p, q, S.d_part.p,
+                                   ns, nseg, S.d_scal.p);
+            }
+            {
+                TimedRegion tr(c, T_VECTOR);
+                hipLaunchKernelGGL(k_seg_fin_alpha, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
+                                   S.d_seg_slice.p, nseg, S.d_scal.p);
+                hipLaunchKernelGGL(k_seg_update<VT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, p, q,
+                                   dinv, x, r, S.d_part.p, ns, nseg, S.d_scal.p);
+                hipLaunchKernelGGL(k_seg_fin_beta, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
+                                   S.d_seg_slice.p, nseg, S.d_scal.p, S.d_ndone.p);
+                hipLaunchKernelGGL(k_seg_direction<VT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, r,
+                                   dinv, p, ns, nseg, S.d_scal.p);
+            }
+        }
+        TG_HIP(hipGetLastError());
+        issued += CHUNK;
+        TG_HIP(hipMemcpyAsync(c->hstatus + 2, S.d_ndone.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+        if (c->hstatus[2] >= nseg || issued >= maxit) break;
+    }
+    // per-segment results
+    S.h_scal.resize((size_t)S_NSCAL * nseg);
+    TG_HIP(hipMemcpyAsync(S.h_scal.data(), S.d_scal.p, sizeof(double) * S_NSCAL * nseg, hipMemcpyDeviceToHost,
+                          c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    if (st) {
+        // aggregate: max iterations, worst relative residual, all converged
+        st->iterations = 0;
+        st->rel_residual = 0.0;
+        st->converged = 1;
+        for (int g = 0; g < nseg; g++) {
+            const double it = S.h_scal[(size_t)S_ITER * nseg + g];
+            const double bb = S.h_scal[(size_t)S_BB * nseg + g];
+            const double rr = S.h_scal[(size_t)S_RR * nseg + g];
+            const double th = S.h_scal[(size_t)S_THRESH * nseg + g];
+            st->iterations = std::max(st->iterations, (int)it);
+            if (bb > 0.0) st->rel_residual = std::max(st->rel_residual, sqrt(rr / bb));
+            if (rr > th) st->converged = 0;
+        }
+    }
+    (void)nactive;
+}
+
+template void pcg_run<double>(tg_ctx*, const double*, const double*, const double*, double*, double*, double*,
+                              double*, double, int, const int*, int, tg_lin_stats*);
+template void pcg_run<float>(tg_ctx*, const float*, const float*, const float*, float*, float*, float*, float*,
+                             double, int, const int*, int, tg_lin_stats*);
+
+}  // namespace tg

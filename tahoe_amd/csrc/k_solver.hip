@@ -11,8 +11,6 @@
 
 namespace tg {
 
-// device scalar slots
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_RZNEW, S_RR, S_BETA, S_BB, S_DONE, S_ITER, S_THRESH, S_MAXIT, S_DOT, S_NSCAL };
 
 constexpr int VB = 256;      // threads per block of the vector kernels
 constexpr int VGRID = 1024;  // blocks of the grid-stride vector kernels
@@ -84,11 +82,19 @@ __device__ __forceinline__ void ordered_total(const double* part, int nparts, do
 // ---------------------------------------------------------------------------
 
 template <class VT>
-__global__ void k_expand(long n, const int* __restrict__ src, const VT* __restrict__ val, VT* __restrict__ out)
+__global__ void k_expand(long nb, const int* __restrict__ src, const VT* __restrict__ val, VT* __restrict__ out)
 {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const int s = src[i];
-        out[i] = s >= 0 ? val[s] : VT(0);
+    // one thread per block slot (lane l of slot blk): its 9 components are
+    // read in turn, so the three entries of one MSR row segment come from
+    // the same cache line
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (long)gridDim.x * blockDim.x) {
+        const long blk = i >> 6, l = i & 63;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const long o = (blk * 9 + k) * 64 + l;
+            const int s = src[o];
+            out[o] = s >= 0 ? val[s] : VT(0);
+        }
     }
 }
 
@@ -139,104 +145,6 @@ __global__ void __launch_bounds__(VB) k_spmv(int nnd, const int* __restrict__ sb
                       (double)y2 * (double)x[3L * a + 2];
     }
     if (PCG) block_partials<1>(dotv, part);
-}
-
-// alpha = rz / (p.q)
-__global__ void __launch_bounds__(VB) k_fin_alpha(const double* __restrict__ part, int nparts, double* __restrict__ scal)
-{
-    if (scal[S_DONE] != 0.0) return;
-    double tot[1];
-    ordered_total<1>(part, nparts, tot);
-    if (threadIdx.x == 0) {
-        scal[S_PQ] = tot[0];
-        scal[S_ALPHA] = scal[S_RZ] / tot[0];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// PCG vector steps
-// ---------------------------------------------------------------------------
-
-// x = 0, r = b, p = D^-1 r; rz = r.D^-1 r, bb = b.b
-template <class VT>
-__global__ void __launch_bounds__(VB) k_pcg_init(long n, const VT* __restrict__ b, const VT* __restrict__ dinv,
-                                                 VT* __restrict__ x, VT* __restrict__ r, VT* __restrict__ p,
-                                                 double* __restrict__ part)
-{
-    double v[2] = {0.0, 0.0};
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const VT bi = b[i], zi = dinv[i] * bi;
-        x[i] = VT(0);
-        r[i] = bi;
-        p[i] = zi;
-        v[0] += (double)bi * (double)zi;
-        v[1] += (double)bi * (double)bi;
-    }
-    block_partials<2>(v, part);
-}
-
-__global__ void __launch_bounds__(VB) k_fin_init(const double* __restrict__ part, int nparts,
-                                                 double* __restrict__ scal, double rtol, int maxit)
-{
-    double tot[2];
-    ordered_total<2>(part, nparts, tot);
-    if (threadIdx.x == 0) {
-        scal[S_RZ] = tot[0];
-        scal[S_BB] = tot[1];
-        scal[S_RR] = tot[1];
-        scal[S_ITER] = 0.0;
-        scal[S_THRESH] = rtol * rtol * tot[1];
-        scal[S_MAXIT] = (double)maxit;
-        scal[S_DONE] = (tot[1] == 0.0 || maxit <= 0) ? 1.0 : 0.0;
-    }
-}
-
-// x += alpha p, r -= alpha q; rz' = r.D^-1 r, rr = r.r; beta = rz'/rz
-template <class VT>
-__global__ void __launch_bounds__(VB) k_pcg_update(long n, const VT* __restrict__ p, const VT* __restrict__ q,
-                                                   const VT* __restrict__ dinv, VT* __restrict__ x,
-                                                   VT* __restrict__ r, double* __restrict__ part,
-                                                   const double* __restrict__ scal)
-{
-    if (scal[S_DONE] != 0.0) return;
-    const VT alpha = (VT)scal[S_ALPHA];
-    double v[2] = {0.0, 0.0};
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        x[i] += alpha * p[i];
-        const VT ri = r[i] - alpha * q[i];
-        r[i] = ri;
-        const double rd = (double)ri;
-        v[0] += rd * (double)dinv[i] * rd;
-        v[1] += rd * rd;
-    }
-    block_partials<2>(v, part);
-}
-
-// beta = rz'/rz; convergence test ||r||^2 <= rtol^2 ||b||^2
-__global__ void __launch_bounds__(VB) k_fin_beta(const double* __restrict__ part, int nparts, double* __restrict__ scal)
-{
-    if (scal[S_DONE] != 0.0) return;
-    double tot[2];
-    ordered_total<2>(part, nparts, tot);
-    if (threadIdx.x == 0) {
-        const double it = scal[S_ITER] + 1.0;
-        scal[S_ITER] = it;
-        scal[S_BETA] = tot[0] / scal[S_RZ];
-        scal[S_RZ] = tot[0];
-        scal[S_RR] = tot[1];
-        if (tot[1] <= scal[S_THRESH] || it >= scal[S_MAXIT]) scal[S_DONE] = 1.0;
-    }
-}
-
-// p = D^-1 r + beta p
-template <class VT>
-__global__ void __launch_bounds__(VB) k_pcg_direction(long n, const VT* __restrict__ r, const VT* __restrict__ dinv,
-                                                      VT* __restrict__ p, const double* __restrict__ scal)
-{
-    if (scal[S_DONE] != 0.0) return;
-    const VT beta = (VT)scal[S_BETA];
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
 __global__ void __launch_bounds__(VB) k_dot(long n, const double* __restrict__ a, const double* __restrict__ b,
@@ -306,7 +214,7 @@ static unsigned grid_for(long n)
 void launch_expand(tg_ctx* c)
 {
     TimedRegion tr(c, T_EXPAND);
-    const long n = c->op.nblocks_padded * 9;
+    const long n = c->op.nblocks_padded;
     hipLaunchKernelGGL(k_expand<double>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p, c->dval.p,
                        c->op.d_val.p);
     const long nn = 3L * c->nnd;
@@ -318,7 +226,7 @@ void launch_expand(tg_ctx* c)
 void launch_expand32(tg_ctx* c)
 {
     TimedRegion tr(c, T_EXPAND);
-    const long n = c->op.nblocks_padded * 9;
+    const long n = c->op.nblocks_padded;
     hipLaunchKernelGGL(k_expand<float>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p,
                        c->dval32.p, c->op.d_val32.p);
     const long nn = 3L * c->nnd;
@@ -388,61 +296,10 @@ void launch_apply_kbc(tg_ctx* c, double t)
     TG_HIP(hipGetLastError());
 }
 
-// Jacobi-PCG on the expanded operator in arithmetic VT; b, x in node space.
-// The reference has no linear CG; this is the textbook method (Saad,
-// Iterative Methods for Sparse Linear Systems, Alg. 9.1) on the reference's
-// building blocks: Multx (MSRMatrixT.cpp:385-420), Jacobi scaling
-// (DiagonalMatrixT.cpp:267-324), InnerProduct (SolverT.cpp:854-860).  Stops at
-// ||r|| <= rtol ||b|| (recurrence residual) or maxit.
-template <class VT>
-static void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r, VT* p, VT* q, double rtol,
-                    int maxit, tg_lin_stats* st)
-{
-    const long n = 3L * c->nnd;
-    const unsigned g = grid_for(n);
-    const unsigned sblocks = (unsigned)((c->nnd + VB - 1) / VB);
-    {
-        TimedRegion tr(c, T_VECTOR);
-        hipLaunchKernelGGL(k_pcg_init<VT>, dim3(g), dim3(VB), 0, c->stream, n, b, dinv, x, r, p, c->dpart.p);
-        hipLaunchKernelGGL(k_fin_init, dim3(1), dim3(VB), 0, c->stream, c->dpart.p, (int)g, c->dscal.p, rtol, maxit);
-    }
-    const int CHUNK = 16;
-    int issued = 0;
-    for (;;) {
-        for (int k = 0; k < CHUNK; k++) {
-            {
-                TimedRegion tr(c, T_SPMV);
-                hipLaunchKernelGGL((k_spmv<true, VT>), dim3(sblocks), dim3(VB), 0, c->stream, c->nnd,
-                                   c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, p, q, c->dpart.p,
-                                   c->dscal.p);
-            }
-            {
-                TimedRegion tr(c, T_VECTOR);
-                hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(VB), 0, c->stream, c->dpart.p, (int)sblocks,
-                                   c->dscal.p);
-                hipLaunchKernelGGL(k_pcg_update<VT>, dim3(g), dim3(VB), 0, c->stream, n, p, q, dinv, x, r, c->dpart.p,
-                                   c->dscal.p);
-                hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(VB), 0, c->stream, c->dpart.p, (int)g, c->dscal.p);
-                hipLaunchKernelGGL(k_pcg_direction<VT>, dim3(g), dim3(VB), 0, c->stream, n, r, dinv, p, c->dscal.p);
-            }
-        }
-        TG_HIP(hipGetLastError());
-        issued += CHUNK;
-        TG_HIP(hipMemcpyAsync(c->hscal, c->dscal.p, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, c->stream));
-        TG_HIP(hipStreamSynchronize(c->stream));
-        if (c->hscal[S_DONE] != 0.0 || issued >= maxit) break;
-    }
-    if (st) {
-        st->iterations = (int)c->hscal[S_ITER];
-        const double bb = c->hscal[S_BB];
-        st->rel_residual = bb > 0.0 ? sqrt(c->hscal[S_RR] / bb) : 0.0;
-        st->converged = (c->hscal[S_RR] <= c->hscal[S_THRESH]) ? 1 : 0;
-    }
-}
-
 void pcg_solve(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
 {
-    pcg_run<double>(c, c->op.d_val.p, c->vdinv.p, b, x, c->vr.p, c->vp.p, c->vq.p, rtol, maxit, st);
+    pcg_run<double>(c, c->op.d_val.p, c->vdinv.p, b, x, c->vr.p, c->vp.p, c->vq.p, rtol, maxit, c->seg.d_active.p,
+                    c->seg.nseg, st);
 }
 
 // Mixed precision inner solve: b (FP64) is rounded to FP32, PCG runs in FP32
@@ -454,7 +311,8 @@ void pcg_solve_mixed(tg_ctx* c, const double* b, double* x, double rtol, int max
         TimedRegion tr(c, T_VECTOR);
         hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(VB), 0, c->stream, n, b, c->fb.p);
     }
-    pcg_run<float>(c, c->op.d_val32.p, c->fdinv.p, c->fb.p, c->fx.p, c->fr.p, c->fp.p, c->fq.p, rtol, maxit, st);
+    pcg_run<float>(c, c->op.d_val32.p, c->fdinv.p, c->fb.p, c->fx.p, c->fr.p, c->fp.p, c->fq.p, rtol, maxit,
+                   c->seg.d_active.p, c->seg.nseg, st);
     {
         TimedRegion tr(c, T_VECTOR);
         hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(VB), 0, c->stream, n, c->fx.p, x);
@@ -463,3 +321,47 @@ void pcg_solve_mixed(tg_ctx* c, const double* b, double* x, double rtol, int max
 }
 
 }  // namespace tg
+
+namespace tg {
+
+__global__ void __launch_bounds__(256) k_copy_calib(long n2, const double2* __restrict__ src, double2* __restrict__ dst)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace tg
+
+extern "C" int tg_calib_copy(int device, long bytes, int reps, double* ms)
+{
+    using namespace tg;
+    if (bytes < 16 || reps < 1 || !ms) return TG_BAD_INPUT;
+    if (hipSetDevice(device) != hipSuccess) return TG_OUT_OF_RANGE;
+    const long n2 = bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void**)&a, n2 * 16) != hipSuccess) return TG_OUT_OF_MEMORY;
+    if (hipMalloc((void**)&b, n2 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return TG_OUT_OF_MEMORY;
+    }
+    int rc = TG_OK;
+    hipEvent_t e0, e1;
+    if (hipMemset(a, 0, n2 * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = TG_GENERAL_FAIL;
+    if (rc == TG_OK) {
+        const unsigned g = 4096;
+        hipLaunchKernelGGL(k_copy_calib, dim3(g), dim3(256), 0, 0, n2, a, b);  // warm-up
+        (void)hipEventRecord(e0, 0);
+        for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_calib, dim3(g), dim3(256), 0, 0, n2, a, b);
+        (void)hipEventRecord(e1, 0);
+        float t = 0.0f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+            rc = TG_GENERAL_FAIL;
+        *ms = (double)t / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}

@@ -90,6 +90,46 @@ static void require_init(const tg_ctx* c)
     if (!c->initialized) fail(TG_GENERAL_FAIL, "tg_initialize has not been called");
 }
 
+// Segment tables: slice -> segment, segment -> first slice, element -> segment.
+static void setup_segments(tg_ctx* c)
+{
+    Segments& S = c->seg;
+    if (S.node_begin.size() != (size_t)S.nseg + 1 || S.node_begin.back() != c->nnd) {
+        S.nseg = 1;
+        S.node_begin = {0, c->nnd};
+    }
+    S.nslices = (c->nnd + 63) / 64;
+    std::vector<int> slice_seg((size_t)S.nslices), seg_slice((size_t)S.nseg + 1);
+    std::vector<int> node_seg((size_t)c->nnd);
+    for (int g = 0; g < S.nseg; g++) {
+        seg_slice[g] = S.node_begin[g] / 64;
+        for (int a = S.node_begin[g]; a < S.node_begin[g + 1]; a++) node_seg[a] = g;
+    }
+    seg_slice[S.nseg] = (int)S.nslices;
+    for (long s = 0; s < S.nslices; s++) slice_seg[s] = node_seg[64 * s];
+    S.elem_seg.resize((size_t)c->nel);
+    for (int e = 0; e < c->nel; e++) {
+        const int g = node_seg[c->conn[8L * e]];
+        for (int a = 1; a < 8; a++)
+            if (node_seg[c->conn[8L * e + a]] != g) fail(TG_BAD_INPUT, "element connects two segments");
+        S.elem_seg[e] = g;
+    }
+    S.d_slice_seg.alloc((size_t)S.nslices);
+    S.d_seg_slice.alloc((size_t)S.nseg + 1);
+    S.d_active.alloc((size_t)S.nseg);
+    S.d_elem_seg.alloc((size_t)c->nel);
+    S.d_scal.alloc((size_t)S_NSCAL * S.nseg);
+    S.d_part.alloc(2 * (size_t)S.nslices);
+    S.d_out.alloc((size_t)S.nseg);
+    S.d_ndone.alloc(1);
+    std::vector<int> ones((size_t)S.nseg, 1);
+    TG_HIP(hipMemcpy(S.d_slice_seg.p, slice_seg.data(), sizeof(int) * slice_seg.size(), hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpy(S.d_seg_slice.p, seg_slice.data(), sizeof(int) * seg_slice.size(), hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpy(S.d_active.p, ones.data(), sizeof(int) * ones.size(), hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpy(S.d_elem_seg.p, S.elem_seg.data(), sizeof(int) * S.elem_seg.size(), hipMemcpyHostToDevice));
+    TG_HIP(hipMemset(S.d_scal.p, 0, sizeof(double) * S_NSCAL * S.nseg));
+}
+
 static void check_status(tg_ctx* c)
 {
     TG_HIP(hipMemcpyAsync(c->hstatus, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -195,6 +235,8 @@ int tg_set_mesh(tg_ctx* c, int nnd, int nel, const double* X, const int* conn, c
         c->nnd = nnd;
         c->nel = nel;
         c->X.assign(X, X + 3L * nnd);
+        c->seg.nseg = 1;
+        c->seg.node_begin = {0, nnd};
         c->conn.assign(conn, conn + 8L * nel);
         c->mat.assign((size_t)nel, 0);
         if (mat)
@@ -242,6 +284,22 @@ int tg_set_precision(tg_ctx* c, int prec)
     });
 }
 
+int tg_set_segments(tg_ctx* c, int nseg, const int* nb)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (nseg < 1 || !nb) fail(TG_BAD_INPUT, "bad segment list");
+        if (nb[0] != 0 || nb[nseg] != c->nnd) fail(TG_BAD_INPUT, "segments must cover the nodes");
+        for (int g = 1; g <= nseg; g++) {
+            if (nb[g] <= nb[g - 1]) fail(TG_BAD_INPUT, "segment node ranges must increase");
+            if (g < nseg && nb[g] % 64 != 0) fail(TG_BAD_INPUT, "segment begins must be multiples of 64 nodes");
+        }
+        c->seg.node_begin.assign(nb, nb + nseg + 1);
+        c->seg.nseg = nseg;
+        c->initialized = false;
+    });
+}
+
 int tg_set_kbc(tg_ctx* c, int nkbc, const int* nodes, const int* dofs, const double* values)
 {
     if (!c) return TG_BAD_INPUT;
@@ -274,6 +332,7 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         msr_pattern(c->n_eq, c->nnd, c->nel, c->conn.data(), c->eqnos.data(), c->bindx);
         c->msr_len = (long)c->bindx.size();
         c->ncolors = color_elements(c->nnd, c->nel, c->conn.data(), c->color_ptr, c->color_elems);
+        c->stiff_chunk_built = -2;
         // device storage
         const size_t nn = 3 * (size_t)c->nnd;
         c->dX.alloc(nn);
@@ -318,6 +377,7 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         launch_build_slots(c);
         build_bsell(c);
         c->op.d_val32.alloc(mixed ? (size_t)c->op.nblocks_padded * 9 : 0);
+        setup_segments(c);
         TG_HIP(hipStreamSynchronize(c->stream));
         c->initialized = true;
         if (n_eq) *n_eq = c->n_eq;
@@ -587,6 +647,72 @@ int tg_newton(tg_ctx* c, const tg_newton_opts* o, tg_newton_stats* s)
     return rc;
 }
 
+// Per-segment NLSolver::Solve: the tests of ExitIteration (NLSolver.cpp:675-757)
+// applied to each segment's own ||R||; segments that converge or fail leave
+// the active set (their PCG does nothing, so their u stays put).
+int tg_newton_batch(tg_ctx* c, const tg_newton_opts* o, tg_newton_stats* stats)
+{
+    if (!c || !o || !stats) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        const int G = c->seg.nseg;
+        std::vector<double> e0(G), e(G);
+        std::vector<int> state(G, 0), iter(G, -1);  // 0 active, 1 converged, 2 failed
+        std::vector<long> lin(G, 0);
+        launch_residual(c, c->du.p, c->dR.p);
+        check_status(c);
+        segment_dots(c, c->dR.p, c->dR.p, e0.data());
+        for (int g = 0; g < G; g++) {
+            e0[g] = std::sqrt(e0[g]);
+            e[g] = e0[g];
+            if (e0[g] < o->abs_tolerance) state[g] = 1;
+        }
+        std::vector<int> active(G);
+        for (;;) {
+            int nact = 0;
+            for (int g = 0; g < G; g++) {
+                active[g] = state[g] == 0;
+                nact += active[g];
+            }
+            if (nact == 0) break;
+            TG_HIP(hipMemcpyAsync(c->seg.d_active.p, active.data(), sizeof(int) * G, hipMemcpyHostToDevice, c->stream));
+            launch_stiffness(c, c->du.p);
+            prepare_operator(c);
+            tg_lin_stats st{};
+            solve_linear(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
+            check_status(c);
+            for (int g = 0; g < G; g++)
+                if (active[g]) lin[g] += (long)c->seg.h_scal[(size_t)S_ITER * G + g];
+            launch_axpy(c, c->du.p, c->vx.p, 1.0, 3L * c->nnd);
+            launch_residual(c, c->du.p, c->dR.p);
+            check_status(c);
+            segment_dots(c, c->dR.p, c->dR.p, e.data());
+            for (int g = 0; g < G; g++) {
+                e[g] = std::sqrt(e[g]);
+                if (!active[g]) continue;
+                iter[g]++;
+                const double rel = e[g] / e0[g];
+                if (rel > o->divergence_tolerance)
+                    state[g] = 2;
+                else if (rel < o->rel_tolerance || e[g] < o->abs_tolerance)
+                    state[g] = 1;
+                else if (iter[g] >= o->max_iterations)
+                    state[g] = 2;
+            }
+        }
+        std::vector<int> ones(G, 1);
+        TG_HIP(hipMemcpyAsync(c->seg.d_active.p, ones.data(), sizeof(int) * G, hipMemcpyHostToDevice, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+        for (int g = 0; g < G; g++) {
+            stats[g].status = state[g] == 1 ? 0 : 1;
+            stats[g].iterations = iter[g] + 1;
+            stats[g].error0 = e0[g];
+            stats[g].error = e[g];
+            stats[g].lin_iterations = lin[g];
+        }
+    });
+}
+
 int tg_synchronize(tg_ctx* c)
 {
     if (!c) return TG_BAD_INPUT;
@@ -617,6 +743,25 @@ int tg_timing_get(tg_ctx* c, tg_timing* out)
     return guarded(c, [&] {
         drain_events(c);
         *out = c->timing.acc;
+    });
+}
+
+int tg_set_tuning(tg_ctx* c, int knob, long value)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (knob != TG_TUNE_STIFF_CHUNK) fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
+        if (value < -1) fail(TG_BAD_INPUT, "stiffness chunk must be -1, 0 or positive");
+        c->stiff_chunk = value;
+    });
+}
+
+int tg_get_tuning(tg_ctx* c, int knob, long* value)
+{
+    if (!c || !value) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (knob != TG_TUNE_STIFF_CHUNK) fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
+        *value = c->stiff_chunk;
     });
 }
 

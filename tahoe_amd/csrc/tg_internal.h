@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -65,6 +67,28 @@ struct Timing {
     tg_timing acc{};
 };
 
+// device scalar slots of an iteration (per segment for the PCG)
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_RZNEW, S_RR, S_BETA, S_BB, S_DONE, S_ITER, S_THRESH, S_MAXIT, S_DOT, S_NSCAL };
+
+// Independent problems packed in one context: segment g owns nodes
+// [node_begin[g], node_begin[g+1]), each starting at a multiple of 64, so a
+// 64-node slice (one wave) never spans two segments.
+struct Segments {
+    int nseg = 1;
+    long nslices = 0;
+    std::vector<int> node_begin;          // [nseg+1]
+    std::vector<int> elem_seg;            // [nel] segment of each element (host)
+    DBuf<int> d_slice_seg;                // [nslices]
+    DBuf<int> d_seg_slice;                // [nseg+1] first slice of each segment
+    DBuf<int> d_active;                   // [nseg] 1 = solve this segment
+    DBuf<int> d_elem_seg;                 // [nel]
+    DBuf<double> d_scal;                  // [S_NSCAL][nseg]
+    DBuf<double> d_part;                  // [2][nslices]
+    DBuf<double> d_out;                   // [nseg]
+    DBuf<int> d_ndone;
+    std::vector<double> h_scal;
+};
+
 }  // namespace tg
 
 struct tg_ctx {
@@ -90,12 +114,19 @@ struct tg_ctx {
     long msr_len = 0;
     std::vector<int> color_ptr, color_elems;
     int ncolors = 0;
+    // stiffness launch order: every segment's elements in ranges of
+    // stiff_chunk (by rank inside the segment), all colours of range j
+    // before range j+1 (0: whole colours)
+    long stiff_chunk = -1;              // -1: chosen from the matrix footprint
+    long stiff_chunk_built = -2;
+    std::vector<std::pair<int, int>> stiff_launches;  // (first, count) into dstiff_elems
 
     // device storage
     tg::DBuf<double> dX, du, dR;        // node space [nnd*3]
     tg::DBuf<double> du_last;           // last converged displacement (CloseStep/ResetStep)
     tg::DBuf<int> dconn, dmat, deqn;    // deqn: 0-based equation or -1
     tg::DBuf<int> dcolor_elems;
+    tg::DBuf<int> dstiff_elems;         // elements in stiffness launch order
     tg::DBuf<double2> dmatprops;        // (mu, kappa) per material
     tg::DBuf<int> dkbc_idx;             // 3*node+dof per card
     tg::DBuf<double> dkbc_val;
@@ -119,6 +150,7 @@ struct tg_ctx {
     tg::DBuf<double> dvol, dvol_last;
 
     tg::Timing timing;
+    tg::Segments seg;
 };
 
 namespace tg {
@@ -140,6 +172,10 @@ void widen_values(tg_ctx* c);
 void pcg_solve_mixed(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
 double device_dot(tg_ctx* c, const double* a, const double* b, long n);
 void pcg_solve(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
+template <class VT>
+void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r, VT* p, VT* q, double rtol,
+             int maxit, const int* d_active, int nactive, tg_lin_stats* st);
+void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out);
 void eq_to_node(tg_ctx* c, const double* src_eq, double* dst_node);
 void node_to_eq(tg_ctx* c, const double* src_node, double* dst_eq);
 void launch_axpy(tg_ctx* c, double* y, const double* x, double a, long n);

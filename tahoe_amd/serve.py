@@ -1,0 +1,180 @@
+"""Continuous-batching server for RVE solve requests on one GPU.
+
+Requests (tahoe_amd.rve.Request) arrive over time.  A request is admitted
+when the device-memory footprint of everything admitted stays under a
+budget; admitted requests of one formulation are grouped into a mini-batch
+(at most `max_batch` requests, or whatever has arrived when a worker is
+free) that is packed into one device context and solved to Newton
+convergence by a worker thread on that context's own HIP stream.  Up to
+`workers` mini-batches are in flight at once, so requests that arrive while
+others run are admitted as memory and workers free up and their work
+overlaps on the device.  Each request's latency is completion time minus
+arrival time.
+
+Results are deterministic per request: a sub-problem's arithmetic does not
+depend on the batch it lands in (segmented reductions, per-problem Newton
+control), so the same request gives the same displacement alone or in any
+mix.
+"""
+from __future__ import annotations
+
+import threading
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import api, batch
+
+
+def footprint_bytes(req, precision=api.MIXED) -> int:
+    """Device bytes of one request inside a batch context (upper bound):
+    per element the 300-entry slot map and colouring; per node the MSR
+    pattern/values (<= 41 upper entries), the 3x3-block operator (27 blocks
+    with map, FP64 and FP32 values) and the node vectors."""
+    nel = req.L ** 3
+    nnd = (req.L + 1) ** 3 + 63
+    per_el = 300 * 4 + 8 * 4 + 8
+    per_node = 3 * 41 * (4 + 8 + (4 if precision == api.MIXED else 0)) + 27 * 64 // 64 * (4 + 9 * 4 + 9 * 8 + 9 * 4) \
+        + 3 * 8 * 12 + 3 * 4 * 8
+    return nel * per_el + nnd * per_node
+
+
+@dataclass
+class Result:
+    rid: int
+    u: np.ndarray
+    newton_iterations: int
+    arrival: float
+    admitted: float
+    finished: float
+    batch_id: int
+    batch_size: int
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+@dataclass
+class ServerStats:
+    results: dict = field(default_factory=dict)
+    batches: list = field(default_factory=list)
+    wall: float = 0.0
+
+
+class RVEServer:
+    def __init__(self, precision=api.MIXED, workers=4, max_batch=256, budget_bytes=64 << 30, lin_rtol=None,
+                 device=0, keep_u=True):
+        self.precision = precision
+        self.workers = workers
+        self.max_batch = max_batch
+        self.budget = budget_bytes
+        self.device = device
+        self.keep_u = keep_u
+        rt = lin_rtol if lin_rtol is not None else (1e-6 if precision == api.MIXED else 1e-12)
+        self.opts = api.Solid.opts(lin_rtol=rt)
+        self._lock = threading.Condition()
+        self._queue = deque()        # arrived, not yet admitted
+        self._in_use = 0
+        self._next_batch = 0
+        self._closed = False
+        self.stats = ServerStats()
+
+    # -- admission ------------------------------------------------------
+    def _take_batch(self):
+        """Called with the lock held: pop a mini-batch of one formulation
+        whose footprint fits the remaining budget; None if nothing fits."""
+        if not self._queue:
+            return None
+        head = self._queue[0]
+        form = head.formulation
+        picked, size = [], 0
+        for r in list(self._queue):
+            if r.formulation != form:
+                continue
+            fb = footprint_bytes(r, self.precision)
+            if self._in_use + size + fb > self.budget and picked:
+                break
+            if self._in_use + size + fb > self.budget and self._in_use > 0:
+                break  # wait for memory to free up
+            picked.append(r)
+            size += fb
+            if len(picked) >= self.max_batch:
+                break
+        if not picked:
+            return None
+        for r in picked:
+            self._queue.remove(r)
+        self._in_use += size
+        bid = self._next_batch
+        self._next_batch += 1
+        return bid, picked, size
+
+    # -- worker ---------------------------------------------------------
+    def _solve(self, bid, reqs, t0):
+        t_adm = time.perf_counter() - t0
+        pk = batch.pack([r.problem() for r in reqs])
+        s = batch.make_solid(pk, formulation=reqs[0].formulation, precision=self.precision, device=self.device)
+        us, st = batch.static_solve_batch(s, pk, 1, self.opts)
+        s.close()
+        t_end = time.perf_counter() - t0
+        out = []
+        for i, r in enumerate(reqs):
+            out.append(Result(r.rid, us[i] if self.keep_u else None, st[0][i].iterations, r.arrival, t_adm, t_end,
+                              bid, len(reqs)))
+        return out
+
+    def _worker(self, t0):
+        while True:
+            with self._lock:
+                job = None
+                while job is None:
+                    job = self._take_batch()
+                    if job is None:
+                        if self._closed and not self._queue:
+                            return
+                        self._lock.wait(timeout=0.01)
+            bid, reqs, size = job
+            try:
+                res = self._solve(bid, reqs, t0)
+            finally:
+                with self._lock:
+                    self._in_use -= size
+                    self._lock.notify_all()
+            with self._lock:
+                for x in res:
+                    self.stats.results[x.rid] = x
+                self.stats.batches.append((bid, len(reqs), reqs[0].formulation, res[0].admitted, res[0].finished))
+
+    # -- driver ---------------------------------------------------------
+    def run(self, requests, realtime=True):
+        """Replay `requests` (sorted by arrival, seconds from start) and serve
+        them; returns ServerStats with per-request results."""
+        t0 = time.perf_counter()
+        threads = [threading.Thread(target=self._worker, args=(t0,), daemon=True) for _ in range(self.workers)]
+        for th in threads:
+            th.start()
+        for r in sorted(requests, key=lambda x: x.arrival):
+            if realtime:
+                dt = r.arrival - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+            with self._lock:
+                self._queue.append(r)
+                self._lock.notify_all()
+        with self._lock:
+            self._closed = True
+            self._lock.notify_all()
+        for th in threads:
+            th.join()
+        self.stats.wall = time.perf_counter() - t0
+        return self.stats
+
+
+def latency_summary(stats: ServerStats) -> dict:
+    lat = np.array([r.latency for r in stats.results.values()])
+    return {"requests": int(len(lat)), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "solves_per_s": len(lat) / stats.wall, "wall_s": stats.wall,
+            "batches": len(stats.batches)}

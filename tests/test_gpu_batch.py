@@ -1,0 +1,63 @@
+"""Batched independent problems in one context (segments): every problem's
+result equals its reference fixture (converged u within 1e-9, same Newton
+iteration counts) and is bitwise identical alone and inside the batch."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def golden_problem(name):
+    from tahoe_amd import batch, mesh as M
+    g = load_golden(name)
+    m = M.HexMesh(g["X"], g["conn"], g["mat"], {}, None)
+    k = M.KBC(g["kbc_nodes"], g["kbc_dofs"], g["kbc_values"])
+    return g, batch.Problem(m, k, list(g["E"]), list(g["nu"]))
+
+
+def rve_problem(n, seed):
+    from tahoe_amd import batch, mesh as M
+    m, H = M.rve(n, seed)
+    return batch.Problem(m, M.kbc_affine(m, H), [1000.0, 10000.0], [0.3, 0.3])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_matches_fixtures_and_alone(precision):
+    from tahoe_amd import api, batch
+    names = ["tl_n2", "rve_n3", "rve_n4"]
+    gs, probs = zip(*[golden_problem(n) for n in names])
+    probs = list(probs) + [rve_problem(5, 77), rve_problem(6, 78)]
+    opts = api.Solid.opts(lin_rtol=1e-12 if precision == 0 else 1e-6)
+    pk = batch.pack(probs)
+    s = batch.make_solid(pk, precision=precision)
+    us, stats = batch.static_solve_batch(s, pk, 2, opts)
+    for i, g in enumerate(gs):
+        ref = g["step_u"][-1]
+        err = np.abs(us[i] - ref).max() / np.abs(ref).max()
+        assert err <= 1e-9, (names[i], err)
+        if precision == 0:
+            assert [st[i].iterations for st in stats] == g["step_iters"].tolist()
+    for i, p in enumerate(probs):
+        pk1 = batch.pack([p])
+        s1 = batch.make_solid(pk1, precision=precision)
+        u1, st1 = batch.static_solve_batch(s1, pk1, 2, opts)
+        assert np.array_equal(u1[0], us[i]), i
+        assert [st[0].iterations for st in st1] == [st[i].iterations for st in stats]
+
+
+def test_single_segment_batch_equals_newton():
+    """A one-problem batch runs the same arithmetic as tg_newton."""
+    from tahoe_amd import api, batch
+    g, p = golden_problem("tl_n4")
+    pk = batch.pack([p])
+    s = batch.make_solid(pk)
+    s.apply_kbc(0.2)
+    st = s.newton_batch()
+    ub = s.displacement()
+    s2 = batch.make_solid(pk)
+    s2.apply_kbc(0.2)
+    st2 = s2.newton()
+    assert np.array_equal(ub, s2.displacement())
+    assert st[0].iterations == st2.iterations

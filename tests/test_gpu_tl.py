@@ -30,12 +30,6 @@ def make_solid(g, formulation=None):
     return s
 
 
-def test_mfma_layout_selftest():
-    from tahoe_amd import api
-    rc, err = api.selftest_mfma(0)
-    assert rc == 0 and err == 0.0
-
-
 TL_CASES = ["tl_n2", "tl_n4", "rve_n3", "rve_n4"]
 
 
