@@ -12,6 +12,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtahoe_gpu.so")
+HOST = os.path.join(HERE, "host")
+HOST_LIB = os.path.join(HERE, "libtahoe_host.so")
+HOST_EXE = os.path.join(HERE, "bin", "tahoe_hip")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+# C++ host layer (plugin-shaped classes + XML/geometry front end) over the C ABI
+HOST_SOURCES = ["TotalLagrangianHIPT.cpp", "HIPMSRMatrixT.cpp", "NLSolverHIP.cpp", "InputDeck.cpp",
+                "FEManagerHIP.cpp"]
 
 SOURCES = ["tg_setup.cpp", "tg_api.cpp", "k_element.hip", "k_solver.hip", "k_pcg.hip"]
 HEADERS = ["tg_internal.h", "tg_kinematics.h", "../../include/tahoe_gpu.h"]
@@ -58,7 +65,35 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_host(verbose)
     return LIB
+
+
+def build_host(verbose: bool = False) -> str:
+    """libtahoe_host.so and the tahoe_hip driver, plain g++ against the C ABI."""
+    cxx = os.environ.get("CXX", "g++")
+    os.makedirs(os.path.dirname(HOST_EXE), exist_ok=True)
+    common = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-I", INCLUDE, "-I", HOST]
+    tmp = HOST_LIB + ".tmp"
+    cmd = [cxx] + common + ["-fPIC", "-shared", "-o", tmp] + [os.path.join(HOST, f) for f in HOST_SOURCES] + \
+        ["-L", HERE, "-ltahoe_gpu", "-Wl,-rpath,$ORIGIN"]
+    for c in (cmd,):
+        if verbose:
+            print(" ".join(c), flush=True)
+        r = subprocess.run(c, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("host layer build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, HOST_LIB)
+    tmp = HOST_EXE + ".tmp"
+    cmd = [cxx] + common + ["-o", tmp, os.path.join(HOST, "tahoe_hip.cpp"), "-L", HERE, "-ltahoe_host",
+                            "-ltahoe_gpu", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("tahoe_hip build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, HOST_EXE)
+    return HOST_EXE
 
 
 if __name__ == "__main__":

@@ -1,0 +1,63 @@
+"""The C++ host layer without a GPU: the tahoe_hip driver reads the
+reference's own input deck and our fixture decks, and rejects what is not on
+the path with the reference's ExceptionT codes (6 bad input value, 9
+database read failure)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from host_decks import DECKS, EXE, run, write_fixture_deck
+
+
+def test_driver_built():
+    assert os.access(EXE, os.X_OK), "build() makes tahoe_amd/bin/tahoe_hip"
+
+
+def test_reference_q1p0_deck_parses():
+    r = run(["-f", os.path.join(DECKS, "Q1P0.3D.b.xml"), "-check"])
+    assert r.returncode == 0, r.stderr
+    # 18 nodes, 4 hexes; 16 mapped nodes x 3 dofs + 2 nodes with dof 3 fixed
+    assert "updated_lagrangian_Q1P0, 18 nodes, 4 elements, 1 materials, 50 kinematic cards (48 scheduled), 2 steps" \
+        in r.stdout
+
+
+def test_fixture_deck_parses(tmp_path):
+    g = load_golden("tl_n16")
+    xml = write_fixture_deck(g, str(tmp_path))
+    r = run(["-f", xml, "-check"])
+    assert r.returncode == 0, r.stderr
+    nk = len(g["kbc_nodes"])
+    nsched = int(np.count_nonzero(g["kbc_values"]))
+    assert f"total_lagrangian, 4913 nodes, 4096 elements, 1 materials, {nk} kinematic cards ({nsched} scheduled), " \
+           "5 steps" in r.stdout
+
+
+@pytest.mark.parametrize("edit,code,text", [
+    (lambda s: s.replace("total_lagrangian", "updated_lagrangian"), 6, "not on the path"),
+    (lambda s: s.replace("Simo_isotropic", "large_strain_StVenant"), 6, "Simo_isotropic"),
+    (lambda s: s.replace('<kinematic_BC dof="1"', '<force_BC dof="1"', 1), 6, "force_BC"),
+    (lambda s: s.replace("</tahoe>", ""), 6, "missing </tahoe>"),
+    (lambda s: s.replace('geometry_file="deck.geom"', 'geometry_file="none.geom"'), 9, "cannot open"),
+])
+def test_deck_errors(tmp_path, edit, code, text):
+    g = load_golden("tl_n2")
+    xml = write_fixture_deck(g, str(tmp_path))
+    s = open(xml).read()
+    s2 = edit(s)
+    assert s2 != s
+    open(xml, "w").write(s2)
+    r = run(["-f", xml, "-check"])
+    assert r.returncode == code, (r.returncode, r.stdout, r.stderr)
+    assert text in r.stderr
+
+
+def test_geometry_errors(tmp_path):
+    g = load_golden("tl_n2")
+    xml = write_fixture_deck(g, str(tmp_path))
+    geom = os.path.join(str(tmp_path), "deck.geom")
+    s = open(geom).read()
+    open(geom, "w").write(s.replace("*nodes\n27\n", "*nodes\n26\n"))
+    r = run(["-f", xml, "-check"])
+    assert r.returncode == 9 and "*nodes header" in r.stderr, r.stderr
