@@ -50,6 +50,16 @@ def parse_args():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: 2 Newton corrections with 48 PCG iterations each")
+    ap.add_argument("--workload", choices=["cube", "mixed-steps", "rve-batch", "rve-serve"], default="cube",
+                    help="cube: one Newton correction on the n^3 cube (default); mixed-steps: the 20%%/10-step "
+                         "load path in mixed precision vs FP64, error per step; rve-batch: 4096 RVEs batched; "
+                         "rve-serve: Poisson stream of RVE requests")
+    ap.add_argument("--rve-count", type=int, default=4096)
+    ap.add_argument("--plain-sample", type=int, default=256,
+                    help="RVEs of the batch solved one at a time in plain FP64 (rate baseline + error check)")
+    ap.add_argument("--batch-elements", type=int, default=1 << 21, help="elements per packed batch")
+    ap.add_argument("--serve-count", type=int, default=512)
+    ap.add_argument("--serve-load", type=float, default=0.5, help="offered load as a fraction of capacity")
     ap.add_argument("--stiff-chunk", type=int, default=None,
                     help="elements per range of the colour-ordered stiffness launches (default: autotuned)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -102,6 +112,12 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU layout is "
               "described in DESIGN.md and is not measured here.", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "mixed-steps":
+        return bench_mixed_steps(a)
+    if a.workload == "rve-batch":
+        return bench_rve_batch(a)
+    if a.workload == "rve-serve":
+        return bench_rve_serve(a)
     from tahoe_amd import api, mesh as M
 
     n = a.n
@@ -210,6 +226,194 @@ def main():
         "timing_ms": tm,
         "autotune_stiffness_ms": {str(k): v for k, v in tuned["ms"].items()},
     }
+    print(json.dumps(line), flush=True)
+
+
+def base_line(value, unit, steps, warmup, ms_per_step, dtype, config, higher=True):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms_per_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+            "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def bench_mixed_steps(a):
+    """The 20% stretch in 10 load steps (static Newton to 1e-10 relative
+    residual every step), solved in FP64 and in mixed precision on the same
+    mesh; relative displacement error of mixed vs FP64 after every step."""
+    from tahoe_amd import api, mesh as M
+    n = a.n if a.n != 128 else 24
+    m = M.cube(n, seed=2024)
+    kbc = M.kbc_uniaxial(m, 0.2)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        s = api.Solid(0)
+        s.set_mesh(m.X, m.conn, m.mat)
+        s.set_material(0, *api.mu_kappa(1000.0, 0.3))
+        s.set_precision(api.MIXED if prec == "mixed" else api.FP64)
+        s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
+        n_eq, _ = s.initialize()
+        opts = s.opts(lin_rtol=1e-10 if prec == "fp64" else 1e-6)
+        s.timing_enable(True)
+        t0 = time.perf_counter()
+        us, its = api.static_solve(s, 10, opts)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        res[prec] = {"u": us, "newton": its, "wall_s": wall, "timing": s.timing()}
+        print(f"# {prec}: {wall:.2f} s, Newton per step {its}", file=sys.stderr, flush=True)
+    err = [float(np.abs(um - ud).max() / np.abs(ud).max()) for um, ud in zip(res["mixed"]["u"], res["fp64"]["u"])]
+    tot = lambda r: r["timing"]["residual_ms"] + r["timing"]["stiffness_ms"]
+    line = base_line(m.nel * sum(res["mixed"]["newton"]) / (tot(res["mixed"]) * 1e-3), "elements/s", 10, 0,
+                     res["mixed"]["wall_s"] * 1e3 / 10, "f32/f64",
+                     {"workload": f"{n}^3 Q1 hex TL neo-Hookean, +-5% perturbed, uniaxial 20% in 10 load steps, "
+                                  "Newton to 1e-10; mixed: FP32 K + FP32 Jacobi-PCG to 1e-6 inside FP64 "
+                                  "residual/Newton; FP64: Jacobi-PCG to 1e-10",
+                      "elements": m.nel, "equations": n_eq})
+    line.update({"rel_err_vs_fp64_per_step": err, "max_rel_err": max(err),
+                 "newton_per_step": {"mixed": res["mixed"]["newton"], "fp64": res["fp64"]["newton"]},
+                 "wall_s": {"mixed": res["mixed"]["wall_s"], "fp64": res["fp64"]["wall_s"]},
+                 "speedup_wall": res["fp64"]["wall_s"] / res["mixed"]["wall_s"],
+                 "timing_ms": {k: res[k]["timing"] for k in res}})
+    print(json.dumps(line), flush=True)
+
+
+def solve_alone(req, precision, opts):
+    """One request in its own context (the plain one-problem-at-a-time path)."""
+    from tahoe_amd import batch
+    pk = batch.pack([req.problem()])
+    t0 = time.perf_counter()
+    s = batch.make_solid(pk, formulation=req.formulation, precision=precision)
+    t1 = time.perf_counter()
+    us, st = batch.static_solve_batch(s, pk, 1, opts)
+    s.synchronize()
+    t2 = time.perf_counter()
+    s.close()
+    return us[0], st[0][0], t2 - t0, t2 - t1
+
+
+def size_batches(reqs, max_el):
+    """Requests sorted by size, cut into batches of at most max_el elements
+    (similar sizes together, so a batch's PCG runs about as long as its
+    members need)."""
+    out, cur, nel = [], [], 0
+    for r in sorted(reqs, key=lambda r: (r.L, r.rid)):
+        if cur and nel + r.elements > max_el:
+            out.append(cur)
+            cur, nel = [], 0
+        cur.append(r)
+        nel += r.elements
+    if cur:
+        out.append(cur)
+    return out
+
+
+def bench_rve_batch(a):
+    """4096 two-phase RVE cubes (edge log-uniform 4-24), affine boundary
+    displacement, each solved to Newton convergence (rel 1e-10): batched
+    mixed precision vs the plain FP64 path (one problem per context, one at a
+    time) on a sample of the same requests."""
+    from tahoe_amd import api, batch, rve
+    reqs = rve.batch_requests(a.rve_count)
+    tot_el = sum(r.elements for r in reqs)
+    fp64 = api.Solid.opts(lin_rtol=1e-10)
+    mixed = api.Solid.opts(lin_rtol=1e-6)
+    # plain FP64, one at a time, on every k-th request
+    k = max(1, len(reqs) // max(1, a.plain_sample))
+    sample = reqs[::k][:a.plain_sample]
+    solve_alone(sample[0], api.FP64, fp64)  # warm-up (library load, first context)
+    plain_u, plain_wall, plain_solve, plain_el = {}, 0.0, 0.0, 0
+    for r in sample:
+        u, st, wall, solve = solve_alone(r, api.FP64, fp64)
+        assert st.status == 0
+        plain_u[r.rid] = u
+        plain_wall += wall
+        plain_solve += solve
+        plain_el += r.elements
+    print(f"# plain FP64: {len(sample)} RVEs, {plain_el} elements in {plain_wall:.2f} s", file=sys.stderr,
+          flush=True)
+    # batched mixed precision over every request
+    batches = size_batches(reqs, a.batch_elements)
+    t0 = time.perf_counter()
+    solve_s = 0.0
+    errs = []
+    newton_max = 0
+    for b in batches:
+        pk = batch.pack([r.problem() for r in b])
+        s = batch.make_solid(pk, formulation=api.TOTAL_LAGRANGIAN, precision=api.MIXED)
+        t1 = time.perf_counter()
+        us, st = batch.static_solve_batch(s, pk, 1, mixed)
+        s.synchronize()
+        solve_s += time.perf_counter() - t1
+        s.close()
+        newton_max = max(newton_max, max(x.iterations for x in st[0]))
+        for r, u in zip(b, us):
+            if r.rid in plain_u:
+                ref = plain_u[r.rid]
+                errs.append(float(np.abs(u - ref).max() / max(np.abs(ref).max(), 1e-300)))
+        print(f"# batch of {len(b)} (L {b[0].L}-{b[-1].L}): {time.perf_counter() - t0:.2f} s so far",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    value = tot_el / wall
+    plain_rate = plain_el / plain_wall
+    line = base_line(value, "elements/s", len(batches), 1, wall * 1e3 / len(batches), "f32/f64",
+                     {"workload": f"{len(reqs)} two-phase RVE cubes, edge log-uniform 4-24 (seed 4242), "
+                                  "Bernoulli(0.3) phase 2 at 10x stiffness, affine u = H X, |H_ij| <= 0.02, "
+                                  "Newton to 1e-10 per RVE; batched mixed precision (FP32 K + FP32 PCG to 1e-6)",
+                      "rves": len(reqs), "elements": tot_el, "batches": len(batches),
+                      "batch_elements": a.batch_elements})
+    line.update({
+        "rves_per_s": len(reqs) / wall,
+        "solve_only_elements_per_s": tot_el / solve_s,
+        "plain_fp64": {"rves": len(sample), "elements": plain_el, "wall_s": plain_wall,
+                       "elements_per_s": plain_rate, "solve_only_elements_per_s": plain_el / plain_solve,
+                       "note": "every k-th request, one context per RVE, FP64 K + Jacobi-PCG to 1e-10"},
+        "speedup_vs_plain_fp64": value / plain_rate,
+        "max_rel_err_vs_fp64": max(errs) if errs else None, "compared_rves": len(errs),
+        "newton_max": newton_max,
+    })
+    print(json.dumps(line), flush=True)
+
+
+def bench_rve_serve(a):
+    """Served stream: Poisson arrivals of RVE requests (edge log-uniform
+    4-32, half Q1 nu=0.3, half Q1P0 nu=0.499), continuous batching with HBM
+    admission; p50/p99 latency and solves/s, next to the plain FP64 path
+    (one request at a time, in arrival order) on the same stream."""
+    from tahoe_amd import api, rve, serve
+    # capacity: everything arriving at once
+    cal = rve.stream_requests(min(128, a.serve_count), rate=1e9, seed=555)
+    srv = serve.RVEServer(precision=api.MIXED, keep_u=False)
+    st = srv.run(cal, realtime=False)
+    cap = len(cal) / st.wall
+    rate = a.serve_load * cap
+    print(f"# capacity {cap:.1f} solves/s -> offered {rate:.1f}/s", file=sys.stderr, flush=True)
+    reqs = rve.stream_requests(a.serve_count, rate=rate)
+    srv = serve.RVEServer(precision=api.MIXED)
+    st = srv.run(reqs, realtime=True)
+    lat = serve.latency_summary(st)
+    # plain FP64 on the same arrivals: one request at a time
+    fp64 = api.Solid.opts(lin_rtol=1e-10)
+    n_plain = min(len(reqs), a.plain_sample)
+    t0 = time.perf_counter()
+    clock, plain_lat, errs = 0.0, [], []
+    for r in reqs[:n_plain]:
+        u, stn, wall, _ = solve_alone(r, api.FP64, fp64)
+        clock = max(clock, r.arrival) + wall
+        plain_lat.append(clock - r.arrival)
+        ref = st.results[r.rid].u
+        errs.append(float(np.abs(ref - u).max() / max(np.abs(u).max(), 1e-300)))
+    plain_wall = time.perf_counter() - t0
+    pl = np.array(plain_lat)
+    els = sum(r.elements for r in reqs)
+    line = base_line(els / st.wall, "elements/s", len(reqs), 0, st.wall * 1e3 / len(reqs), "f32/f64",
+                     {"workload": f"{len(reqs)} RVE requests, Poisson arrivals at {rate:.1f}/s "
+                                  f"({a.serve_load:g} of measured capacity), edge log-uniform 4-32, 50% Q1 nu=0.3 "
+                                  "(mixed precision) / 50% Q1P0 nu=0.499 (FP64), Newton to 1e-10",
+                      "requests": len(reqs), "elements": els})
+    line.update({"latency": lat, "capacity_solves_per_s": cap,
+                 "plain_fp64": {"requests": n_plain, "p50_s": float(np.percentile(pl, 50)),
+                                "p99_s": float(np.percentile(pl, 99)), "wall_s": plain_wall,
+                                "solves_per_s": n_plain / plain_wall,
+                                "note": "same arrival times, one request at a time, FP64 throughout"},
+                 "max_rel_err_vs_fp64": max(errs)})
     print(json.dumps(line), flush=True)
 
 

@@ -41,9 +41,10 @@ enum { TG_TOTAL_LAGRANGIAN = 0, TG_Q1P0 = 1 };
 
 /* arithmetic of the stiffness and the inner linear solve:
  * TG_FP64  everything in FP64;
- * TG_MIXED element stiffness and Jacobi-PCG in FP32 (values, vectors and the
- *          SpMV), dot products accumulated in FP64; residual, Newton test and
- *          update stay FP64, so Newton acts as the iterative refinement. */
+ * TG_MIXED element stiffness assembled in FP32 and the PCG operator stored
+ *          in FP32 (half the SpMV bytes); Krylov vectors, SpMV sums and dot
+ *          products FP64; residual, Newton test and update FP64, so Newton
+ *          acts as the iterative refinement of the FP32 operator. */
 enum { TG_FP64 = 0, TG_MIXED = 1 };
 
 typedef struct tg_ctx tg_ctx;

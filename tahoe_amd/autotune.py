@@ -20,7 +20,7 @@ from . import api
 
 def stiffness_chunk_candidates(nel: int) -> list:
     out = [0]
-    c = 16384
+    c = 65536
     while c < nel:
         out.append(c)
         c *= 2

@@ -328,8 +328,8 @@ static const std::vector<std::pair<int, int>>& stiffness_launches(tg_ctx* c)
     long chunk = c->stiff_chunk;
     if (chunk < 0) {
         const double vb = (c->precision == TG_MIXED ? 4.0 : 8.0) * (double)c->msr_len / std::max(1, c->nel);
-        chunk = (long)(64.0 * (1 << 20) / std::max(vb, 1.0));
-        chunk = std::max(chunk, 32768L);
+        chunk = (long)(256.0 * (1 << 20) / std::max(vb, 1.0));
+        chunk = std::max(chunk, 131072L);
     }
     if (chunk == 0 || chunk >= c->nel) chunk = c->nel;
     if (c->stiff_chunk_built == chunk) return c->stiff_launches;

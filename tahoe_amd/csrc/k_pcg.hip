@@ -57,9 +57,9 @@ __device__ __forceinline__ void seg_total(const double* part, long nslices, int 
 }
 
 // x = 0, r = b, p = D^-1 b; per segment rz = b.D^-1 b, bb = b.b
-template <class VT>
-__global__ void __launch_bounds__(PB) k_seg_init(int nnd, const VT* __restrict__ b, const VT* __restrict__ dinv,
-                                                 VT* __restrict__ x, VT* __restrict__ r, VT* __restrict__ p,
+__global__ void __launch_bounds__(PB) k_seg_init(int nnd, const double* __restrict__ b,
+                                                 const double* __restrict__ dinv, double* __restrict__ x,
+                                                 double* __restrict__ r, double* __restrict__ p,
                                                  double* __restrict__ part, long nslices)
 {
     const int a = blockIdx.x * PB + threadIdx.x;
@@ -68,12 +68,12 @@ __global__ void __launch_bounds__(PB) k_seg_init(int nnd, const VT* __restrict__
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const long k = 3L * a + i;
-            const VT bi = b[k], zi = dinv[k] * bi;
-            x[k] = VT(0);
+            const double bi = b[k], zi = dinv[k] * bi;
+            x[k] = 0.0;
             r[k] = bi;
             p[k] = zi;
-            v0 += (double)bi * (double)zi;
-            v1 += (double)bi * (double)bi;
+            v0 += bi * zi;
+            v1 += bi * bi;
         }
     }
     v0 = wsum(v0);
@@ -108,12 +108,14 @@ __global__ void __launch_bounds__(FB) k_seg_fin_init(const double* __restrict__ 
     }
 }
 
-// q = A p on the block operator, with per-slice partials of p.q
-template <class VT>
+// q = A p on the block operator, with per-slice partials of p.q.  MT is the
+// storage type of the operator values (double, or float in mixed
+// precision); products and sums are FP64.
+template <class MT>
 __global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict__ slice_seg,
                                                  const int* __restrict__ sbase, const int* __restrict__ swidth,
-                                                 const int* __restrict__ bcol, const VT* __restrict__ bval,
-                                                 const VT* __restrict__ x, VT* __restrict__ y,
+                                                 const int* __restrict__ bcol, const MT* __restrict__ bval,
+                                                 const double* __restrict__ x, double* __restrict__ y,
                                                  double* __restrict__ part, long nslices, int nseg,
                                                  const double* __restrict__ sc)
 {
@@ -126,21 +128,21 @@ __global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict_
     if (a < nnd) {
         const long b0 = sbase[s];
         const int w = swidth[s];
-        VT y0 = 0, y1 = 0, y2 = 0;
+        double y0 = 0, y1 = 0, y2 = 0;
         for (int k = 0; k < w; k++) {
             const long blk = b0 + k;
             const int c = bcol[blk * 64 + l];
             if (c < 0) continue;
-            const VT x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
-            const VT* v = bval + blk * 9 * 64 + l;
-            y0 += v[0 * 64] * x0 + v[1 * 64] * x1 + v[2 * 64] * x2;
-            y1 += v[3 * 64] * x0 + v[4 * 64] * x1 + v[5 * 64] * x2;
-            y2 += v[6 * 64] * x0 + v[7 * 64] * x1 + v[8 * 64] * x2;
+            const double x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
+            const MT* v = bval + blk * 9 * 64 + l;
+            y0 += (double)v[0 * 64] * x0 + (double)v[1 * 64] * x1 + (double)v[2 * 64] * x2;
+            y1 += (double)v[3 * 64] * x0 + (double)v[4 * 64] * x1 + (double)v[5 * 64] * x2;
+            y2 += (double)v[6 * 64] * x0 + (double)v[7 * 64] * x1 + (double)v[8 * 64] * x2;
         }
         y[3L * a] = y0;
         y[3L * a + 1] = y1;
         y[3L * a + 2] = y2;
-        dv = (double)y0 * (double)x[3L * a] + (double)y1 * (double)x[3L * a + 1] + (double)y2 * (double)x[3L * a + 2];
+        dv = y0 * x[3L * a] + y1 * x[3L * a + 1] + y2 * x[3L * a + 2];
     }
     dv = wsum(dv);
     if (l == 0) part[s] = dv;
@@ -161,11 +163,10 @@ __global__ void __launch_bounds__(FB) k_seg_fin_alpha(const double* __restrict__
 }
 
 // x += alpha p, r -= alpha q; per-slice partials of r.D^-1 r and r.r
-template <class VT>
 __global__ void __launch_bounds__(PB) k_seg_update(int nnd, const int* __restrict__ slice_seg,
-                                                   const VT* __restrict__ p, const VT* __restrict__ q,
-                                                   const VT* __restrict__ dinv, VT* __restrict__ x,
-                                                   VT* __restrict__ r, double* __restrict__ part, long nslices,
+                                                   const double* __restrict__ p, const double* __restrict__ q,
+                                                   const double* __restrict__ dinv, double* __restrict__ x,
+                                                   double* __restrict__ r, double* __restrict__ part, long nslices,
                                                    int nseg, const double* __restrict__ sc)
 {
     const int a = blockIdx.x * PB + threadIdx.x;
@@ -173,18 +174,17 @@ __global__ void __launch_bounds__(PB) k_seg_update(int nnd, const int* __restric
     if (a >= 64L * nslices) return;
     const int g = slice_seg[s];
     if (*sf((double*)sc, S_DONE, nseg, g) != 0.0) return;
-    const VT alpha = (VT)*sf((double*)sc, S_ALPHA, nseg, g);
+    const double alpha = *sf((double*)sc, S_ALPHA, nseg, g);
     double v0 = 0.0, v1 = 0.0;
     if (a < nnd) {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const long k = 3L * a + i;
             x[k] += alpha * p[k];
-            const VT ri = r[k] - alpha * q[k];
+            const double ri = r[k] - alpha * q[k];
             r[k] = ri;
-            const double rd = (double)ri;
-            v0 += rd * (double)dinv[k] * rd;
-            v1 += rd * rd;
+            v0 += ri * dinv[k] * ri;
+            v1 += ri * ri;
         }
     }
     v0 = wsum(v0);
@@ -218,17 +218,16 @@ __global__ void __launch_bounds__(FB) k_seg_fin_beta(const double* __restrict__ 
 }
 
 // p = D^-1 r + beta p
-template <class VT>
 __global__ void __launch_bounds__(PB) k_seg_direction(int nnd, const int* __restrict__ slice_seg,
-                                                      const VT* __restrict__ r, const VT* __restrict__ dinv,
-                                                      VT* __restrict__ p, long nslices, int nseg,
+                                                      const double* __restrict__ r, const double* __restrict__ dinv,
+                                                      double* __restrict__ p, long nslices, int nseg,
                                                       const double* __restrict__ sc)
 {
     const int a = blockIdx.x * PB + threadIdx.x;
     if (a >= nnd) return;
     const int g = slice_seg[a >> 6];
     if (*sf((double*)sc, S_DONE, nseg, g) != 0.0) return;
-    const VT beta = (VT)*sf((double*)sc, S_BETA, nseg, g);
+    const double beta = *sf((double*)sc, S_BETA, nseg, g);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const long k = 3L * a + i;
@@ -278,9 +277,9 @@ void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out)
     TG_HIP(hipStreamSynchronize(c->stream));
 }
 
-template <class VT>
-void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r, VT* p, VT* q, double rtol,
-             int maxit, const int* d_active, int nactive, tg_lin_stats* st)
+template <class MT>
+void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, double* x, double* r, double* p,
+             double* q, double rtol, int maxit, const int* d_active, int nactive, tg_lin_stats* st)
 {
     Segments& S = c->seg;
     const long ns = S.nslices;
@@ -289,7 +288,7 @@ void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r
     TG_HIP(hipMemsetAsync(S.d_ndone.p, 0, sizeof(int), c->stream));
     {
         TimedRegion tr(c, T_VECTOR);
-        hipLaunchKernelGGL(k_seg_init<VT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, b, dinv, x, r, p, S.d_part.p, ns);
+        hipLaunchKernelGGL(k_seg_init, dim3(nb), dim3(PB), 0, c->stream, c->nnd, b, dinv, x, r, p, S.d_part.p, ns);
         hipLaunchKernelGGL(k_seg_fin_init, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns, S.d_seg_slice.p,
                            d_active, nseg, S.d_scal.p, rtol, maxit, S.d_ndone.p);
     }
@@ -299,7 +298,7 @@ void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r
         for (int k = 0; k < CHUNK; k++) {
             {
                 TimedRegion tr(c, T_SPMV);
-                hipLaunchKernelGGL(k_seg_spmv<VT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p,
+                hipLaunchKernelGGL(k_seg_spmv<MT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p,
                                    c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, p, q, S.d_part.p,
                                    ns, nseg, S.d_scal.p);
             }
@@ -307,11 +306,11 @@ void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r
                 TimedRegion tr(c, T_VECTOR);
                 hipLaunchKernelGGL(k_seg_fin_alpha, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
                                    S.d_seg_slice.p, nseg, S.d_scal.p);
-                hipLaunchKernelGGL(k_seg_update<VT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, p, q,
+                hipLaunchKernelGGL(k_seg_update, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, p, q,
                                    dinv, x, r, S.d_part.p, ns, nseg, S.d_scal.p);
                 hipLaunchKernelGGL(k_seg_fin_beta, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
                                    S.d_seg_slice.p, nseg, S.d_scal.p, S.d_ndone.p);
-                hipLaunchKernelGGL(k_seg_direction<VT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, r,
+                hipLaunchKernelGGL(k_seg_direction, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, r,
                                    dinv, p, ns, nseg, S.d_scal.p);
             }
         }
@@ -346,7 +345,7 @@ void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r
 
 template void pcg_run<double>(tg_ctx*, const double*, const double*, const double*, double*, double*, double*,
                               double*, double, int, const int*, int, tg_lin_stats*);
-template void pcg_run<float>(tg_ctx*, const float*, const float*, const float*, float*, float*, float*, float*,
-                             double, int, const int*, int, tg_lin_stats*);
+template void pcg_run<float>(tg_ctx*, const float*, const double*, const double*, double*, double*, double*,
+                             double*, double, int, const int*, int, tg_lin_stats*);
 
 }  // namespace tg

@@ -101,12 +101,12 @@ __global__ void k_expand(long nb, const int* __restrict__ src, const VT* __restr
 // inverse diagonal in node space (Jacobi; DiagonalMatrixT::Factorize/BackSubstitute,
 // DiagonalMatrixT.cpp:267-324)
 template <class VT>
-__global__ void k_dinv(long n, const int* __restrict__ eqn, const VT* __restrict__ val, VT* __restrict__ dinv)
+__global__ void k_dinv(long n, const int* __restrict__ eqn, const VT* __restrict__ val, double* __restrict__ dinv)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const int e = eqn[i];
-        const VT d = e >= 0 ? val[e] : VT(0);
-        dinv[i] = d != VT(0) ? VT(1) / d : VT(0);
+        const double d = e >= 0 ? (double)val[e] : 0.0;
+        dinv[i] = d != 0.0 ? 1.0 / d : 0.0;
     }
 }
 
@@ -231,7 +231,7 @@ void launch_expand32(tg_ctx* c)
                        c->dval32.p, c->op.d_val32.p);
     const long nn = 3L * c->nnd;
     hipLaunchKernelGGL(k_dinv<float>, dim3(grid_for(nn)), dim3(VB), 0, c->stream, nn, c->deqn.p, c->dval32.p,
-                       c->fdinv.p);
+                       c->vdinv.p);
     TG_HIP(hipGetLastError());
 }
 
@@ -302,22 +302,14 @@ void pcg_solve(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg
                     c->seg.nseg, st);
 }
 
-// Mixed precision inner solve: b (FP64) is rounded to FP32, PCG runs in FP32
-// with FP64 dot products, the correction is widened back to FP64.
+// Mixed precision inner solve: the operator values are the FP32 stiffness;
+// the Krylov vectors, products and dot products stay FP64.  (CG with FP32
+// vectors stalls on these systems: at 128^3 the condition number is ~1e7,
+// the order of 1/eps_32, and the FP32 recurrence loses the residual.)
 void pcg_solve_mixed(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
 {
-    const long n = 3L * c->nnd;
-    {
-        TimedRegion tr(c, T_VECTOR);
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(VB), 0, c->stream, n, b, c->fb.p);
-    }
-    pcg_run<float>(c, c->op.d_val32.p, c->fdinv.p, c->fb.p, c->fx.p, c->fr.p, c->fp.p, c->fq.p, rtol, maxit,
-                   c->seg.d_active.p, c->seg.nseg, st);
-    {
-        TimedRegion tr(c, T_VECTOR);
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(VB), 0, c->stream, n, c->fx.p, x);
-    }
-    TG_HIP(hipGetLastError());
+    pcg_run<float>(c, c->op.d_val32.p, c->vdinv.p, b, x, c->vr.p, c->vp.p, c->vq.p, rtol, maxit, c->seg.d_active.p,
+                   c->seg.nseg, st);
 }
 
 }  // namespace tg

@@ -350,7 +350,6 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         for (auto* v : {&c->vx, &c->vr, &c->vz, &c->vp, &c->vq, &c->vdinv, &c->vwork}) v->alloc(nn);
         const bool mixed = c->precision == TG_MIXED;
         c->dval32.alloc(mixed ? (size_t)c->msr_len : 0);
-        for (auto* v : {&c->fx, &c->fr, &c->fp, &c->fq, &c->fdinv, &c->fb}) v->alloc(mixed ? nn : 0);
         const size_t sblocks = ((size_t)c->nnd + 255) / 256;
         c->dpart.alloc(2 * std::max<size_t>(sblocks, 1024) + 16);
         std::vector<int> eqn0(nn);

@@ -139,7 +139,6 @@ struct tg_ctx {
 
     // PCG work vectors in node space
     tg::DBuf<double> vx, vr, vz, vp, vq, vdinv, vwork;
-    tg::DBuf<float> fx, fr, fp, fq, fdinv, fb;   // mixed precision inner PCG vectors
     tg::DBuf<double> dpart;             // partial reduction sums
     tg::DBuf<double> dscal;             // device scalars
     tg::DBuf<int> dstatus;              // device error flag (7 = bad Jacobian)
@@ -172,9 +171,9 @@ void widen_values(tg_ctx* c);
 void pcg_solve_mixed(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
 double device_dot(tg_ctx* c, const double* a, const double* b, long n);
 void pcg_solve(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
-template <class VT>
-void pcg_run(tg_ctx* c, const VT* val, const VT* dinv, const VT* b, VT* x, VT* r, VT* p, VT* q, double rtol,
-             int maxit, const int* d_active, int nactive, tg_lin_stats* st);
+template <class MT>
+void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, double* x, double* r, double* p,
+             double* q, double rtol, int maxit, const int* d_active, int nactive, tg_lin_stats* st);
 void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out);
 void eq_to_node(tg_ctx* c, const double* src_eq, double* dst_node);
 void node_to_eq(tg_ctx* c, const double* src_node, double* dst_eq);
