@@ -62,6 +62,8 @@ def parse_args():
     ap.add_argument("--serve-load", type=float, default=0.5, help="offered load as a fraction of capacity")
     ap.add_argument("--stiff-chunk", type=int, default=None,
                     help="elements per range of the colour-ordered stiffness launches (default: autotuned)")
+    ap.add_argument("--stiff-shape", type=int, default=None,
+                    help="stiffness workgroup shape 0..5 (default: autotuned)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the plain copy kernel (1 GiB) used to calibrate HBM counters")
     return ap.parse_args()
@@ -133,13 +135,15 @@ def main():
     n_eq, msr_len = s.initialize()
     print(f"# setup {time.perf_counter() - t_setup:.1f} s: nel={m.nel} nnd={m.nnd} n_eq={n_eq} "
           f"msr_upper_nnz={msr_len - 1}", file=sys.stderr, flush=True)
-    if a.stiff_chunk is not None:
-        s.set_tuning(api.TUNE_STIFF_CHUNK, a.stiff_chunk)
-        tuned = {"best": a.stiff_chunk, "ms": {}}
+    if a.stiff_chunk is not None or a.stiff_shape is not None:
+        s.set_tuning(api.TUNE_STIFF_CHUNK, -1 if a.stiff_chunk is None else a.stiff_chunk)
+        s.set_tuning(api.TUNE_STIFF_SHAPE, 0 if a.stiff_shape is None else a.stiff_shape)
+        tuned = {"shape": s.tuning(api.TUNE_STIFF_SHAPE), "chunk": s.tuning(api.TUNE_STIFF_CHUNK), "ms": {}}
     else:
         from tahoe_amd import autotune
-        tuned = autotune.tune_stiffness_chunk(s, reps=1)
-    print(f"# stiffness chunk {tuned['best']} (autotune ms: {tuned['ms']})", file=sys.stderr, flush=True)
+        tuned = autotune.tune_stiffness(s, reps=1)
+    print(f"# stiffness shape {tuned['shape']} chunk {tuned['chunk']} (autotune ms: {tuned['ms']})",
+          file=sys.stderr, flush=True)
     opts = s.opts(lin_rtol=a.lin_rtol, lin_maxit=48 if a.profile else 200000)
     t_load = 0.2
 
@@ -209,7 +213,8 @@ def main():
                                "uniaxial 5% load step 1/5, one Newton correction: R + K into MSR + Jacobi-PCG "
                                f"to {a.lin_rtol:g}",
                    "elements": m.nel, "nodes": m.nnd, "equations": n_eq, "msr_upper_nnz": msr_len - 1,
-                   "stiffness_chunk": tuned["best"]},
+                   "stiffness_shape_elements_per_workgroup": api.STIFF_SHAPE_EPB[tuned["shape"]],
+                   "stiffness_chunk": tuned["chunk"]},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": spmv_gbs / (HBM_PEAK / 1e9), "traffic": traffic_from_profiles(),
                      "kernel": "k_spmv<true> (block sliced-ELL SpMV + p.q)",
@@ -224,7 +229,7 @@ def main():
                 "dof_iterations_per_s": n_eq * its / (pcg_ms * 1e-3) if pcg_ms > 0 else None,
                 "expand_ms": tm["expand_ms"] / K},
         "timing_ms": tm,
-        "autotune_stiffness_ms": {str(k): v for k, v in tuned["ms"].items()},
+        "autotune_stiffness_ms": {k: {str(x): y for x, y in v.items()} for k, v in tuned["ms"].items()},
     }
     print(json.dumps(line), flush=True)
 

@@ -182,8 +182,20 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  * autotuner in tahoe_amd/autotune.py or by hand):
  * TG_TUNE_STIFF_CHUNK  elements per range of the colour-ordered stiffness
  *                      launches (0: one launch per colour, -1: derived from
- *                      the matrix footprint). */
-enum { TG_TUNE_STIFF_CHUNK = 1 };
+ *                      the matrix footprint);
+ * TG_TUNE_STIFF_SHAPE  block shape of the stiffness kernel: 0..3 for 16,
+ *                      12, 8 or 4 elements per workgroup with element-major
+ *                      lanes, 4..5 for 16 or 8 with block-major lanes;
+ * TG_TUNE_NEWTON_ASSEMBLY  where tg_newton / tg_newton_batch /
+ *                      tg_newton_correction put the tangent: 1 (default)
+ *                      straight into the inner solve's block operator, 0
+ *                      into the MSR and then expanded.  The operator is the
+ *                      same either way (same sums in the same order); with
+ *                      1 the MSR queries (tg_get_msr_values, tg_multx,
+ *                      tg_copy_diagonal) see the matrix of the last
+ *                      tg_form_lhs, as GlobalMatrixT's do between
+ *                      assemblies. */
+enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3 };
 int tg_set_tuning(tg_ctx* ctx, int knob, long value);
 int tg_get_tuning(tg_ctx* ctx, int knob, long* value);
 

@@ -15,7 +15,9 @@ LIB_PATH = os.path.join(HERE, "libtahoe_gpu.so")
 
 TOTAL_LAGRANGIAN, Q1P0 = 0, 1
 FP64, MIXED = 0, 1
-TUNE_STIFF_CHUNK = 1
+TUNE_STIFF_CHUNK, TUNE_STIFF_SHAPE, TUNE_NEWTON_ASSEMBLY = 1, 2, 3
+STIFF_SHAPE_EPB = (16, 12, 8, 4, 16, 8)
+STIFF_SHAPE_BLOCK_MAJOR = (False, False, False, False, True, True)
 
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)

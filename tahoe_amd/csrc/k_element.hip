@@ -153,14 +153,22 @@ __device__ __forceinline__ int upper_index(int m, int n) { return m * 24 - (m * 
 // D_q = s_q c_q and T_q = s_q sigma_q (s_q = det0 w J, Voigt 11,22,33,23,13,12):
 //   K(ai,bj) = sum_q sum_kl g_q[k][a] g_q[l][b] ( D_q[V(i,k)][V(j,l)] + delta_ij T_q[V(k,l)] )
 // i.e. B^T D B plus Expand(gradN^T T gradN, 3), accumulated over integration
-// points in order.  A block of 576 lanes owns 16 elements: 8 lanes per
+// points in order.  A block of 36*EPB lanes owns EPB elements: 8 lanes per
 // element first evaluate the integration points into LDS, then 36 lanes per
 // element form one 3x3 block each (no divergence, 18 accumulators) and
 // scatter its upper entries through the slot map.  FP64 MFMA peaks at the
 // FP64 VALU rate on gfx950 while the dense 24x48x24 tile product does about
 // twice the arithmetic of this sparse form, so the FP64 path uses the VALU.
-constexpr int SK_EPB = 16;  // elements per block
-constexpr int SK_THREADS = 36 * SK_EPB;
+//
+// The block shape trades lane utilisation against occupancy: at ~120 VGPRs a
+// SIMD holds 4 waves, so a 16-element block (9 waves) runs alone on its CU
+// while 8-element blocks (5 waves) run three at a time and hide each other's
+// barrier and scatter waits.  EPB (elements per block) and the phase-2 lane
+// order are template parameters; the autotuner picks among the compiled
+// shapes.
+constexpr int STIFF_EPB[] = {16, 12, 8, 4, 16, 8};
+constexpr bool STIFF_PM[] = {false, false, false, false, true, true};
+constexpr int N_STIFF_SHAPES = sizeof(STIFF_EPB) / sizeof(STIFF_EPB[0]);
 
 // Voigt index of (i,k): (11,22,33,23,13,12)
 __device__ __forceinline__ constexpr int voigt(int i, int k) { return i == k ? i : 6 - i - k; }
@@ -174,24 +182,38 @@ __device__ __forceinline__ constexpr int sym6(int v, int w)
 // VT is the arithmetic of the 3x3 block products and of the assembled
 // values: double (FP64 path) or float (mixed path, integration-point
 // kinematics and material stay FP64 and are rounded once to FP32).
-template <class VT>
-__global__ void __launch_bounds__(SK_THREADS) k_stiffness_tl(const int* __restrict__ elems, int count,
+// TO_OP = false: upper entries into the MSR values through the slot map
+// (MSRMatrixT::Assemble).  TO_OP = true: the block and its transpose
+// straight into the solver's block operator through the operator slot map,
+// prescribed rows and columns left zero -- the same sums in the same order
+// as expanding the assembled MSR, without the MSR round trip.
+// PM selects the lane order of phase 2: false = element-major (lane
+// le*36 + p), true = block-major (lane p*EPB + le), where one wave holds the
+// same block p of many elements -- their scatter targets are neighbouring
+// rows -- and the LDS tables are element-minor so those lanes read
+// consecutive words.
+template <class VT, int EPB, bool TO_OP, bool PM>
+__global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict__ elems, int count,
                                                              const int* __restrict__ conn,
                                                              const int* __restrict__ emat,
                                                              const double2* __restrict__ props,
                                                              const double* __restrict__ X,
                                                              const double* __restrict__ U,
-                                                             const int* __restrict__ slot, VT* __restrict__ val,
+                                                             const int* __restrict__ slot,
+                                                             const int* __restrict__ eqn, VT* __restrict__ val,
                                                              int* __restrict__ status)
 {
-    __shared__ double sxu[SK_EPB][8][6];
-    __shared__ VT sg[SK_EPB][8][3][8];  // dN/dx [element][ip][dir][node]
-    __shared__ VT sd[SK_EPB][8][21];    // s c, packed symmetric Voigt
-    __shared__ VT st[SK_EPB][8][6];     // s sigma, Voigt
+    __shared__ double sxu[EPB][8][6];
+    __shared__ VT sg[EPB * 8 * 24];  // dN/dx per (element, ip, dir, node)
+    __shared__ VT sd[EPB * 8 * 21];  // s c, packed symmetric Voigt
+    __shared__ VT st[EPB * 8 * 6];   // s sigma, Voigt
+    auto gi = [](int le, int q, int j, int a) { return PM ? ((q * 3 + j) * 8 + a) * EPB + le : ((le * 8 + q) * 3 + j) * 8 + a; };
+    auto di = [](int le, int q, int k) { return PM ? (q * 21 + k) * EPB + le : (le * 8 + q) * 21 + k; };
+    auto ti = [](int le, int q, int k) { return PM ? (q * 6 + k) * EPB + le : (le * 8 + q) * 6 + k; };
     const int t = threadIdx.x;
-    const long base = (long)blockIdx.x * SK_EPB;
+    const long base = (long)blockIdx.x * EPB;
     bool bad = false;
-    if (t < 8 * SK_EPB) {
+    if (t < 8 * EPB) {
         const int le = t >> 3, q = t & 7;
         if (base + le < count) {
             const int node = conn[8L * elems[base + le] + q];
@@ -204,7 +226,7 @@ __global__ void __launch_bounds__(SK_THREADS) k_stiffness_tl(const int* __restri
     }
     __syncthreads();
     // ---- phase 1: integration-point data ----
-    if (t < 8 * SK_EPB) {
+    if (t < 8 * EPB) {
         const int le = t >> 3, q = t & 7;
         const bool valid = base + le < count;
         double scale = 0.0, Jb = 1.0, bb[6] = {1, 1, 1, 0, 0, 0}, sig[6] = {0, 0, 0, 0, 0, 0};
@@ -241,19 +263,19 @@ __global__ void __launch_bounds__(SK_THREADS) k_stiffness_tl(const int* __restri
 #pragma unroll
         for (int a = 0; a < 8; a++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) sg[le][q][j][a] = (VT)g[j][a];
+            for (int j = 0; j < 3; j++) sg[gi(le, q, j, a)] = (VT)g[j][a];
         double cm[6][6];
         simo_moduli(mk.x, mk.y, Jb, bb, cm);
 #pragma unroll
         for (int v = 0; v < 6; v++)
 #pragma unroll
-            for (int w = v; w < 6; w++) sd[le][q][sym6(v, w)] = (VT)(scale * cm[v][w]);
+            for (int w = v; w < 6; w++) sd[di(le, q, sym6(v, w))] = (VT)(scale * cm[v][w]);
 #pragma unroll
-        for (int k = 0; k < 6; k++) st[le][q][k] = (VT)(scale * sig[k]);
+        for (int k = 0; k < 6; k++) st[ti(le, q, k)] = (VT)(scale * sig[k]);
     }
     __syncthreads();
     // ---- phase 2: one 3x3 block per lane ----
-    const int le = t / 36, p = t % 36;
+    const int le = PM ? t % EPB : t / 36, p = PM ? t / EPB : t % 36;
     if (base + le < count) {
         // p -> (a, b), a <= b, row-major over the upper triangle of 8x8
         int a = 0, rem = p;
@@ -267,16 +289,17 @@ __global__ void __launch_bounds__(SK_THREADS) k_stiffness_tl(const int* __restri
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) K[i][j] = 0;
+#pragma unroll 1
         for (int q = 0; q < 8; q++) {
             VT D[21], T[6], ga[3], gb[3];
 #pragma unroll
-            for (int k = 0; k < 21; k++) D[k] = sd[le][q][k];
+            for (int k = 0; k < 21; k++) D[k] = sd[di(le, q, k)];
 #pragma unroll
-            for (int k = 0; k < 6; k++) T[k] = st[le][q][k];
+            for (int k = 0; k < 6; k++) T[k] = st[ti(le, q, k)];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                ga[k] = sg[le][q][k][a];
-                gb[k] = sg[le][q][k][b];
+                ga[k] = sg[gi(le, q, k, a)];
+                gb[k] = sg[gi(le, q, k, b)];
             }
             VT h[3][3], geo = 0;
 #pragma unroll
@@ -300,16 +323,41 @@ __global__ void __launch_bounds__(SK_THREADS) k_stiffness_tl(const int* __restri
 #pragma unroll
             for (int i = 0; i < 3; i++) K[i][i] += geo;
         }
-        const int* es = slot + 300L * elems[base + le];
+        const int e = elems[base + le];
+        if (TO_OP) {
+            const int b1 = slot[72L * e + 2 * p], b2 = slot[72L * e + 2 * p + 1];
+            const int A = conn[8L * e + a], B = conn[8L * e + b];
+            bool ra[3], cb[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int m = 3 * a + i, n = 3 * b + j;
-                if (n < m) continue;  // lower half of a diagonal block
-                const int s = es[upper_index(m, n)];
-                if (s >= 0) val[s] += K[i][j];
+            for (int i = 0; i < 3; i++) {
+                ra[i] = eqn[3L * A + i] >= 0;
+                cb[i] = eqn[3L * B + i] >= 0;
             }
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    if (!(ra[i] && cb[j]) || b1 < 0) continue;
+                    if (a == b) {
+                        // both triangles of a diagonal block from the upper value
+                        val[b1 + (3 * i + j) * 64] += K[i < j ? i : j][i < j ? j : i];
+                    } else {
+                        val[b1 + (3 * i + j) * 64] += K[i][j];
+                        if (b2 >= 0) val[b2 + (3 * j + i) * 64] += K[i][j];
+                    }
+                }
+        } else {
+            const int* es = slot + 300L * e;
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int m = 3 * a + i, n = 3 * b + j;
+                    if (n < m) continue;  // lower half of a diagonal block
+                    const int s = es[upper_index(m, n)];
+                    if (s >= 0) val[s] += K[i][j];
+                }
+        }
     }
     if (bad) atomicOr(status, 7);
 }
@@ -366,6 +414,30 @@ static const std::vector<std::pair<int, int>>& stiffness_launches(tg_ctx* c)
     return c->stiff_launches;
 }
 
+template <class VT, int EPB, bool TO_OP, bool PM>
+static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u, VT* val)
+{
+    const int blocks = (count + EPB - 1) / EPB;
+    hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
+                       c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                       TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
+}
+
+template <class VT, bool TO_OP>
+static void launch_stiffness_shape(tg_ctx* c, int v, int first, int count, const double* u, VT* val)
+{
+#define TG_SHAPE(i) launch_stiffness_v<VT, STIFF_EPB[i], TO_OP, STIFF_PM[i]>(c, first, count, u, val)
+    switch (v) {
+    case 0: TG_SHAPE(0); break;
+    case 1: TG_SHAPE(1); break;
+    case 2: TG_SHAPE(2); break;
+    case 3: TG_SHAPE(3); break;
+    case 4: TG_SHAPE(4); break;
+    default: TG_SHAPE(5); break;
+    }
+#undef TG_SHAPE
+}
+
 void launch_stiffness(tg_ctx* c, const double* u)
 {
     TimedRegion tr(c, T_STIFFNESS);
@@ -378,17 +450,33 @@ void launch_stiffness(tg_ctx* c, const double* u)
         launch_q1p0(c, u, c->dR.p, true);
         return;
     }
+    const int v = std::min(std::max(c->stiff_variant, 0), N_STIFF_SHAPES - 1);
     for (const auto& fc : stiffness_launches(c)) {
-        const int first = fc.first, count = fc.second;
-        const int blocks = (count + SK_EPB - 1) / SK_EPB;
         if (mixed)
-            hipLaunchKernelGGL(k_stiffness_tl<float>, dim3(blocks), dim3(SK_THREADS), 0, c->stream,
-                               c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
-                               c->dslot.p, c->dval32.p, c->dstatus.p);
+            launch_stiffness_shape<float, false>(c, v, fc.first, fc.second, u, c->dval32.p);
         else
-            hipLaunchKernelGGL(k_stiffness_tl<double>, dim3(blocks), dim3(SK_THREADS), 0, c->stream,
-                               c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
-                               c->dslot.p, c->dval.p, c->dstatus.p);
+            launch_stiffness_shape<double, false>(c, v, fc.first, fc.second, u, c->dval.p);
+    }
+    TG_HIP(hipGetLastError());
+}
+
+// Newton's stiffness straight into the solver operator (total-Lagrangian
+// element): cleared values, the colour-ordered passes of TO_OP kernels.
+void launch_stiffness_operator(tg_ctx* c, const double* u)
+{
+    TimedRegion tr(c, T_STIFFNESS);
+    const bool mixed = c->precision == TG_MIXED;
+    const size_t nv = (size_t)c->op.nblocks_padded * 9;
+    if (mixed)
+        TG_HIP(hipMemsetAsync(c->op.d_val32.p, 0, sizeof(float) * nv, c->stream));
+    else
+        TG_HIP(hipMemsetAsync(c->op.d_val.p, 0, sizeof(double) * nv, c->stream));
+    const int v = std::min(std::max(c->stiff_variant, 0), N_STIFF_SHAPES - 1);
+    for (const auto& fc : stiffness_launches(c)) {
+        if (mixed)
+            launch_stiffness_shape<float, true>(c, v, fc.first, fc.second, u, c->op.d_val32.p);
+        else
+            launch_stiffness_shape<double, true>(c, v, fc.first, fc.second, u, c->op.d_val.p);
     }
     TG_HIP(hipGetLastError());
 }
@@ -437,6 +525,62 @@ __global__ void k_build_slots(int nel, const int* __restrict__ conn, const int* 
         }
     }
     slot[idx] = s;
+}
+
+// Element block (a, b), a <= b, local upper order p -> value index of
+// component 0 of operator block (row A, column B) and (row B, column A):
+// slice A/64, lane A%64, the slot k whose column is B.
+__global__ void k_build_operator_slots(int nel, const int* __restrict__ conn, const int* __restrict__ sbase,
+                                       const int* __restrict__ swidth, const int* __restrict__ bcol,
+                                       int* __restrict__ oslot)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 72L * nel) return;
+    const int e = (int)(idx / 72), t = (int)(idx % 72), p = t >> 1, dir = t & 1;
+    int a = 0, rem = p;
+    while (rem >= 8 - a) {
+        rem -= 8 - a;
+        a++;
+    }
+    const int b = a + rem;
+    int out = -1;
+    if (!(dir == 1 && a == b)) {
+        const int row = conn[8L * e + (dir ? b : a)], col = conn[8L * e + (dir ? a : b)];
+        const int s = row >> 6, l = row & 63;
+        for (int k = 0; k < swidth[s]; k++)
+            if (bcol[((long)sbase[s] + k) * 64 + l] == col) {
+                out = (int)((((long)sbase[s] + k) * 9) * 64 + l);
+                break;
+            }
+    }
+    oslot[idx] = out;
+}
+
+// value index of component 0 of each row's diagonal block
+__global__ void k_build_operator_diag(int nnd, const int* __restrict__ sbase, const int* __restrict__ swidth,
+                                      const int* __restrict__ bcol, int* __restrict__ diag)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnd) return;
+    const int s = a >> 6, l = a & 63;
+    int out = -1;
+    for (int k = 0; k < swidth[s]; k++)
+        if (bcol[((long)sbase[s] + k) * 64 + l] == a) {
+            out = (int)((((long)sbase[s] + k) * 9) * 64 + l);
+            break;
+        }
+    diag[a] = out;
+}
+
+void launch_build_operator_slots(tg_ctx* c)
+{
+    const long n = 72L * c->nel;
+    const int bs = 256;
+    hipLaunchKernelGGL(k_build_operator_slots, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nel,
+                       c->dconn.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->op.d_oslot.p);
+    hipLaunchKernelGGL(k_build_operator_diag, dim3((unsigned)((c->nnd + bs - 1) / bs)), dim3(bs), 0, c->stream,
+                       c->nnd, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->op.d_diag.p);
+    TG_HIP(hipGetLastError());
 }
 
 void launch_build_slots(tg_ctx* c)

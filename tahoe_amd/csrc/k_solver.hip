@@ -110,6 +110,22 @@ __global__ void k_dinv(long n, const int* __restrict__ eqn, const VT* __restrict
     }
 }
 
+// inverse diagonal from the operator's diagonal blocks (the MSR diagonal
+// when the stiffness went straight into the operator)
+template <class VT>
+__global__ void k_dinv_op(int nnd, const int* __restrict__ eqn, const int* __restrict__ diag,
+                          const VT* __restrict__ val, double* __restrict__ dinv)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnd) return;
+    const int dg = diag[a];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double d = (eqn[3L * a + i] >= 0 && dg >= 0) ? (double)val[dg + 4 * i * 64] : 0.0;
+        dinv[3L * a + i] = d != 0.0 ? 1.0 / d : 0.0;
+    }
+}
+
 // y = A x for block rows of one thread each; with PCG also the block
 // partials of p.q (x = p, y = q) for k_fin_alpha.
 template <bool PCG, class VT>
@@ -232,6 +248,19 @@ void launch_expand32(tg_ctx* c)
     const long nn = 3L * c->nnd;
     hipLaunchKernelGGL(k_dinv<float>, dim3(grid_for(nn)), dim3(VB), 0, c->stream, nn, c->deqn.p, c->dval32.p,
                        c->vdinv.p);
+    TG_HIP(hipGetLastError());
+}
+
+void launch_operator_dinv(tg_ctx* c)
+{
+    TimedRegion tr(c, T_EXPAND);
+    const unsigned blocks = (unsigned)((c->nnd + VB - 1) / VB);
+    if (c->precision == TG_MIXED)
+        hipLaunchKernelGGL(k_dinv_op<float>, dim3(blocks), dim3(VB), 0, c->stream, c->nnd, c->deqn.p,
+                           c->op.d_diag.p, c->op.d_val32.p, c->vdinv.p);
+    else
+        hipLaunchKernelGGL(k_dinv_op<double>, dim3(blocks), dim3(VB), 0, c->stream, c->nnd, c->deqn.p,
+                           c->op.d_diag.p, c->op.d_val.p, c->vdinv.p);
     TG_HIP(hipGetLastError());
 }
 

@@ -375,6 +375,14 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         }
         launch_build_slots(c);
         build_bsell(c);
+        if (c->formulation == TG_TOTAL_LAGRANGIAN) {
+            c->op.d_oslot.alloc(72 * (size_t)c->nel);
+            c->op.d_diag.alloc((size_t)c->nnd);
+            launch_build_operator_slots(c);
+        } else {
+            c->op.d_oslot.release();
+            c->op.d_diag.release();
+        }
         c->op.d_val32.alloc(mixed ? (size_t)c->op.nblocks_padded * 9 : 0);
         setup_segments(c);
         TG_HIP(hipStreamSynchronize(c->stream));
@@ -497,6 +505,19 @@ static void prepare_operator(tg_ctx* c)
         launch_expand(c);
 }
 
+// Newton's tangent in the inner solve's operator: straight into the block
+// operator (total-Lagrangian), or assembled into the MSR and expanded.
+static void newton_operator(tg_ctx* c)
+{
+    if (c->formulation == TG_TOTAL_LAGRANGIAN && c->newton_assembly == 1 && c->op.d_oslot.p) {
+        launch_stiffness_operator(c, c->du.p);
+        launch_operator_dinv(c);
+    } else {
+        launch_stiffness(c, c->du.p);
+        prepare_operator(c);
+    }
+}
+
 static void solve_linear(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
 {
     if (mixed_values(c))
@@ -568,8 +589,7 @@ static void correction(tg_ctx* c, const tg_newton_opts* o, double* err_before, l
     launch_residual(c, c->du.p, c->dR.p);
     check_status(c);
     *err_before = std::sqrt(device_dot(c, c->dR.p, c->dR.p, 3L * c->nnd));
-    launch_stiffness(c, c->du.p);
-    prepare_operator(c);
+    newton_operator(c);
     tg_lin_stats st{};
     solve_linear(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
     check_status(c);
@@ -615,8 +635,7 @@ int tg_newton(tg_ctx* c, const tg_newton_opts* o, tg_newton_stats* s)
         int iter = -1;
         long lin = 0;
         for (;;) {
-            launch_stiffness(c, c->du.p);
-            prepare_operator(c);
+            newton_operator(c);
             tg_lin_stats st{};
             solve_linear(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
             check_status(c);
@@ -675,8 +694,7 @@ int tg_newton_batch(tg_ctx* c, const tg_newton_opts* o, tg_newton_stats* stats)
             }
             if (nact == 0) break;
             TG_HIP(hipMemcpyAsync(c->seg.d_active.p, active.data(), sizeof(int) * G, hipMemcpyHostToDevice, c->stream));
-            launch_stiffness(c, c->du.p);
-            prepare_operator(c);
+            newton_operator(c);
             tg_lin_stats st{};
             solve_linear(c, c->dR.p, c->vx.p, o->lin_rtol, o->lin_maxit, &st);
             check_status(c);
@@ -749,9 +767,17 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
 {
     if (!c) return TG_BAD_INPUT;
     return guarded(c, [&] {
-        if (knob != TG_TUNE_STIFF_CHUNK) fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
-        if (value < -1) fail(TG_BAD_INPUT, "stiffness chunk must be -1, 0 or positive");
-        c->stiff_chunk = value;
+        if (knob == TG_TUNE_STIFF_CHUNK) {
+            if (value < -1) fail(TG_BAD_INPUT, "stiffness chunk must be -1, 0 or positive");
+            c->stiff_chunk = value;
+        } else if (knob == TG_TUNE_STIFF_SHAPE) {
+            if (value < 0 || value > 5) fail(TG_OUT_OF_RANGE, "stiffness shape must be 0..5");
+            c->stiff_variant = (int)value;
+        } else if (knob == TG_TUNE_NEWTON_ASSEMBLY) {
+            if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "Newton assembly must be 0 or 1");
+            c->newton_assembly = (int)value;
+        } else
+            fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
     });
 }
 
@@ -759,8 +785,14 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
 {
     if (!c || !value) return TG_BAD_INPUT;
     return guarded(c, [&] {
-        if (knob != TG_TUNE_STIFF_CHUNK) fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
-        *value = c->stiff_chunk;
+        if (knob == TG_TUNE_STIFF_CHUNK)
+            *value = c->stiff_chunk;
+        else if (knob == TG_TUNE_STIFF_SHAPE)
+            *value = c->stiff_variant;
+        else if (knob == TG_TUNE_NEWTON_ASSEMBLY)
+            *value = c->newton_assembly;
+        else
+            fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
     });
 }
 

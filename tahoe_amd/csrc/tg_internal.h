@@ -60,6 +60,9 @@ struct Bsell {
     DBuf<int> d_src;                       // [nblocks_padded * 9] MSR val index, -1 = zero
     DBuf<double> d_val;                    // [nblocks_padded * 9]
     DBuf<float> d_val32;                   // FP32 copy for the mixed-precision inner solve
+    DBuf<int> d_oslot;                     // [nel][36][2] element block (a<=b) -> value index of
+                                           // block (A,B) and of (B,A) (-1 when a == b)
+    DBuf<int> d_diag;                      // [nnd] value index of the diagonal block of each row
 };
 
 struct Timing {
@@ -118,6 +121,8 @@ struct tg_ctx {
     // stiff_chunk (by rank inside the segment), all colours of range j
     // before range j+1 (0: whole colours)
     long stiff_chunk = -1;              // -1: chosen from the matrix footprint
+    int stiff_variant = 0;              // block shape of the stiffness kernel
+    int newton_assembly = 1;            // 1: Newton's K straight into the solver operator, 0: via MSR
     long stiff_chunk_built = -2;
     std::vector<std::pair<int, int>> stiff_launches;  // (first, count) into dstiff_elems
 
@@ -163,6 +168,9 @@ void build_bsell(tg_ctx* c);
 // kernels launched from the ABI (k_element.hip, k_solver.hip)
 void launch_residual(tg_ctx* c, const double* u, double* R_node);
 void launch_stiffness(tg_ctx* c, const double* u);
+void launch_stiffness_operator(tg_ctx* c, const double* u);
+void launch_build_operator_slots(tg_ctx* c);
+void launch_operator_dinv(tg_ctx* c);
 void launch_build_slots(tg_ctx* c);
 void launch_expand(tg_ctx* c);
 void launch_spmv(tg_ctx* c, const double* x, double* y);

@@ -161,3 +161,48 @@ def test_large_mesh_against_oracle_and_invariants():
     x = rng.standard_normal(n)
     y = rng.standard_normal(n)
     assert abs(y @ s.multx(x) - x @ s.multx(y)) <= 1e-10 * np.linalg.norm(x) * np.linalg.norm(y) * np.abs(vo).max()
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("chunk", [0, 100])
+def test_stiffness_every_launch_shape(shape, chunk):
+    """Every compiled workgroup shape and launch-range split assembles the
+    same stiffness (the oracle's, to the FP64 bar) on a ragged element count
+    (tl_n8: 512 elements, not a multiple of any shape; chunk 100 splits every
+    colour into several launches)."""
+    from oracle import oracle as O
+    from tahoe_amd import api
+    g = load_golden("tl_n8")
+    s = make_solid(g)
+    s.set_tuning(api.TUNE_STIFF_SHAPE, shape)
+    s.set_tuning(api.TUNE_STIFF_CHUNK, chunk)
+    assert s.tuning(api.TUNE_STIFF_SHAPE) == shape and s.tuning(api.TUNE_STIFF_CHUNK) == chunk
+    u = g["step_u"][0]
+    s.set_displacement(u)
+    s.form_lhs()
+    val = s.msr_values()
+    mu, ka = O.mu_kappa(1000.0, 0.3)
+    oval = O.stiffness(O.TL, g["X"], g["conn"], g["mat"], mu, ka, u, g["eqnos"], s.n_eq, s.msr_bindx())
+    assert np.abs(val - oval).max() <= TOL_RK * np.abs(oval).max()
+
+
+@pytest.mark.parametrize("name", ["tl_n8", "rve_n4"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_operator_assembly_equals_msr_expansion(name, precision):
+    """Newton's tangent assembled straight into the solver operator gives
+    the operator that expanding the assembled MSR gives (same sums, same
+    order): the Newton corrections agree bitwise, prescribed dofs included."""
+    from tahoe_amd import api
+    g = load_golden(name)
+    out = []
+    for mode in (0, 1):
+        s = make_solid(g)
+        if precision:
+            s.set_precision(api.MIXED)
+            s.initialize()
+        s.set_tuning(api.TUNE_NEWTON_ASSEMBLY, mode)
+        s.apply_kbc(1.0 / int(g["num_steps"]))
+        st = s.newton_correction(s.opts(lin_rtol=1e-10))
+        out.append((s.displacement(), st.lin_iterations))
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
