@@ -48,6 +48,9 @@ def parse_args():
     ap.add_argument("--lin-rtol", type=float, default=1e-10)
     ap.add_argument("--cpu-n", type=int, default=64, help="edge of the CPU-baseline sample cube (0: skip)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precond", choices=["jacobi", "block"], default="jacobi",
+                    help="inner PCG preconditioner: point Jacobi (the reference's DiagonalMatrixT) or 3x3 nodal "
+                         "block Jacobi")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: 2 Newton corrections with 48 PCG iterations each")
     ap.add_argument("--workload", choices=["cube", "mixed-steps", "rve-batch", "rve-serve"], default="cube",
@@ -132,6 +135,7 @@ def main():
     s.set_formulation(api.TOTAL_LAGRANGIAN)
     s.set_precision(api.MIXED if a.precision == "mixed" else api.FP64)
     s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
+    s.set_preconditioner(api.PC_BLOCK_JACOBI if a.precond == "block" else api.PC_JACOBI)
     n_eq, msr_len = s.initialize()
     print(f"# setup {time.perf_counter() - t_setup:.1f} s: nel={m.nel} nnd={m.nnd} n_eq={n_eq} "
           f"msr_upper_nnz={msr_len - 1}", file=sys.stderr, flush=True)
@@ -210,8 +214,8 @@ def main():
         "dtype": "f64" if a.precision == "fp64" else "f32/f64",
         "data": "synthetic",
         "config": {"workload": f"{n}^3 Q1 hex TL neo-Hookean (Simo_isotropic E=1000 nu=0.3), +-5% perturbed, "
-                               "uniaxial 5% load step 1/5, one Newton correction: R + K into MSR + Jacobi-PCG "
-                               f"to {a.lin_rtol:g}",
+                               "uniaxial 5% load step 1/5, one Newton correction: R + K into the solver operator + "
+                               f"{'block-' if a.precond == 'block' else ''}Jacobi-PCG to {a.lin_rtol:g}",
                    "elements": m.nel, "nodes": m.nnd, "equations": n_eq, "msr_upper_nnz": msr_len - 1,
                    "stiffness_shape_elements_per_workgroup": api.STIFF_SHAPE_EPB[tuned["shape"]],
                    "stiffness_chunk": tuned["chunk"]},

@@ -98,6 +98,12 @@ int tg_set_formulation(tg_ctx* ctx, int formulation);
 /* TG_FP64 (default) or TG_MIXED; no counterpart in the reference, which is
  * FP64 only.  TG_MIXED applies to the total-Lagrangian element. */
 int tg_set_precision(tg_ctx* ctx, int precision);
+/* Preconditioner of the inner PCG: TG_PC_JACOBI (default) is the
+ * reference's DiagonalMatrixT (1/diag, DiagonalMatrixT.cpp:267-324);
+ * TG_PC_BLOCK_JACOBI inverts each node's 3x3 diagonal block (no counterpart
+ * in the reference).  Either may be set before or after tg_initialize. */
+enum { TG_PC_JACOBI = 0, TG_PC_BLOCK_JACOBI = 1 };
+int tg_set_preconditioner(tg_ctx* ctx, int pc);
 /* Kinematic BC cards: prescribed value = values[k] * t (KBC_CardT::Value,
  * tahoe/src/primitives/KBC_CardT.cpp:53-64, linear schedule). */
 int tg_set_kbc(tg_ctx* ctx, int nkbc, const int* nodes, const int* dofs, const double* values);

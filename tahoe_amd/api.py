@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libtahoe_gpu.so")
 TOTAL_LAGRANGIAN, Q1P0 = 0, 1
 FP64, MIXED = 0, 1
 TUNE_STIFF_CHUNK, TUNE_STIFF_SHAPE, TUNE_NEWTON_ASSEMBLY = 1, 2, 3
+PC_JACOBI, PC_BLOCK_JACOBI = 0, 1
 STIFF_SHAPE_EPB = (16, 12, 8, 4, 16, 8)
 STIFF_SHAPE_BLOCK_MAJOR = (False, False, False, False, True, True)
 
@@ -25,7 +26,7 @@ I = C.POINTER(C.c_int)
 # exported symbols, in declaration order of include/tahoe_gpu.h
 SYMBOLS = [
     "tg_create", "tg_destroy", "tg_last_error", "tg_set_mesh", "tg_set_material", "tg_set_formulation",
-    "tg_set_precision", "tg_set_kbc", "tg_set_segments", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
+    "tg_set_precision", "tg_set_preconditioner", "tg_set_kbc", "tg_set_segments", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
     "tg_get_displacement", "tg_apply_kbc", "tg_close_step", "tg_reset_step", "tg_form_rhs", "tg_form_lhs",
     "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_batch", "tg_newton_correction",
     "tg_synchronize", "tg_timing_enable", "tg_timing_reset", "tg_timing_get", "tg_calib_copy", "tg_set_tuning",
@@ -85,6 +86,7 @@ def lib():
         L.tg_set_material.argtypes = [vp, C.c_int, C.c_double, C.c_double]
         L.tg_set_formulation.argtypes = [vp, C.c_int]
         L.tg_set_precision.argtypes = [vp, C.c_int]
+        L.tg_set_preconditioner.argtypes = [vp, C.c_int]
         L.tg_set_kbc.argtypes = [vp, C.c_int, I, I, D]
         L.tg_set_segments.argtypes = [vp, C.c_int, I]
         L.tg_initialize.argtypes = [vp, I, C.POINTER(C.c_long)]
@@ -175,6 +177,10 @@ class Solid:
 
     def set_precision(self, p):
         self._ok(lib().tg_set_precision(self._h, p))
+
+    def set_preconditioner(self, pc):
+        """PC_JACOBI (the reference's DiagonalMatrixT) or PC_BLOCK_JACOBI."""
+        self._ok(lib().tg_set_preconditioner(self._h, pc))
 
     def set_kbc(self, nodes, dofs, values):
         n = np.ascontiguousarray(nodes, dtype=np.int32)

@@ -578,6 +578,12 @@ void launch_build_operator_slots(tg_ctx* c)
     const int bs = 256;
     hipLaunchKernelGGL(k_build_operator_slots, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nel,
                        c->dconn.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->op.d_oslot.p);
+    TG_HIP(hipGetLastError());
+}
+
+void launch_build_operator_diag(tg_ctx* c)
+{
+    const int bs = 256;
     hipLaunchKernelGGL(k_build_operator_diag, dim3((unsigned)((c->nnd + bs - 1) / bs)), dim3(bs), 0, c->stream,
                        c->nnd, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->op.d_diag.p);
     TG_HIP(hipGetLastError());

@@ -28,6 +28,22 @@ __device__ __forceinline__ double wsum(double v)
 
 __device__ __forceinline__ double* sf(double* s, int field, int nseg, int seg) { return s + (long)field * nseg + seg; }
 
+// z = M^-1 r at node a.  PC = 0: Jacobi, m = 1/diag [nnd][3]
+// (DiagonalMatrixT); PC = 1: 3x3 nodal block Jacobi, m = inverse diagonal
+// blocks [nnd][9] (row-major, prescribed rows and columns zero).
+template <int PC>
+__device__ __forceinline__ void precondition(const double* __restrict__ m, long a, const double (&r)[3], double (&z)[3])
+{
+    if (PC == 0) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) z[i] = m[3 * a + i] * r[i];
+    } else {
+        const double* b = m + 9 * a;
+#pragma unroll
+        for (int i = 0; i < 3; i++) z[i] = b[3 * i] * r[0] + b[3 * i + 1] * r[1] + b[3 * i + 2] * r[2];
+    }
+}
+
 // ---- per-slice partials -> per-segment totals ------------------------------
 
 // Sum of part[k][sb..se) by one block of FB threads: thread i adds slices
@@ -56,24 +72,28 @@ __device__ __forceinline__ void seg_total(const double* part, long nslices, int 
         }
 }
 
-// x = 0, r = b, p = D^-1 b; per segment rz = b.D^-1 b, bb = b.b
+// x = 0, r = b, p = M^-1 b; per segment rz = b.M^-1 b, bb = b.b
+template <int PC>
 __global__ void __launch_bounds__(PB) k_seg_init(int nnd, const double* __restrict__ b,
-                                                 const double* __restrict__ dinv, double* __restrict__ x,
+                                                 const double* __restrict__ m, double* __restrict__ x,
                                                  double* __restrict__ r, double* __restrict__ p,
                                                  double* __restrict__ part, long nslices)
 {
     const int a = blockIdx.x * PB + threadIdx.x;
     double v0 = 0.0, v1 = 0.0;
     if (a < nnd) {
+        double bi[3], zi[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) bi[i] = b[3L * a + i];
+        precondition<PC>(m, a, bi, zi);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const long k = 3L * a + i;
-            const double bi = b[k], zi = dinv[k] * bi;
             x[k] = 0.0;
-            r[k] = bi;
-            p[k] = zi;
-            v0 += bi * zi;
-            v1 += bi * bi;
+            r[k] = bi[i];
+            p[k] = zi[i];
+            v0 += bi[i] * zi[i];
+            v1 += bi[i] * bi[i];
         }
     }
     v0 = wsum(v0);
@@ -162,10 +182,11 @@ __global__ void __launch_bounds__(FB) k_seg_fin_alpha(const double* __restrict__
     }
 }
 
-// x += alpha p, r -= alpha q; per-slice partials of r.D^-1 r and r.r
+// x += alpha p, r -= alpha q; per-slice partials of r.M^-1 r and r.r
+template <int PC>
 __global__ void __launch_bounds__(PB) k_seg_update(int nnd, const int* __restrict__ slice_seg,
                                                    const double* __restrict__ p, const double* __restrict__ q,
-                                                   const double* __restrict__ dinv, double* __restrict__ x,
+                                                   const double* __restrict__ m, double* __restrict__ x,
                                                    double* __restrict__ r, double* __restrict__ part, long nslices,
                                                    int nseg, const double* __restrict__ sc)
 {
@@ -177,14 +198,19 @@ __global__ void __launch_bounds__(PB) k_seg_update(int nnd, const int* __restric
     const double alpha = *sf((double*)sc, S_ALPHA, nseg, g);
     double v0 = 0.0, v1 = 0.0;
     if (a < nnd) {
+        double ri[3], zi[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const long k = 3L * a + i;
             x[k] += alpha * p[k];
-            const double ri = r[k] - alpha * q[k];
-            r[k] = ri;
-            v0 += ri * dinv[k] * ri;
-            v1 += ri * ri;
+            ri[i] = r[k] - alpha * q[k];
+            r[k] = ri[i];
+        }
+        precondition<PC>(m, a, ri, zi);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            v0 += ri[i] * zi[i];
+            v1 += ri[i] * ri[i];
         }
     }
     v0 = wsum(v0);
@@ -217,9 +243,10 @@ __global__ void __launch_bounds__(FB) k_seg_fin_beta(const double* __restrict__ 
     }
 }
 
-// p = D^-1 r + beta p
+// p = M^-1 r + beta p
+template <int PC>
 __global__ void __launch_bounds__(PB) k_seg_direction(int nnd, const int* __restrict__ slice_seg,
-                                                      const double* __restrict__ r, const double* __restrict__ dinv,
+                                                      const double* __restrict__ r, const double* __restrict__ m,
                                                       double* __restrict__ p, long nslices, int nseg,
                                                       const double* __restrict__ sc)
 {
@@ -228,10 +255,14 @@ __global__ void __launch_bounds__(PB) k_seg_direction(int nnd, const int* __rest
     const int g = slice_seg[a >> 6];
     if (*sf((double*)sc, S_DONE, nseg, g) != 0.0) return;
     const double beta = *sf((double*)sc, S_BETA, nseg, g);
+    double ri[3], zi[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) ri[i] = r[3L * a + i];
+    precondition<PC>(m, a, ri, zi);
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         const long k = 3L * a + i;
-        p[k] = dinv[k] * r[k] + beta * p[k];
+        p[k] = zi[i] + beta * p[k];
     }
 }
 
@@ -277,9 +308,24 @@ void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out)
     TG_HIP(hipStreamSynchronize(c->stream));
 }
 
+template <class MT, int PC>
+static void pcg_loop(tg_ctx* c, const MT* val, const double* m, const double* b, double* x, double* r, double* p,
+                     double* q, double rtol, int maxit, const int* d_active, tg_lin_stats* st);
+
 template <class MT>
 void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, double* x, double* r, double* p,
              double* q, double rtol, int maxit, const int* d_active, int nactive, tg_lin_stats* st)
+{
+    (void)nactive;
+    if (c->precond == TG_PC_BLOCK_JACOBI)
+        pcg_loop<MT, 1>(c, val, c->vbinv.p, b, x, r, p, q, rtol, maxit, d_active, st);
+    else
+        pcg_loop<MT, 0>(c, val, dinv, b, x, r, p, q, rtol, maxit, d_active, st);
+}
+
+template <class MT, int PC>
+static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double* b, double* x, double* r, double* p,
+                     double* q, double rtol, int maxit, const int* d_active, tg_lin_stats* st)
 {
     Segments& S = c->seg;
     const long ns = S.nslices;
@@ -288,7 +334,7 @@ void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, doub
     TG_HIP(hipMemsetAsync(S.d_ndone.p, 0, sizeof(int), c->stream));
     {
         TimedRegion tr(c, T_VECTOR);
-        hipLaunchKernelGGL(k_seg_init, dim3(nb), dim3(PB), 0, c->stream, c->nnd, b, dinv, x, r, p, S.d_part.p, ns);
+        hipLaunchKernelGGL(k_seg_init<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, b, dinv, x, r, p, S.d_part.p, ns);
         hipLaunchKernelGGL(k_seg_fin_init, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns, S.d_seg_slice.p,
                            d_active, nseg, S.d_scal.p, rtol, maxit, S.d_ndone.p);
     }
@@ -306,11 +352,11 @@ void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, doub
                 TimedRegion tr(c, T_VECTOR);
                 hipLaunchKernelGGL(k_seg_fin_alpha, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
                                    S.d_seg_slice.p, nseg, S.d_scal.p);
-                hipLaunchKernelGGL(k_seg_update, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, p, q,
+                hipLaunchKernelGGL(k_seg_update<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, p, q,
                                    dinv, x, r, S.d_part.p, ns, nseg, S.d_scal.p);
                 hipLaunchKernelGGL(k_seg_fin_beta, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
                                    S.d_seg_slice.p, nseg, S.d_scal.p, S.d_ndone.p);
-                hipLaunchKernelGGL(k_seg_direction, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, r,
+                hipLaunchKernelGGL(k_seg_direction<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, r,
                                    dinv, p, ns, nseg, S.d_scal.p);
             }
         }
@@ -340,7 +386,6 @@ void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, doub
             if (rr > th) st->converged = 0;
         }
     }
-    (void)nactive;
 }
 
 template void pcg_run<double>(tg_ctx*, const double*, const double*, const double*, double*, double*, double*,

@@ -126,6 +126,50 @@ __global__ void k_dinv_op(int nnd, const int* __restrict__ eqn, const int* __res
     }
 }
 
+// 3x3 nodal block Jacobi: inverse of each row's diagonal block over its
+// active dofs (prescribed rows and columns of the operator are zero; they
+// stay zero in the inverse).  Singular active blocks are left zero, the
+// block analogue of DiagonalMatrixT's skipped zero pivots.
+template <class VT>
+__global__ void k_binv_op(int nnd, const int* __restrict__ eqn, const int* __restrict__ diag,
+                          const VT* __restrict__ val, double* __restrict__ binv)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnd) return;
+    const int dg = diag[a];
+    bool act[3];
+    double M[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++) act[i] = eqn[3L * a + i] >= 0 && dg >= 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            M[3 * i + j] = (act[i] && act[j]) ? (double)val[dg + (3 * i + j) * 64] : (i == j ? 1.0 : 0.0);
+    const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+                       M[2] * (M[3] * M[7] - M[4] * M[6]);
+    double Mi[9];
+    if (det != 0.0) {
+        const double id = 1.0 / det;
+        Mi[0] = (M[4] * M[8] - M[5] * M[7]) * id;
+        Mi[1] = (M[2] * M[7] - M[1] * M[8]) * id;
+        Mi[2] = (M[1] * M[5] - M[2] * M[4]) * id;
+        Mi[3] = (M[5] * M[6] - M[3] * M[8]) * id;
+        Mi[4] = (M[0] * M[8] - M[2] * M[6]) * id;
+        Mi[5] = (M[2] * M[3] - M[0] * M[5]) * id;
+        Mi[6] = (M[3] * M[7] - M[4] * M[6]) * id;
+        Mi[7] = (M[1] * M[6] - M[0] * M[7]) * id;
+        Mi[8] = (M[0] * M[4] - M[1] * M[3]) * id;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) Mi[k] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) binv[9L * a + 3 * i + j] = (act[i] && act[j]) ? Mi[3 * i + j] : 0.0;
+}
+
 // y = A x for block rows of one thread each; with PCG also the block
 // partials of p.q (x = p, y = q) for k_fin_alpha.
 template <bool PCG, class VT>
@@ -261,6 +305,21 @@ void launch_operator_dinv(tg_ctx* c)
     else
         hipLaunchKernelGGL(k_dinv_op<double>, dim3(blocks), dim3(VB), 0, c->stream, c->nnd, c->deqn.p,
                            c->op.d_diag.p, c->op.d_val.p, c->vdinv.p);
+    TG_HIP(hipGetLastError());
+}
+
+void launch_block_inverse(tg_ctx* c)
+{
+    if (c->precond != TG_PC_BLOCK_JACOBI) return;
+    TimedRegion tr(c, T_EXPAND);
+    const unsigned blocks = (unsigned)((c->nnd + VB - 1) / VB);
+    const bool f32 = c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN;
+    if (f32)
+        hipLaunchKernelGGL(k_binv_op<float>, dim3(blocks), dim3(VB), 0, c->stream, c->nnd, c->deqn.p,
+                           c->op.d_diag.p, c->op.d_val32.p, c->vbinv.p);
+    else
+        hipLaunchKernelGGL(k_binv_op<double>, dim3(blocks), dim3(VB), 0, c->stream, c->nnd, c->deqn.p,
+                           c->op.d_diag.p, c->op.d_val.p, c->vbinv.p);
     TG_HIP(hipGetLastError());
 }
 

@@ -300,6 +300,16 @@ int tg_set_segments(tg_ctx* c, int nseg, const int* nb)
     });
 }
 
+int tg_set_preconditioner(tg_ctx* c, int pc)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (pc != TG_PC_JACOBI && pc != TG_PC_BLOCK_JACOBI) fail(TG_BAD_INPUT, "unknown preconditioner");
+        c->precond = pc;
+        if (c->initialized) c->vbinv.alloc(pc == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
+    });
+}
+
 int tg_set_kbc(tg_ctx* c, int nkbc, const int* nodes, const int* dofs, const double* values)
 {
     if (!c) return TG_BAD_INPUT;
@@ -375,14 +385,15 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         }
         launch_build_slots(c);
         build_bsell(c);
+        c->op.d_diag.alloc((size_t)c->nnd);
+        launch_build_operator_diag(c);
         if (c->formulation == TG_TOTAL_LAGRANGIAN) {
             c->op.d_oslot.alloc(72 * (size_t)c->nel);
-            c->op.d_diag.alloc((size_t)c->nnd);
             launch_build_operator_slots(c);
         } else {
             c->op.d_oslot.release();
-            c->op.d_diag.release();
         }
+        c->vbinv.alloc(c->precond == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
         c->op.d_val32.alloc(mixed ? (size_t)c->op.nblocks_padded * 9 : 0);
         setup_segments(c);
         TG_HIP(hipStreamSynchronize(c->stream));
@@ -503,6 +514,7 @@ static void prepare_operator(tg_ctx* c)
         launch_expand32(c);
     else
         launch_expand(c);
+    launch_block_inverse(c);
 }
 
 // Newton's tangent in the inner solve's operator: straight into the block
@@ -512,6 +524,7 @@ static void newton_operator(tg_ctx* c)
     if (c->formulation == TG_TOTAL_LAGRANGIAN && c->newton_assembly == 1 && c->op.d_oslot.p) {
         launch_stiffness_operator(c, c->du.p);
         launch_operator_dinv(c);
+        launch_block_inverse(c);
     } else {
         launch_stiffness(c, c->du.p);
         prepare_operator(c);

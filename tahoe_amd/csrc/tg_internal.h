@@ -123,6 +123,7 @@ struct tg_ctx {
     long stiff_chunk = -1;              // -1: chosen from the matrix footprint
     int stiff_variant = 0;              // block shape of the stiffness kernel
     int newton_assembly = 1;            // 1: Newton's K straight into the solver operator, 0: via MSR
+    int precond = TG_PC_JACOBI;         // preconditioner of the inner PCG
     long stiff_chunk_built = -2;
     std::vector<std::pair<int, int>> stiff_launches;  // (first, count) into dstiff_elems
 
@@ -144,6 +145,7 @@ struct tg_ctx {
 
     // PCG work vectors in node space
     tg::DBuf<double> vx, vr, vz, vp, vq, vdinv, vwork;
+    tg::DBuf<double> vbinv;             // [nnd][9] inverse diagonal blocks (block Jacobi)
     tg::DBuf<double> dpart;             // partial reduction sums
     tg::DBuf<double> dscal;             // device scalars
     tg::DBuf<int> dstatus;              // device error flag (7 = bad Jacobian)
@@ -171,6 +173,8 @@ void launch_stiffness(tg_ctx* c, const double* u);
 void launch_stiffness_operator(tg_ctx* c, const double* u);
 void launch_build_operator_slots(tg_ctx* c);
 void launch_operator_dinv(tg_ctx* c);
+void launch_block_inverse(tg_ctx* c);
+void launch_build_operator_diag(tg_ctx* c);
 void launch_build_slots(tg_ctx* c);
 void launch_expand(tg_ctx* c);
 void launch_spmv(tg_ctx* c, const double* x, double* y);
