@@ -201,7 +201,12 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      tg_copy_diagonal) see the matrix of the last
  *                      tg_form_lhs, as GlobalMatrixT's do between
  *                      assemblies. */
-enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3 };
+/* TG_TUNE_PCG_GRAPH    1 (default): the PCG's 16-iteration chunk is captured
+ *                      once as a HIP graph and replayed (one launch instead
+ *                      of 80, for small, launch-bound problems); 0: direct
+ *                      launches.  Per-kernel timing (tg_timing_enable)
+ *                      always uses direct launches. */
+enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3, TG_TUNE_PCG_GRAPH = 4 };
 int tg_set_tuning(tg_ctx* ctx, int knob, long value);
 int tg_get_tuning(tg_ctx* ctx, int knob, long* value);
 

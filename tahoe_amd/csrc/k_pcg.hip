@@ -12,6 +12,8 @@
 // Iterative Methods for Sparse Linear Systems, Alg. 9.1) on the reference's
 // building blocks: Multx (MSRMatrixT.cpp:385-420), Jacobi scaling
 // (DiagonalMatrixT.cpp:267-324) and InnerProduct (SolverT.cpp:854-860).
+#include <cstring>
+
 #include "tg_internal.h"
 
 namespace tg {
@@ -339,8 +341,7 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
                            d_active, nseg, S.d_scal.p, rtol, maxit, S.d_ndone.p);
     }
     const int CHUNK = 16;
-    int issued = 0;
-    for (;;) {
+    auto issue_chunk = [&]() {
         for (int k = 0; k < CHUNK; k++) {
             {
                 TimedRegion tr(c, T_SPMV);
@@ -360,12 +361,52 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
                                    dinv, p, ns, nseg, S.d_scal.p);
             }
         }
+    };
+    // The chunk's kernels only ever see the same pointers, so one capture
+    // serves every later solve of this context with the same vectors; the
+    // iteration state lives in device memory (done flags make finished
+    // segments' work a no-op).  The graph holds kernel nodes only.
+    const bool graph = c->pcg_graph && !c->timing.enabled;
+    if (graph) {
+        const void* key[12] = {val, dinv, x, r, p, q, S.d_part.p, S.d_scal.p, S.d_ndone.p,
+                               (const void*)(long)PC, (const void*)(long)sizeof(MT), (const void*)(long)c->nnd};
+        if (!c->pg.exec || std::memcmp(key, c->pg.key, sizeof key) != 0) {
+            if (c->pg.exec) TG_HIP(hipGraphExecDestroy(c->pg.exec));
+            if (c->pg.graph) TG_HIP(hipGraphDestroy(c->pg.graph));
+            c->pg.exec = nullptr;
+            c->pg.graph = nullptr;
+            TG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            issue_chunk();
+            TG_HIP(hipStreamEndCapture(c->stream, &c->pg.graph));
+            TG_HIP(hipGraphInstantiate(&c->pg.exec, c->pg.graph, nullptr, nullptr, 0));
+            std::memcpy(c->pg.key, key, sizeof key);
+        }
+    }
+    for (auto& e : c->poll_ev)
+        if (!e) TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // Chunk k+1 is queued before chunk k's done count is read, so the GPU
+    // never waits for the host; a chunk queued after every segment finished
+    // does nothing.
+    int issued = 0, chunks = 0;
+    for (;;) {
+        if (graph)
+            TG_HIP(hipGraphLaunch(c->pg.exec, c->stream));
+        else
+            issue_chunk();
         TG_HIP(hipGetLastError());
         issued += CHUNK;
-        TG_HIP(hipMemcpyAsync(c->hstatus + 2, S.d_ndone.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        TG_HIP(hipStreamSynchronize(c->stream));
-        if (c->hstatus[2] >= nseg || issued >= maxit) break;
+        const int slot = chunks & 1;
+        TG_HIP(hipMemcpyAsync(c->hstatus + 2 + slot, S.d_ndone.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipEventRecord(c->poll_ev[slot], c->stream));
+        chunks++;
+        if (issued >= maxit) break;
+        if (chunks >= 2) {
+            const int prev = (chunks - 2) & 1;
+            TG_HIP(hipEventSynchronize(c->poll_ev[prev]));
+            if (c->hstatus[2 + prev] >= nseg) break;
+        }
     }
+    TG_HIP(hipStreamSynchronize(c->stream));
     // per-segment results
     S.h_scal.resize((size_t)S_NSCAL * nseg);
     TG_HIP(hipMemcpyAsync(S.h_scal.data(), S.d_scal.p, sizeof(double) * S_NSCAL * nseg, hipMemcpyDeviceToHost,

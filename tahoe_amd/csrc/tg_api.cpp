@@ -211,6 +211,10 @@ int tg_destroy(tg_ctx* c)
             g_events.erase(it);
         }
     }
+    if (c->pg.exec) (void)hipGraphExecDestroy(c->pg.exec);
+    if (c->pg.graph) (void)hipGraphDestroy(c->pg.graph);
+    for (hipEvent_t e : c->poll_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->hscal) (void)hipHostFree(c->hscal);
     if (c->hstatus) (void)hipHostFree(c->hstatus);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -789,6 +793,9 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
         } else if (knob == TG_TUNE_NEWTON_ASSEMBLY) {
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "Newton assembly must be 0 or 1");
             c->newton_assembly = (int)value;
+        } else if (knob == TG_TUNE_PCG_GRAPH) {
+            if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "PCG graph must be 0 or 1");
+            c->pcg_graph = (int)value;
         } else
             fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
     });
@@ -804,6 +811,8 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
             *value = c->stiff_variant;
         else if (knob == TG_TUNE_NEWTON_ASSEMBLY)
             *value = c->newton_assembly;
+        else if (knob == TG_TUNE_PCG_GRAPH)
+            *value = c->pcg_graph;
         else
             fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
     });

@@ -124,6 +124,13 @@ struct tg_ctx {
     int stiff_variant = 0;              // block shape of the stiffness kernel
     int newton_assembly = 1;            // 1: Newton's K straight into the solver operator, 0: via MSR
     int precond = TG_PC_JACOBI;         // preconditioner of the inner PCG
+    int pcg_graph = 1;                  // replay PCG chunks as a captured HIP graph
+    struct {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        const void* key[12] = {};
+    } pg;                               // the captured chunk and the arguments it was captured with
+    hipEvent_t poll_ev[2] = {nullptr, nullptr};
     long stiff_chunk_built = -2;
     std::vector<std::pair<int, int>> stiff_launches;  // (first, count) into dstiff_elems
 

@@ -61,3 +61,24 @@ def test_single_segment_batch_equals_newton():
     st2 = s2.newton()
     assert np.array_equal(ub, s2.displacement())
     assert st[0].iterations == st2.iterations
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_replayed_pcg_equals_direct_launches(precision):
+    """The PCG chunk captured once as a HIP graph and replayed across every
+    Newton iteration and load step (with eager assembly kernels and host
+    syncs in between) gives bitwise the direct-launch results."""
+    from tahoe_amd import api, batch
+    probs = [golden_problem(n)[1] for n in ["tl_n2", "rve_n3", "rve_n4"]] + [rve_problem(5, 81)]
+    opts = api.Solid.opts(lin_rtol=1e-12 if precision == 0 else 1e-6)
+    out = []
+    for graph in (0, 1):
+        pk = batch.pack(probs)
+        s = batch.make_solid(pk, precision=precision)
+        s.set_tuning(api.TUNE_PCG_GRAPH, graph)
+        us, st = batch.static_solve_batch(s, pk, 2, opts)
+        out.append((us, [[x.iterations for x in row] for row in st]))
+        s.close()
+    assert out[0][1] == out[1][1]
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
