@@ -329,7 +329,9 @@ def bench_rve_batch(a):
     sample = reqs[::k][:a.plain_sample]
     solve_alone(sample[0], api.FP64, fp64)  # warm-up (library load, first context)
     plain_u, plain_wall, plain_solve, plain_el = {}, 0.0, 0.0, 0
-    for r in sample:
+    for i, r in enumerate(sample):
+        if i % 32 == 0:
+            print(f"# plain FP64 RVE {i}/{len(sample)}", file=sys.stderr, flush=True)
         u, st, wall, solve = solve_alone(r, api.FP64, fp64)
         assert st.status == 0
         plain_u[r.rid] = u
@@ -389,13 +391,17 @@ def bench_rve_serve(a):
     from tahoe_amd import api, rve, serve
     # capacity: everything arriving at once
     cal = rve.stream_requests(min(128, a.serve_count), rate=1e9, seed=555)
-    srv = serve.RVEServer(precision=api.MIXED, keep_u=False)
+    def progress(bid, size, form, sec):
+        print(f"# batch {bid}: {size} requests ({'Q1P0' if form == api.Q1P0 else 'Q1'}) in {sec:.2f} s",
+              file=sys.stderr, flush=True)
+
+    srv = serve.RVEServer(precision=api.MIXED, keep_u=False, on_batch=progress)
     st = srv.run(cal, realtime=False)
     cap = len(cal) / st.wall
     rate = a.serve_load * cap
     print(f"# capacity {cap:.1f} solves/s -> offered {rate:.1f}/s", file=sys.stderr, flush=True)
     reqs = rve.stream_requests(a.serve_count, rate=rate)
-    srv = serve.RVEServer(precision=api.MIXED)
+    srv = serve.RVEServer(precision=api.MIXED, on_batch=progress)
     st = srv.run(reqs, realtime=True)
     lat = serve.latency_summary(st)
     # plain FP64 on the same arrivals: one request at a time
@@ -403,7 +409,9 @@ def bench_rve_serve(a):
     n_plain = min(len(reqs), a.plain_sample)
     t0 = time.perf_counter()
     clock, plain_lat, errs = 0.0, [], []
-    for r in reqs[:n_plain]:
+    for i, r in enumerate(reqs[:n_plain]):
+        if i % 32 == 0:
+            print(f"# plain FP64 request {i}/{n_plain}", file=sys.stderr, flush=True)
         u, stn, wall, _ = solve_alone(r, api.FP64, fp64)
         clock = max(clock, r.arrival) + wall
         plain_lat.append(clock - r.arrival)

@@ -29,15 +29,17 @@ from . import api, batch
 
 
 def footprint_bytes(req, precision=api.MIXED) -> int:
-    """Device bytes of one request inside a batch context (upper bound):
-    per element the 300-entry slot map and colouring; per node the MSR
-    pattern/values (<= 41 upper entries), the 3x3-block operator (27 blocks
-    with map, FP64 and FP32 values) and the node vectors."""
+    """Device bytes of one request inside a batch context (upper bound for
+    a cube of L^3 hexes): per element the 300-entry MSR slot map, the 72-entry
+    operator slot map, connectivity, material and colour lists; per node the
+    upper MSR pattern and values (<= 123 entries), 27 operator blocks (column,
+    9 FP64 values, 9 FP32 values in mixed precision, 9-entry expansion map)
+    and about a dozen node-space vectors."""
     nel = req.L ** 3
-    nnd = (req.L + 1) ** 3 + 63
-    per_el = 300 * 4 + 8 * 4 + 8
-    per_node = 3 * 41 * (4 + 8 + (4 if precision == api.MIXED else 0)) + 27 * 64 // 64 * (4 + 9 * 4 + 9 * 8 + 9 * 4) \
-        + 3 * 8 * 12 + 3 * 4 * 8
+    nnd = (req.L + 1) ** 3 + 63          # padding to a 64-node boundary
+    per_el = 4 * (300 + 72 + 8 + 1 + 2)
+    val_b = 8 + (4 if precision == api.MIXED else 0)
+    per_node = 123 * (4 + val_b) + 27 * (4 + 9 * val_b + 9 * 4) + 12 * 3 * 8
     return nel * per_el + nnd * per_node
 
 
@@ -66,13 +68,14 @@ class ServerStats:
 
 class RVEServer:
     def __init__(self, precision=api.MIXED, workers=4, max_batch=256, budget_bytes=64 << 30, lin_rtol=None,
-                 device=0, keep_u=True):
+                 device=0, keep_u=True, on_batch=None):
         self.precision = precision
         self.workers = workers
         self.max_batch = max_batch
         self.budget = budget_bytes
         self.device = device
         self.keep_u = keep_u
+        self.on_batch = on_batch     # called as on_batch(batch_id, size, formulation, seconds) after each batch
         rt = lin_rtol if lin_rtol is not None else (1e-6 if precision == api.MIXED else 1e-12)
         self.opts = api.Solid.opts(lin_rtol=rt)
         self._lock = threading.Condition()
@@ -147,6 +150,8 @@ class RVEServer:
                 for x in res:
                     self.stats.results[x.rid] = x
                 self.stats.batches.append((bid, len(reqs), reqs[0].formulation, res[0].admitted, res[0].finished))
+            if self.on_batch:
+                self.on_batch(bid, len(reqs), reqs[0].formulation, res[0].finished - res[0].admitted)
 
     # -- driver ---------------------------------------------------------
     def run(self, requests, realtime=True):
