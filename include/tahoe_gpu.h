@@ -191,7 +191,9 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      the matrix footprint);
  * TG_TUNE_STIFF_SHAPE  block shape of the stiffness kernel: 0..3 for 16,
  *                      12, 8 or 4 elements per workgroup with element-major
- *                      lanes, 4..5 for 16 or 8 with block-major lanes;
+ *                      lanes, 4..5 for 16 or 8 with block-major lanes,
+ *                      6..8 for 16, 8 or 4 in the D*B form (D_q B shared
+ *                      by an element's lanes through LDS);
  * TG_TUNE_NEWTON_ASSEMBLY  where tg_newton / tg_newton_batch /
  *                      tg_newton_correction put the tangent: 1 (default)
  *                      straight into the inner solve's block operator, 0

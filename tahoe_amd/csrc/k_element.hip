@@ -166,8 +166,9 @@ __device__ __forceinline__ int upper_index(int m, int n) { return m * 24 - (m * 
 // barrier and scatter waits.  EPB (elements per block) and the phase-2 lane
 // order are template parameters; the autotuner picks among the compiled
 // shapes.
-constexpr int STIFF_EPB[] = {16, 12, 8, 4, 16, 8};
-constexpr bool STIFF_PM[] = {false, false, false, false, true, true};
+constexpr int STIFF_EPB[] = {16, 12, 8, 4, 16, 8, 16, 8, 4};
+constexpr bool STIFF_PM[] = {false, false, false, false, true, true, false, false, false};
+constexpr bool STIFF_DB[] = {false, false, false, false, false, false, true, true, true};
 constexpr int N_STIFF_SHAPES = sizeof(STIFF_EPB) / sizeof(STIFF_EPB[0]);
 
 // Voigt index of (i,k): (11,22,33,23,13,12)
@@ -182,6 +183,114 @@ __device__ __forceinline__ constexpr int sym6(int v, int w)
 // VT is the arithmetic of the 3x3 block products and of the assembled
 // values: double (FP64 path) or float (mixed path, integration-point
 // kinematics and material stay FP64 and are rounded once to FP32).
+// Integration point q of one element (TotalLagrangianT::FormStiffness's
+// per-IP data): g = dN/dx (ShapeFunctionT::TransformDerivatives), s c and
+// s sigma with s = det0 * J * w (w = 1), all in FP64; xu(a, c) reads node a's
+// X (c < 3) or u (c >= 3).  Returns false on J <= 0 or det0 <= 0 (the data
+// are then zero).  An invalid slot (no element) gives zeros.
+template <class XU>
+__device__ __forceinline__ bool stiffness_ip(int q, bool valid, double2 mk, XU xu, double (&g)[3][8],
+                                             double (&D)[21], double (&T)[6])
+{
+    bool bad = false;
+    double scale = 0.0, Jb = 1.0, bb[6] = {1, 1, 1, 0, 0, 0}, sig[6] = {0, 0, 0, 0, 0, 0};
+    if (valid) {
+        double d[3][8], j0[9], ji[9], g0[3][8], F[9], Fi[9];
+        parent_derivs(q, d);
+        jacobian([&](int a, int c) { return xu(a, c); }, d, j0);
+        const double det0 = det3(j0);
+        bad |= !(det0 > 0.0);
+        inv3(j0, det0, ji);
+        spatial_derivs(ji, d, g0);
+        deformation_gradient([&](int a, int c) { return xu(a, 3 + c); }, g0, F);
+        const double J = det3(F);
+        bad |= !(J > 0.0);
+        bad |= !simo_kinematics(F, Jb, bb);
+        simo_cauchy(mk.x, mk.y, Jb, bb, sig);
+        inv3(F, J, Fi);
+        scale = bad ? 0.0 : det0 * J;  // weight 1
+        // dN/dx_j = sum_k Fi(k,j) dN/dX_k
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                g[j][a] = Fi[0 + 3 * j] * g0[0][a] + Fi[1 + 3 * j] * g0[1][a] + Fi[2 + 3 * j] * g0[2][a];
+    } else {
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) g[j][a] = 0.0;
+    }
+    double cm[6][6];
+    simo_moduli(mk.x, mk.y, Jb, bb, cm);
+#pragma unroll
+    for (int v = 0; v < 6; v++)
+#pragma unroll
+        for (int w = v; w < 6; w++) D[sym6(v, w)] = scale * cm[v][w];
+#pragma unroll
+    for (int k = 0; k < 6; k++) T[k] = scale * sig[k];
+    return !bad;
+}
+
+// Scatter of one upper 3x3 node block (a <= b, local upper index p) of
+// element e: into the MSR values through the slot map (TO_OP = false,
+// MSRMatrixT::Assemble) or the block and its transpose into the solver
+// operator through the operator slot map (TO_OP = true, prescribed rows and
+// columns left zero).
+template <class VT, bool TO_OP>
+__device__ __forceinline__ void scatter_block(int e, int a, int b, int p, const VT (&K)[3][3],
+                                              const int* __restrict__ conn, const int* __restrict__ slot,
+                                              const int* __restrict__ eqn, VT* __restrict__ val)
+{
+    if (TO_OP) {
+        const int b1 = slot[72L * e + 2 * p], b2 = slot[72L * e + 2 * p + 1];
+        if (b1 < 0) return;
+        const int A = conn[8L * e + a], B = conn[8L * e + b];
+        bool ra[3], cb[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            ra[i] = eqn[3L * A + i] >= 0;
+            cb[i] = eqn[3L * B + i] >= 0;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                if (!(ra[i] && cb[j])) continue;
+                if (a == b) {
+                    // both triangles of a diagonal block from the upper value
+                    val[b1 + (3 * i + j) * 64] += K[i < j ? i : j][i < j ? j : i];
+                } else {
+                    val[b1 + (3 * i + j) * 64] += K[i][j];
+                    if (b2 >= 0) val[b2 + (3 * j + i) * 64] += K[i][j];
+                }
+            }
+    } else {
+        const int* es = slot + 300L * e;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int m = 3 * a + i, n = 3 * b + j;
+                if (n < m) continue;  // lower half of a diagonal block
+                const int s = es[upper_index(m, n)];
+                if (s >= 0) val[s] += K[i][j];
+            }
+    }
+}
+
+// upper block index p (row-major over the upper triangle of 8x8) -> (a, b)
+__device__ __forceinline__ void upper_block(int p, int& a, int& b)
+{
+    a = 0;
+    int rem = p;
+    while (rem >= 8 - a) {
+        rem -= 8 - a;
+        a++;
+    }
+    b = a + rem;
+}
+
 // TO_OP = false: upper entries into the MSR values through the slot map
 // (MSRMatrixT::Assemble).  TO_OP = true: the block and its transpose
 // straight into the solver's block operator through the operator slot map,
@@ -229,61 +338,24 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
     if (t < 8 * EPB) {
         const int le = t >> 3, q = t & 7;
         const bool valid = base + le < count;
-        double scale = 0.0, Jb = 1.0, bb[6] = {1, 1, 1, 0, 0, 0}, sig[6] = {0, 0, 0, 0, 0, 0};
-        double2 mk = make_double2(0.0, 0.0);
-        double g[3][8];
-        if (valid) {
-            mk = props[emat[elems[base + le]]];
-            double d[3][8], j0[9], ji[9], g0[3][8], F[9], Fi[9];
-            parent_derivs(q, d);
-            jacobian([&](int a, int c) { return sxu[le][a][c]; }, d, j0);
-            const double det0 = det3(j0);
-            bad |= !(det0 > 0.0);
-            inv3(j0, det0, ji);
-            spatial_derivs(ji, d, g0);
-            deformation_gradient([&](int a, int c) { return sxu[le][a][3 + c]; }, g0, F);
-            const double J = det3(F);
-            bad |= !(J > 0.0);
-            bad |= !simo_kinematics(F, Jb, bb);
-            simo_cauchy(mk.x, mk.y, Jb, bb, sig);
-            inv3(F, J, Fi);
-            scale = bad ? 0.0 : det0 * J;  // weight 1
-            // dN/dx_j = sum_k Fi(k,j) dN/dX_k (ShapeFunctionT::TransformDerivatives)
-#pragma unroll
-            for (int a = 0; a < 8; a++)
-#pragma unroll
-                for (int j = 0; j < 3; j++)
-                    g[j][a] = Fi[0 + 3 * j] * g0[0][a] + Fi[1 + 3 * j] * g0[1][a] + Fi[2 + 3 * j] * g0[2][a];
-        } else {
-#pragma unroll
-            for (int a = 0; a < 8; a++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) g[j][a] = 0.0;
-        }
+        const double2 mk = valid ? props[emat[elems[base + le]]] : make_double2(0.0, 0.0);
+        double g[3][8], D[21], T[6];
+        bad |= !stiffness_ip(q, valid, mk, [&](int a, int c) { return sxu[le][a][c]; }, g, D, T);
 #pragma unroll
         for (int a = 0; a < 8; a++)
 #pragma unroll
             for (int j = 0; j < 3; j++) sg[gi(le, q, j, a)] = (VT)g[j][a];
-        double cm[6][6];
-        simo_moduli(mk.x, mk.y, Jb, bb, cm);
 #pragma unroll
-        for (int v = 0; v < 6; v++)
+        for (int k = 0; k < 21; k++) sd[di(le, q, k)] = (VT)D[k];
 #pragma unroll
-            for (int w = v; w < 6; w++) sd[di(le, q, sym6(v, w))] = (VT)(scale * cm[v][w]);
-#pragma unroll
-        for (int k = 0; k < 6; k++) st[ti(le, q, k)] = (VT)(scale * sig[k]);
+        for (int k = 0; k < 6; k++) st[ti(le, q, k)] = (VT)T[k];
     }
     __syncthreads();
     // ---- phase 2: one 3x3 block per lane ----
     const int le = PM ? t % EPB : t / 36, p = PM ? t / EPB : t % 36;
     if (base + le < count) {
-        // p -> (a, b), a <= b, row-major over the upper triangle of 8x8
-        int a = 0, rem = p;
-        while (rem >= 8 - a) {
-            rem -= 8 - a;
-            a++;
-        }
-        const int b = a + rem;
+        int a, b;
+        upper_block(p, a, b);
         VT K[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -323,41 +395,7 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
 #pragma unroll
             for (int i = 0; i < 3; i++) K[i][i] += geo;
         }
-        const int e = elems[base + le];
-        if (TO_OP) {
-            const int b1 = slot[72L * e + 2 * p], b2 = slot[72L * e + 2 * p + 1];
-            const int A = conn[8L * e + a], B = conn[8L * e + b];
-            bool ra[3], cb[3];
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                ra[i] = eqn[3L * A + i] >= 0;
-                cb[i] = eqn[3L * B + i] >= 0;
-            }
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    if (!(ra[i] && cb[j]) || b1 < 0) continue;
-                    if (a == b) {
-                        // both triangles of a diagonal block from the upper value
-                        val[b1 + (3 * i + j) * 64] += K[i < j ? i : j][i < j ? j : i];
-                    } else {
-                        val[b1 + (3 * i + j) * 64] += K[i][j];
-                        if (b2 >= 0) val[b2 + (3 * j + i) * 64] += K[i][j];
-                    }
-                }
-        } else {
-            const int* es = slot + 300L * e;
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const int m = 3 * a + i, n = 3 * b + j;
-                    if (n < m) continue;  // lower half of a diagonal block
-                    const int s = es[upper_index(m, n)];
-                    if (s >= 0) val[s] += K[i][j];
-                }
-        }
+        scatter_block<VT, TO_OP>(elems[base + le], a, b, p, K, conn, slot, eqn, val);
     }
     if (bad) atomicOr(status, 7);
 }
@@ -414,26 +452,141 @@ static const std::vector<std::pair<int, int>>& stiffness_launches(tg_ctx* c)
     return c->stiff_launches;
 }
 
-template <class VT, int EPB, bool TO_OP, bool PM>
+// Element stiffness, "DB form": the same K_ab, accumulated as
+//   K_ab += B_a^T (D_q B_b) + (g_a . T_q g_b) 1
+// with the 6x24 product D_q B (and the 8 vectors T_q g_b) formed once per
+// element and integration point -- 36 lanes produce its 168 entries into
+// LDS -- instead of every lane recontracting D_q with its own g_b.  Per
+// element and point that is 168 x 3 + 36 x 33 FMA instead of 36 x 99, with
+// fewer live registers.  Same mapping otherwise: EPB elements per
+// workgroup, phase 1 on 8 lanes per element, lane (le, p) owns upper block
+// p.  Voigt rows of B for node b (11,22,33,23,13,12): u1 -> (g1,0,0,0,g3,g2),
+// u2 -> (0,g2,0,g3,0,g1), u3 -> (0,0,g3,g2,g1,0) (Hughes 2.8.21).
+template <class VT, int EPB, bool TO_OP>
+__global__ void __launch_bounds__(36 * EPB) k_stiffness_db(const int* __restrict__ elems, int count,
+                                                             const int* __restrict__ conn,
+                                                             const int* __restrict__ emat,
+                                                             const double2* __restrict__ props,
+                                                             const double* __restrict__ X,
+                                                             const double* __restrict__ U,
+                                                             const int* __restrict__ slot,
+                                                             const int* __restrict__ eqn, VT* __restrict__ val,
+                                                             int* __restrict__ status)
+{
+    constexpr int NDB = 6 * 24 + 8 * 3;  // D B (row-major 6 x 24), then T g_b (8 x 3)
+    __shared__ double sxu[EPB][8][6];
+    __shared__ VT sg[EPB][8][24];        // dN/dx [element][ip][dir * 8 + node]
+    __shared__ VT sd[EPB][8][21];        // s c, packed symmetric Voigt
+    __shared__ VT st[EPB][8][6];         // s sigma, Voigt
+    __shared__ VT sdb[EPB][NDB];
+    const int t = threadIdx.x;
+    const long base = (long)blockIdx.x * EPB;
+    bool bad = false;
+    if (t < 8 * EPB) {
+        const int le = t >> 3, q = t & 7;
+        if (base + le < count) {
+            const int node = conn[8L * elems[base + le] + q];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                sxu[le][q][c] = X[3L * node + c];
+                sxu[le][q][3 + c] = U[3L * node + c];
+            }
+        }
+    }
+    __syncthreads();
+    if (t < 8 * EPB) {
+        const int le = t >> 3, q = t & 7;
+        const bool valid = base + le < count;
+        const double2 mk = valid ? props[emat[elems[base + le]]] : make_double2(0.0, 0.0);
+        double g[3][8], D[21], T[6];
+        bad |= !stiffness_ip(q, valid, mk, [&](int a, int c) { return sxu[le][a][c]; }, g, D, T);
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int a = 0; a < 8; a++) sg[le][q][j * 8 + a] = (VT)g[j][a];
+#pragma unroll
+        for (int k = 0; k < 21; k++) sd[le][q][k] = (VT)D[k];
+#pragma unroll
+        for (int k = 0; k < 6; k++) st[le][q][k] = (VT)T[k];
+    }
+    __syncthreads();
+    const int le = t / 36, p = t % 36;
+    int a, b;
+    upper_block(p, a, b);
+    VT K[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) K[i][j] = 0;
+#pragma unroll 1
+    for (int q = 0; q < 8; q++) {
+        // D_q B and T_q g_b of element le: entries p, p+36, ...
+        for (int o = p; o < NDB; o += 36) {
+            VT v;
+            if (o < 144) {
+                const int vv = o / 24, n = o % 24, bn = n / 3, j = n % 3;
+                v = sd[le][q][sym6(vv, voigt(j, 0))] * sg[le][q][bn] +
+                    sd[le][q][sym6(vv, voigt(j, 1))] * sg[le][q][8 + bn] +
+                    sd[le][q][sym6(vv, voigt(j, 2))] * sg[le][q][16 + bn];
+            } else {
+                const int m = o - 144, bn = m / 3, k = m % 3;
+                v = st[le][q][voigt(k, 0)] * sg[le][q][bn] + st[le][q][voigt(k, 1)] * sg[le][q][8 + bn] +
+                    st[le][q][voigt(k, 2)] * sg[le][q][16 + bn];
+            }
+            sdb[le][o] = v;
+        }
+        __syncthreads();
+        VT ga[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) ga[k] = sg[le][q][k * 8 + a];
+        VT geo = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) geo += ga[k] * sdb[le][144 + 3 * b + k];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                VT x = 0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) x += ga[k] * sdb[le][voigt(i, k) * 24 + 3 * b + j];
+                K[i][j] += x;
+            }
+#pragma unroll
+        for (int i = 0; i < 3; i++) K[i][i] += geo;
+        __syncthreads();
+    }
+    if (base + le < count) scatter_block<VT, TO_OP>(elems[base + le], a, b, p, K, conn, slot, eqn, val);
+    if (bad) atomicOr(status, 7);
+}
+
+template <class VT, int EPB, bool TO_OP, bool PM, bool DB>
 static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u, VT* val)
 {
     const int blocks = (count + EPB - 1) / EPB;
-    hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
-                       c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
-                       TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
+    if (DB)
+        hipLaunchKernelGGL((k_stiffness_db<VT, EPB, TO_OP>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
+                           c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                           TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
+    else
+        hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
+                           c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                           TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
 }
 
 template <class VT, bool TO_OP>
 static void launch_stiffness_shape(tg_ctx* c, int v, int first, int count, const double* u, VT* val)
 {
-#define TG_SHAPE(i) launch_stiffness_v<VT, STIFF_EPB[i], TO_OP, STIFF_PM[i]>(c, first, count, u, val)
+#define TG_SHAPE(i) launch_stiffness_v<VT, STIFF_EPB[i], TO_OP, STIFF_PM[i], STIFF_DB[i]>(c, first, count, u, val)
     switch (v) {
     case 0: TG_SHAPE(0); break;
     case 1: TG_SHAPE(1); break;
     case 2: TG_SHAPE(2); break;
     case 3: TG_SHAPE(3); break;
     case 4: TG_SHAPE(4); break;
-    default: TG_SHAPE(5); break;
+    case 5: TG_SHAPE(5); break;
+    case 6: TG_SHAPE(6); break;
+    case 7: TG_SHAPE(7); break;
+    default: TG_SHAPE(8); break;
     }
 #undef TG_SHAPE
 }
