@@ -194,7 +194,7 @@ def main():
     spmv_avg_ms = tm["spmv_ms"] / max(1, tm["spmv_launches"])
     nblocks = (n + 1 + 2 * n) ** 3  # (3n+1)^3 node-neighbour blocks of the structured cube
     vbytes = 8 if a.precision == "fp64" else 4
-    spmv_bytes = nblocks * (9 * vbytes + 4) + 2 * 3 * vbytes * m.nnd
+    spmv_bytes = nblocks * (9 * vbytes + 4) + 2 * 3 * 8 * m.nnd  # operator values + columns, x and y (FP64)
     spmv_gbs = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9
     res_gflops = RES_FLOP_PER_EL * m.nel / (r_ms * 1e-3) / 1e9
     stf_gflops = STIFF_FLOP_PER_EL * m.nel / (k_ms * 1e-3) / 1e9
@@ -221,7 +221,8 @@ def main():
                    "stiffness_chunk": tuned["chunk"]},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": spmv_gbs / (HBM_PEAK / 1e9), "traffic": traffic_from_profiles(),
-                     "kernel": "k_spmv<true> (block sliced-ELL SpMV + p.q)",
+                     "kernel": f"k_seg_spmv<{'double' if a.precision == 'fp64' else 'float'}> "
+                               "(3x3-block sliced-ELL SpMV + per-slice p.q partials)",
                      "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_avg_ms},
         "cpu_baseline": cpu_baseline(a.cpu_n) if a.cpu_n > 0 else None,
         "newton_solves_per_s": 1e3 / ms_per_step,
