@@ -65,6 +65,7 @@ def parse_args():
     ap.add_argument("--serve-load", type=float, default=0.5, help="offered load as a fraction of capacity")
     ap.add_argument("--stiff-chunk", type=int, default=None,
                     help="elements per range of the colour-ordered stiffness launches (default: autotuned)")
+    ap.add_argument("--spmv-xcd", type=int, default=None, help="PCG SpMV XCD row mapping 0/1 (default: autotuned)")
     ap.add_argument("--stiff-shape", type=int, default=None,
                     help="stiffness workgroup shape 0..5 (default: autotuned)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -150,6 +151,20 @@ def main():
           file=sys.stderr, flush=True)
     opts = s.opts(lin_rtol=a.lin_rtol, lin_maxit=48 if a.profile else 200000)
     t_load = 0.2
+    if a.spmv_xcd is not None:
+        s.set_tuning(api.TUNE_SPMV_XCD, a.spmv_xcd)
+        tuned_spmv = {"xcd": a.spmv_xcd, "ms": {}}
+    else:
+        from tahoe_amd import autotune
+        short = s.opts(lin_rtol=a.lin_rtol, lin_maxit=32)
+
+        def short_pcg():
+            s.reset_step()
+            s.apply_kbc(t_load)
+            s.newton_correction(short)
+
+        tuned_spmv = autotune.tune_spmv_xcd(s, short_pcg)
+    print(f"# SpMV XCD mapping {tuned_spmv['xcd']} (autotune ms: {tuned_spmv['ms']})", file=sys.stderr, flush=True)
 
     def step():
         s.reset_step()
@@ -218,7 +233,7 @@ def main():
                                f"{'block-' if a.precond == 'block' else ''}Jacobi-PCG to {a.lin_rtol:g}",
                    "elements": m.nel, "nodes": m.nnd, "equations": n_eq, "msr_upper_nnz": msr_len - 1,
                    "stiffness_shape_elements_per_workgroup": api.STIFF_SHAPE_EPB[tuned["shape"]],
-                   "stiffness_chunk": tuned["chunk"]},
+                   "stiffness_chunk": tuned["chunk"], "spmv_xcd_rows": tuned_spmv["xcd"]},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": spmv_gbs / (HBM_PEAK / 1e9), "traffic": traffic_from_profiles(),
                      "kernel": f"k_seg_spmv<{'double' if a.precision == 'fp64' else 'float'}> "
@@ -235,6 +250,7 @@ def main():
                 "expand_ms": tm["expand_ms"] / K},
         "timing_ms": tm,
         "autotune_stiffness_ms": {k: {str(x): y for x, y in v.items()} for k, v in tuned["ms"].items()},
+        "autotune_spmv_ms": {str(k): v for k, v in tuned_spmv["ms"].items()},
     }
     print(json.dumps(line), flush=True)
 

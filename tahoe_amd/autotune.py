@@ -65,3 +65,26 @@ def tune_stiffness(solid: api.Solid, shapes=None, chunks=None, reps: int = 2) ->
     finally:
         solid.timing_enable(False)
     return {"shape": best_shape, "chunk": best_chunk, "ms": {"shape": t_shape, "chunk": t_chunk}}
+
+
+def tune_spmv_xcd(solid: api.Solid, run_pcg) -> dict:
+    """Time the PCG SpMV with hardware workgroup order and with XCD-contiguous
+    row ranges; run_pcg() must run a short PCG (e.g. one Newton correction
+    with a small iteration cap) from a fixed state.  Sets the faster."""
+    times = {}
+    solid.timing_enable(True)
+    try:
+        for v in (0, 1):
+            solid.set_tuning(api.TUNE_SPMV_XCD, v)
+            run_pcg()
+            solid.synchronize()
+            solid.timing_reset()
+            run_pcg()
+            solid.synchronize()
+            t = solid.timing()
+            times[v] = t["spmv_ms"] / max(1, t["spmv_launches"])
+    finally:
+        solid.timing_enable(False)
+    best = min(times, key=times.get)
+    solid.set_tuning(api.TUNE_SPMV_XCD, best)
+    return {"xcd": best, "ms": times}

@@ -130,10 +130,23 @@ __global__ void __launch_bounds__(FB) k_seg_fin_init(const double* __restrict__ 
     }
 }
 
+// Workgroup -> rows.  Workgroups are dealt to the 8 XCDs round-robin, and
+// each XCD has its own L2; with XCD = true workgroup w takes the row range
+// (w % 8) * (G / 8) + w / 8 (G a multiple of 8), so each XCD sweeps one
+// contiguous eighth of the rows and the x entries its rows gather (the
+// neighbouring node layers) stay in its L2.
+template <bool XCD>
+__device__ __forceinline__ int row_block()
+{
+    if (!XCD) return blockIdx.x;
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
 // q = A p on the block operator, with per-slice partials of p.q.  MT is the
 // storage type of the operator values (double, or float in mixed
 // precision); products and sums are FP64.
-template <class MT>
+template <class MT, bool XCD>
 __global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict__ slice_seg,
                                                  const int* __restrict__ sbase, const int* __restrict__ swidth,
                                                  const int* __restrict__ bcol, const MT* __restrict__ bval,
@@ -141,7 +154,7 @@ __global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict_
                                                  double* __restrict__ part, long nslices, int nseg,
                                                  const double* __restrict__ sc)
 {
-    const int a = blockIdx.x * PB + threadIdx.x;
+    const int a = row_block<XCD>() * PB + threadIdx.x;
     const int s = a >> 6, l = a & 63;
     if (a >= 64L * nslices) return;
     const int g = slice_seg[s];  // wave-uniform
@@ -345,9 +358,14 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
         for (int k = 0; k < CHUNK; k++) {
             {
                 TimedRegion tr(c, T_SPMV);
-                hipLaunchKernelGGL(k_seg_spmv<MT>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p,
-                                   c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, p, q, S.d_part.p,
-                                   ns, nseg, S.d_scal.p);
+                if (c->spmv_xcd)
+                    hipLaunchKernelGGL((k_seg_spmv<MT, true>), dim3((nb + 7) / 8 * 8), dim3(PB), 0, c->stream, c->nnd,
+                                       S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p,
+                                       val, p, q, S.d_part.p, ns, nseg, S.d_scal.p);
+                else
+                    hipLaunchKernelGGL((k_seg_spmv<MT, false>), dim3(nb), dim3(PB), 0, c->stream, c->nnd,
+                                       S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p,
+                                       val, p, q, S.d_part.p, ns, nseg, S.d_scal.p);
             }
             {
                 TimedRegion tr(c, T_VECTOR);
@@ -369,7 +387,8 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
     const bool graph = c->pcg_graph && !c->timing.enabled;
     if (graph) {
         const void* key[12] = {val, dinv, x, r, p, q, S.d_part.p, S.d_scal.p, S.d_ndone.p,
-                               (const void*)(long)PC, (const void*)(long)sizeof(MT), (const void*)(long)c->nnd};
+                               (const void*)(long)(PC + 2 * c->spmv_xcd), (const void*)(long)sizeof(MT),
+                               (const void*)(long)c->nnd};
         if (!c->pg.exec || std::memcmp(key, c->pg.key, sizeof key) != 0) {
             if (c->pg.exec) TG_HIP(hipGraphExecDestroy(c->pg.exec));
             if (c->pg.graph) TG_HIP(hipGraphDestroy(c->pg.graph));

@@ -796,6 +796,9 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
         } else if (knob == TG_TUNE_PCG_GRAPH) {
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "PCG graph must be 0 or 1");
             c->pcg_graph = (int)value;
+        } else if (knob == TG_TUNE_SPMV_XCD) {
+            if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "SpMV XCD mapping must be 0 or 1");
+            c->spmv_xcd = (int)value;
         } else
             fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
     });
@@ -813,6 +816,8 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
             *value = c->newton_assembly;
         else if (knob == TG_TUNE_PCG_GRAPH)
             *value = c->pcg_graph;
+        else if (knob == TG_TUNE_SPMV_XCD)
+            *value = c->spmv_xcd;
         else
             fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
     });

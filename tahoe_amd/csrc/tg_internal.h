@@ -125,6 +125,7 @@ struct tg_ctx {
     int newton_assembly = 1;            // 1: Newton's K straight into the solver operator, 0: via MSR
     int precond = TG_PC_JACOBI;         // preconditioner of the inner PCG
     int pcg_graph = 1;                  // replay PCG chunks as a captured HIP graph
+    int spmv_xcd = 0;                   // XCD-contiguous row ranges in the PCG SpMV
     struct {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;

@@ -228,3 +228,23 @@ def test_block_jacobi_newton(name):
         assert list(iters) == g["step_iters"].tolist()
         its[pc] = sum(lin)
     assert its[api.PC_BLOCK_JACOBI] < its[api.PC_JACOBI], its
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_spmv_xcd_mapping_bitwise(precision):
+    """XCD-contiguous row ranges in the PCG SpMV change only which
+    workgroup computes which rows: the Newton correction is bitwise equal."""
+    from tahoe_amd import api
+    g = load_golden("rve_n4")
+    out = []
+    for xcd in (0, 1):
+        s = make_solid(g)
+        if precision:
+            s.set_precision(api.MIXED)
+            s.initialize()
+        s.set_tuning(api.TUNE_SPMV_XCD, xcd)
+        s.apply_kbc(1.0)
+        st = s.newton_correction(s.opts(lin_rtol=1e-10))
+        out.append((s.displacement(), st.lin_iterations))
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])
