@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -70,11 +71,16 @@ class TahoeGpuError(RuntimeError):
 
 
 _lib = None
+_lib_lock = threading.Lock()
 
 
 def lib():
     global _lib
-    if _lib is None:
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m tahoe_amd.build`")
         L = C.CDLL(LIB_PATH)
