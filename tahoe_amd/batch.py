@@ -23,6 +23,7 @@ class Problem:
     kbc: KBC
     E: list      # Young's modulus per local material id
     nu: list     # Poisson ratio per local material id
+    u0: np.ndarray = None  # optional predictor [nnd][3] at the full load (e.g. the affine field u = H X)
 
 
 @dataclass
@@ -34,12 +35,13 @@ class Packed:
     node_begin: np.ndarray   # [nseg+1] first node of each sub-problem (padded)
     node_count: np.ndarray   # [nseg] real nodes of each sub-problem
     materials: list          # [(E, nu)] global table
+    u0: np.ndarray = None    # [nnd][3] predictor at the full load, or None
 
 
 def pack(problems: list) -> Packed:
     mats = []
     index = {}
-    Xs, conns, matl, kn, kd, kv = [], [], [], [], [], []
+    Xs, conns, matl, kn, kd, kv, u0s = [], [], [], [], [], [], []
     begins = [0]
     counts = []
     off = 0
@@ -57,6 +59,10 @@ def pack(problems: list) -> Packed:
         X = np.zeros((nnd + pad, 3))
         X[:nnd] = p.mesh.X
         Xs.append(X)
+        u0 = np.zeros((nnd + pad, 3))
+        if p.u0 is not None:
+            u0[:nnd] = p.u0
+        u0s.append(u0)
         conns.append(p.mesh.conn + off)
         matl.append(gid[p.mesh.mat])
         kn.append(p.kbc.nodes + off)
@@ -71,9 +77,11 @@ def pack(problems: list) -> Packed:
         off += nnd + pad
         begins.append(off)
     # the last sub-problem needs no trailing padding, but keeping it is harmless
+    any_u0 = any(p.u0 is not None for p in problems)
     return Packed(np.concatenate(Xs), np.concatenate(conns).astype(np.int32), np.concatenate(matl).astype(np.int32),
                   KBC(np.concatenate(kn), np.concatenate(kd), np.concatenate(kv)),
-                  np.asarray(begins, dtype=np.int32), np.asarray(counts, dtype=np.int32), mats)
+                  np.asarray(begins, dtype=np.int32), np.asarray(counts, dtype=np.int32), mats,
+                  np.concatenate(u0s) if any_u0 else None)
 
 
 def make_solid(pk: Packed, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64, device=0) -> api.Solid:
@@ -96,8 +104,14 @@ def unpack(pk: Packed, u: np.ndarray) -> list:
 def static_solve_batch(s: api.Solid, pk: Packed, num_steps: int, opts=None):
     """FEManagerT::Solve's static loop for every sub-problem at once; returns
     per-problem converged displacements of the last step and the per-step
-    Newton statistics [step][problem]."""
+    Newton statistics [step][problem].  With predictors (Problem.u0) the
+    first step starts from u0 / num_steps instead of zero; the reference
+    starts every run from zero, so this changes the Newton path, not the
+    converged displacements."""
     stats = []
+    if pk.u0 is not None:
+        # predictor for the first load step, scaled like the prescribed values
+        s.set_displacement(pk.u0 / num_steps)
     for k in range(1, num_steps + 1):
         s.apply_kbc(k / num_steps)
         st = s.newton_batch(opts)

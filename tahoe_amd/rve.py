@@ -4,7 +4,9 @@ solves.
 Every request is a unit cube of L^3 voxel hexahedra, two-phase
 neo-Hookean (phase 2 with probability p2 per element, stiffness ratio
 `ratio`), loaded by the affine boundary displacement u = H X on all outer
-nodes with H_ij ~ U(-hmax, hmax).  Draws come from the counter-based
+nodes with H_ij ~ U(-hmax, hmax).  The Newton predictor is the affine field
+u = H X on every node (the Taylor assumption of FE^2 homogenisation), which
+keeps the near-incompressible Q1P0 requests robust in a single load step.  Draws come from the counter-based
 SplitMix64 of tahoe_amd.mesh with one stream per request.
 """
 from __future__ import annotations
@@ -35,7 +37,8 @@ class Request:
         m.mat = ph.astype(np.int32)
         hu = M.uniform01(self.seed, np.uint64(1 << 40) + np.arange(9, dtype=np.uint64))
         H = ((2.0 * hu - 1.0) * hmax).reshape(3, 3)
-        return batch.Problem(m, M.kbc_affine(m, H), [E1, ratio * E1], [self.nu, self.nu])
+        # predictor: the affine field u = H X everywhere (exact for one phase)
+        return batch.Problem(m, M.kbc_affine(m, H), [E1, ratio * E1], [self.nu, self.nu], m.X @ H.T)
 
 
 def log_uniform_edges(n: int, lo: int, hi: int, seed: int) -> np.ndarray:

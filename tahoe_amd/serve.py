@@ -53,6 +53,8 @@ class Result:
     finished: float
     batch_id: int
     batch_size: int
+    status: int = 0          # 0 converged; otherwise the ExceptionT code / Newton failure of the last attempt
+    load_steps: int = 1
 
     @property
     def latency(self) -> float:
@@ -67,6 +69,8 @@ class ServerStats:
 
 
 class RVEServer:
+    RETRY_STEPS = 10   # load steps for a request that fails in one step
+
     def __init__(self, precision=api.MIXED, workers=4, max_batch=256, budget_bytes=64 << 30, lin_rtol=None,
                  device=0, keep_u=True, on_batch=None):
         self.precision = precision
@@ -116,17 +120,42 @@ class RVEServer:
         return bid, picked, size
 
     # -- worker ---------------------------------------------------------
-    def _solve(self, bid, reqs, t0):
-        t_adm = time.perf_counter() - t0
+    def _solve_group(self, reqs, steps):
+        """Solve reqs as one batch context; raises TahoeGpuError when the
+        batch fails (any sub-problem with J <= 0 or a failed Newton)."""
         pk = batch.pack([r.problem() for r in reqs])
         s = batch.make_solid(pk, formulation=reqs[0].formulation, precision=self.precision, device=self.device)
-        us, st = batch.static_solve_batch(s, pk, 1, self.opts)
-        s.close()
+        try:
+            us, st = batch.static_solve_batch(s, pk, steps, self.opts)
+        finally:
+            s.close()
+        return us, [x.iterations for x in st[-1]]
+
+    def _solve(self, bid, reqs, t0):
+        """A mini-batch; when it fails, each request is solved alone, and a
+        request that still fails gets RETRY_STEPS load steps before it is
+        reported as failed (status != 0, u None).  A worker never dies on a
+        bad request."""
+        t_adm = time.perf_counter() - t0
+        done = {}
+        try:
+            us, its = self._solve_group(reqs, 1)
+            done = {r.rid: (u, n, 0, 1) for r, u, n in zip(reqs, us, its)}
+        except api.TahoeGpuError:
+            for r in reqs:
+                for steps in ((1, self.RETRY_STEPS) if len(reqs) > 1 else (self.RETRY_STEPS,)):
+                    try:
+                        us, its = self._solve_group([r], steps)
+                        done[r.rid] = (us[0], its[0], 0, steps)
+                        break
+                    except api.TahoeGpuError as e:
+                        done[r.rid] = (None, 0, e.code, steps)
         t_end = time.perf_counter() - t0
         out = []
-        for i, r in enumerate(reqs):
-            out.append(Result(r.rid, us[i] if self.keep_u else None, st[0][i].iterations, r.arrival, t_adm, t_end,
-                              bid, len(reqs)))
+        for r in reqs:
+            u, n, code, steps = done[r.rid]
+            out.append(Result(r.rid, u if self.keep_u else None, n, r.arrival, t_adm, t_end, bid, len(reqs), code,
+                              steps))
         return out
 
     def _worker(self, t0):
@@ -179,7 +208,10 @@ class RVEServer:
 
 
 def latency_summary(stats: ServerStats) -> dict:
-    lat = np.array([r.latency for r in stats.results.values()])
-    return {"requests": int(len(lat)), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "mean_s": float(lat.mean()), "solves_per_s": len(lat) / stats.wall, "wall_s": stats.wall,
+    ok = [r for r in stats.results.values() if r.status == 0]
+    lat = np.array([r.latency for r in ok])
+    return {"requests": len(stats.results), "converged": len(ok), "failed": len(stats.results) - len(ok),
+            "retried_with_load_steps": sum(1 for r in ok if r.load_steps > 1),
+            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "solves_per_s": len(ok) / stats.wall, "wall_s": stats.wall,
             "batches": len(stats.batches)}
