@@ -65,7 +65,7 @@ def parse_args():
     ap.add_argument("--serve-load", type=float, default=0.5, help="offered load as a fraction of capacity")
     ap.add_argument("--stiff-chunk", type=int, default=None,
                     help="elements per range of the colour-ordered stiffness launches (default: autotuned)")
-    ap.add_argument("--spmv-xcd", type=int, default=None, help="PCG SpMV XCD row mapping 0/1 (default: autotuned)")
+    ap.add_argument("--spmv-xcd", type=int, default=None, help="PCG SpMV variant 0..3 (default: autotuned)")
     ap.add_argument("--stiff-shape", type=int, default=None,
                     help="stiffness workgroup shape 0..5 (default: autotuned)")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -208,8 +208,10 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      of 80, for small, launch-bound problems); 0: direct
  *                      launches.  Per-kernel timing (tg_timing_enable)
  *                      always uses direct launches. */
-/* TG_TUNE_SPMV_XCD     1: the PCG SpMV deals each XCD (own L2) one contiguous
- *                      eighth of the rows; 0 (default): hardware order. */
+/* TG_TUNE_SPMV_XCD     PCG SpMV variant, bit 0: each XCD (own L2) takes one
+ *                      contiguous eighth of the rows instead of the hardware
+ *                      order; bit 1: branch-free inner loop unrolled by 3
+ *                      (padding slots multiply zeros).  0 (default) .. 3. */
 enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3, TG_TUNE_PCG_GRAPH = 4,
        TG_TUNE_SPMV_XCD = 5 };
 int tg_set_tuning(tg_ctx* ctx, int knob, long value);

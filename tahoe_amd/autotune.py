@@ -68,13 +68,14 @@ def tune_stiffness(solid: api.Solid, shapes=None, chunks=None, reps: int = 2) ->
 
 
 def tune_spmv_xcd(solid: api.Solid, run_pcg) -> dict:
-    """Time the PCG SpMV with hardware workgroup order and with XCD-contiguous
-    row ranges; run_pcg() must run a short PCG (e.g. one Newton correction
-    with a small iteration cap) from a fixed state.  Sets the faster."""
+    """Time the four PCG SpMV variants (hardware or XCD-contiguous row order,
+    plain or branch-free unrolled inner loop); run_pcg() must run a short PCG
+    (e.g. one Newton correction with a small iteration cap) from a fixed
+    state.  Sets the fastest."""
     times = {}
     solid.timing_enable(True)
     try:
-        for v in (0, 1):
+        for v in (0, 1, 2, 3):
             solid.set_tuning(api.TUNE_SPMV_XCD, v)
             run_pcg()
             solid.synchronize()

@@ -146,7 +146,7 @@ __device__ __forceinline__ int row_block()
 // q = A p on the block operator, with per-slice partials of p.q.  MT is the
 // storage type of the operator values (double, or float in mixed
 // precision); products and sums are FP64.
-template <class MT, bool XCD>
+template <class MT, bool XCD, bool UNR>
 __global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict__ slice_seg,
                                                  const int* __restrict__ sbase, const int* __restrict__ swidth,
                                                  const int* __restrict__ bcol, const MT* __restrict__ bval,
@@ -164,15 +164,33 @@ __global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict_
         const long b0 = sbase[s];
         const int w = swidth[s];
         double y0 = 0, y1 = 0, y2 = 0;
-        for (int k = 0; k < w; k++) {
-            const long blk = b0 + k;
-            const int c = bcol[blk * 64 + l];
-            if (c < 0) continue;
-            const double x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
-            const MT* v = bval + blk * 9 * 64 + l;
-            y0 += (double)v[0 * 64] * x0 + (double)v[1 * 64] * x1 + (double)v[2 * 64] * x2;
-            y1 += (double)v[3 * 64] * x0 + (double)v[4 * 64] * x1 + (double)v[5 * 64] * x2;
-            y2 += (double)v[6 * 64] * x0 + (double)v[7 * 64] * x1 + (double)v[8 * 64] * x2;
+        if (UNR) {
+            // padding slots (column -1) hold zero values: they multiply the
+            // row's own x, which keeps the loop branch-free so its loads can
+            // be issued ahead; adding the zero products leaves the sums
+            // unchanged
+#pragma unroll 3
+            for (int k = 0; k < w; k++) {
+                const long blk = b0 + k;
+                const int cr = bcol[blk * 64 + l];
+                const int c = cr < 0 ? a : cr;
+                const double x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
+                const MT* v = bval + blk * 9 * 64 + l;
+                y0 += (double)v[0 * 64] * x0 + (double)v[1 * 64] * x1 + (double)v[2 * 64] * x2;
+                y1 += (double)v[3 * 64] * x0 + (double)v[4 * 64] * x1 + (double)v[5 * 64] * x2;
+                y2 += (double)v[6 * 64] * x0 + (double)v[7 * 64] * x1 + (double)v[8 * 64] * x2;
+            }
+        } else {
+            for (int k = 0; k < w; k++) {
+                const long blk = b0 + k;
+                const int c = bcol[blk * 64 + l];
+                if (c < 0) continue;
+                const double x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
+                const MT* v = bval + blk * 9 * 64 + l;
+                y0 += (double)v[0 * 64] * x0 + (double)v[1 * 64] * x1 + (double)v[2 * 64] * x2;
+                y1 += (double)v[3 * 64] * x0 + (double)v[4 * 64] * x1 + (double)v[5 * 64] * x2;
+                y2 += (double)v[6 * 64] * x0 + (double)v[7 * 64] * x1 + (double)v[8 * 64] * x2;
+            }
         }
         y[3L * a] = y0;
         y[3L * a + 1] = y1;
@@ -358,14 +376,17 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
         for (int k = 0; k < CHUNK; k++) {
             {
                 TimedRegion tr(c, T_SPMV);
-                if (c->spmv_xcd)
-                    hipLaunchKernelGGL((k_seg_spmv<MT, true>), dim3((nb + 7) / 8 * 8), dim3(PB), 0, c->stream, c->nnd,
-                                       S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p,
-                                       val, p, q, S.d_part.p, ns, nseg, S.d_scal.p);
-                else
-                    hipLaunchKernelGGL((k_seg_spmv<MT, false>), dim3(nb), dim3(PB), 0, c->stream, c->nnd,
-                                       S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p,
-                                       val, p, q, S.d_part.p, ns, nseg, S.d_scal.p);
+#define TG_SPMV(X, U)                                                                                      \
+    hipLaunchKernelGGL((k_seg_spmv<MT, X, U>), dim3(X ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream, c->nnd, \
+                       S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, p, q,  \
+                       S.d_part.p, ns, nseg, S.d_scal.p)
+                switch (c->spmv_xcd) {
+                case 0: TG_SPMV(false, false); break;
+                case 1: TG_SPMV(true, false); break;
+                case 2: TG_SPMV(false, true); break;
+                default: TG_SPMV(true, true); break;
+                }
+#undef TG_SPMV
             }
             {
                 TimedRegion tr(c, T_VECTOR);

@@ -797,7 +797,7 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "PCG graph must be 0 or 1");
             c->pcg_graph = (int)value;
         } else if (knob == TG_TUNE_SPMV_XCD) {
-            if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "SpMV XCD mapping must be 0 or 1");
+            if (value < 0 || value > 3) fail(TG_OUT_OF_RANGE, "SpMV variant must be 0..3");
             c->spmv_xcd = (int)value;
         } else
             fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
