@@ -121,3 +121,42 @@ def static_solve_batch(s: api.Solid, pk: Packed, num_steps: int, opts=None):
         s.close_step()
         stats.append(st)
     return unpack(pk, s.displacement()), stats
+
+
+def solve_adaptive(s: api.Solid, pk: Packed, opts=None, min_dt=1.0 / 1024):
+    """The static load path to t = 1 with step cutting: a step that fails
+    (J <= 0 or Newton failure in any sub-problem) is reset and retried at
+    half the increment, a converged step doubles the next one
+    (FEManagerT's time-step cutting on a failed solve, FEManagerT.cpp:243-266,
+    with ResetStep).  Each step starts from the last converged state plus the
+    increment of the predictor (Problem.u0) when one is given.  Returns the
+    per-problem displacements, the number of converged steps and the Newton
+    corrections summed per problem."""
+    t, dt, steps = 0.0, 1.0, 0
+    newton = np.zeros(len(pk.node_count), dtype=np.int64)
+    u = s.displacement()
+    while t < 1.0 - 1e-12:
+        dt = min(dt, 1.0 - t)
+        if pk.u0 is not None:
+            s.set_displacement(u + dt * pk.u0)
+        s.apply_kbc(t + dt)
+        try:
+            st = s.newton_batch(opts)
+            ok = all(x.status == 0 for x in st)
+        except api.TahoeGpuError as e:
+            if e.code != 7:
+                raise
+            ok, st = False, None
+        if ok:
+            s.close_step()
+            u = s.displacement()
+            t += dt
+            steps += 1
+            newton += np.array([x.iterations for x in st])
+            dt *= 2.0
+        else:
+            s.reset_step()
+            dt *= 0.5
+            if dt < min_dt:
+                raise api.TahoeGpuError(1, f"load step below {min_dt} at t = {t}")
+    return unpack(pk, s.displacement()), steps, newton

@@ -68,9 +68,30 @@ class ServerStats:
     wall: float = 0.0
 
 
-class RVEServer:
-    RETRY_STEPS = 10   # load steps for a request that fails in one step
+def solve_request(req, precision, opts, device=0):
+    """One request on its own context: one load step from the affine
+    predictor, and when that fails the adaptive load path (batch.solve_adaptive).
+    Returns (u, Newton corrections, load steps).  A request's result does not
+    depend on which batch, if any, it was served in."""
+    pk = batch.pack([req.problem()])
+    s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device)
+    try:
+        try:
+            us, st = batch.static_solve_batch(s, pk, 1, opts)
+            return us[0], st[-1][0].iterations, 1
+        except api.TahoeGpuError:
+            pass
+    finally:
+        s.close()
+    s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device)
+    try:
+        us, steps, newton = batch.solve_adaptive(s, pk, opts)
+    finally:
+        s.close()
+    return us[0], int(newton[0]), steps
 
+
+class RVEServer:
     def __init__(self, precision=api.MIXED, workers=4, max_batch=256, budget_bytes=64 << 30, lin_rtol=None,
                  device=0, keep_u=True, on_batch=None):
         self.precision = precision
@@ -120,36 +141,33 @@ class RVEServer:
         return bid, picked, size
 
     # -- worker ---------------------------------------------------------
-    def _solve_group(self, reqs, steps):
-        """Solve reqs as one batch context; raises TahoeGpuError when the
-        batch fails (any sub-problem with J <= 0 or a failed Newton)."""
+    def _solve_group(self, reqs):
+        """Solve reqs as one batch context in one load step (from the affine
+        predictor); raises TahoeGpuError when any sub-problem fails."""
         pk = batch.pack([r.problem() for r in reqs])
         s = batch.make_solid(pk, formulation=reqs[0].formulation, precision=self.precision, device=self.device)
         try:
-            us, st = batch.static_solve_batch(s, pk, steps, self.opts)
+            us, st = batch.static_solve_batch(s, pk, 1, self.opts)
         finally:
             s.close()
         return us, [x.iterations for x in st[-1]]
 
     def _solve(self, bid, reqs, t0):
-        """A mini-batch; when it fails, each request is solved alone, and a
-        request that still fails gets RETRY_STEPS load steps before it is
-        reported as failed (status != 0, u None).  A worker never dies on a
-        bad request."""
+        """A mini-batch in one load step; when it fails, each request goes
+        through solve_request on its own.  A request that cannot be solved is
+        reported with its status and u None; a worker never dies on one."""
         t_adm = time.perf_counter() - t0
         done = {}
         try:
-            us, its = self._solve_group(reqs, 1)
+            us, its = self._solve_group(reqs)
             done = {r.rid: (u, n, 0, 1) for r, u, n in zip(reqs, us, its)}
         except api.TahoeGpuError:
             for r in reqs:
-                for steps in ((1, self.RETRY_STEPS) if len(reqs) > 1 else (self.RETRY_STEPS,)):
-                    try:
-                        us, its = self._solve_group([r], steps)
-                        done[r.rid] = (us[0], its[0], 0, steps)
-                        break
-                    except api.TahoeGpuError as e:
-                        done[r.rid] = (None, 0, e.code, steps)
+                try:
+                    u, n, steps = solve_request(r, self.precision, self.opts, self.device)
+                    done[r.rid] = (u, n, 0, steps)
+                except api.TahoeGpuError as e:
+                    done[r.rid] = (None, 0, e.code, 0)
         t_end = time.perf_counter() - t0
         out = []
         for r in reqs:

@@ -1,8 +1,9 @@
 """Continuous-batching server: requests of both formulations arriving over
 time are grouped into mini-batches on several contexts at once; every
 result equals the same request solved alone (bitwise: a sub-problem's
-arithmetic does not depend on the batch it lands in) and matches the plain
-FP64 solve within 1e-9."""
+arithmetic does not depend on the batch it lands in; a batch that fails is
+re-solved request by request through the same single-request path) and
+matches the plain FP64 solve within 1e-9."""
 import numpy as np
 import pytest
 
@@ -21,15 +22,9 @@ def test_served_equals_alone():
     fp64 = api.Solid.opts(lin_rtol=1e-10)
     for r in reqs:
         got = st.results[r.rid].u
-        steps = st.results[r.rid].load_steps
-        pk = batch.pack([r.problem()])
-        s = batch.make_solid(pk, formulation=r.formulation, precision=api.MIXED)
-        alone, _ = batch.static_solve_batch(s, pk, steps, srv.opts)
-        s.close()
-        assert np.array_equal(got, alone[0]), r.rid
-        s = batch.make_solid(pk, formulation=r.formulation, precision=api.FP64)
-        ref, _ = batch.static_solve_batch(s, pk, steps, fp64)
-        s.close()
-        assert np.abs(got - ref[0]).max() <= 1e-9 * np.abs(ref[0]).max(), r.rid
+        alone, _, _ = serve.solve_request(r, api.MIXED, srv.opts)
+        assert np.array_equal(got, alone), r.rid
+        ref, _, _ = serve.solve_request(r, api.FP64, fp64)
+        assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max(), r.rid
     lat = serve.latency_summary(st)
     assert lat["p99_s"] >= lat["p50_s"] > 0
