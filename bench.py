@@ -66,8 +66,10 @@ def parse_args():
     ap.add_argument("--stiff-chunk", type=int, default=None,
                     help="elements per range of the colour-ordered stiffness launches (default: autotuned)")
     ap.add_argument("--spmv-xcd", type=int, default=None, help="PCG SpMV variant 0..3 (default: autotuned)")
+    ap.add_argument("--assembly", type=int, default=None,
+                    help="Newton tangent: 0 into the MSR and expanded, 1 straight into the operator (default: autotuned)")
     ap.add_argument("--stiff-shape", type=int, default=None,
-                    help="stiffness workgroup shape 0..5 (default: autotuned)")
+                    help="stiffness workgroup shape 0..8 (default: autotuned)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the plain copy kernel (1 GiB) used to calibrate HBM counters")
     return ap.parse_args()
@@ -140,29 +142,30 @@ def main():
     n_eq, msr_len = s.initialize()
     print(f"# setup {time.perf_counter() - t_setup:.1f} s: nel={m.nel} nnd={m.nnd} n_eq={n_eq} "
           f"msr_upper_nnz={msr_len - 1}", file=sys.stderr, flush=True)
-    if a.stiff_chunk is not None or a.stiff_shape is not None:
-        s.set_tuning(api.TUNE_STIFF_CHUNK, -1 if a.stiff_chunk is None else a.stiff_chunk)
-        s.set_tuning(api.TUNE_STIFF_SHAPE, 0 if a.stiff_shape is None else a.stiff_shape)
-        tuned = {"shape": s.tuning(api.TUNE_STIFF_SHAPE), "chunk": s.tuning(api.TUNE_STIFF_CHUNK), "ms": {}}
-    else:
-        from tahoe_amd import autotune
-        tuned = autotune.tune_stiffness(s, reps=1)
-    print(f"# stiffness shape {tuned['shape']} chunk {tuned['chunk']} (autotune ms: {tuned['ms']})",
-          file=sys.stderr, flush=True)
     opts = s.opts(lin_rtol=a.lin_rtol, lin_maxit=48 if a.profile else 200000)
     t_load = 0.2
+    short = s.opts(lin_rtol=a.lin_rtol, lin_maxit=8)
+
+    def short_pcg():
+        s.reset_step()
+        s.apply_kbc(t_load)
+        s.newton_correction(short)
+
+    from tahoe_amd import autotune
+    if a.stiff_chunk is not None or a.stiff_shape is not None or a.assembly is not None:
+        s.set_tuning(api.TUNE_STIFF_CHUNK, -1 if a.stiff_chunk is None else a.stiff_chunk)
+        s.set_tuning(api.TUNE_STIFF_SHAPE, 0 if a.stiff_shape is None else a.stiff_shape)
+        s.set_tuning(api.TUNE_NEWTON_ASSEMBLY, 0 if a.assembly is None else a.assembly)
+        tuned = {"assembly": s.tuning(api.TUNE_NEWTON_ASSEMBLY), "shape": s.tuning(api.TUNE_STIFF_SHAPE),
+                 "chunk": s.tuning(api.TUNE_STIFF_CHUNK), "ms": {}}
+    else:
+        tuned = autotune.tune_stiffness(s, reps=1, run_correction=short_pcg)
+    print(f"# Newton assembly {tuned['assembly']} stiffness shape {tuned['shape']} chunk {tuned['chunk']} "
+          f"(autotune ms: {tuned['ms']})", file=sys.stderr, flush=True)
     if a.spmv_xcd is not None:
         s.set_tuning(api.TUNE_SPMV_XCD, a.spmv_xcd)
         tuned_spmv = {"xcd": a.spmv_xcd, "ms": {}}
     else:
-        from tahoe_amd import autotune
-        short = s.opts(lin_rtol=a.lin_rtol, lin_maxit=32)
-
-        def short_pcg():
-            s.reset_step()
-            s.apply_kbc(t_load)
-            s.newton_correction(short)
-
         tuned_spmv = autotune.tune_spmv_xcd(s, short_pcg)
     print(f"# SpMV XCD mapping {tuned_spmv['xcd']} (autotune ms: {tuned_spmv['ms']})", file=sys.stderr, flush=True)
 
@@ -232,6 +235,7 @@ def main():
                                "uniaxial 5% load step 1/5, one Newton correction: R + K into the solver operator + "
                                f"{'block-' if a.precond == 'block' else ''}Jacobi-PCG to {a.lin_rtol:g}",
                    "elements": m.nel, "nodes": m.nnd, "equations": n_eq, "msr_upper_nnz": msr_len - 1,
+                   "newton_assembly": "operator" if tuned["assembly"] == 1 else "msr+expansion",
                    "stiffness_shape_elements_per_workgroup": api.STIFF_SHAPE_EPB[tuned["shape"]],
                    "stiffness_chunk": tuned["chunk"], "spmv_xcd_rows": tuned_spmv["xcd"]},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
@@ -241,6 +245,7 @@ def main():
                      "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_avg_ms},
         "cpu_baseline": cpu_baseline(a.cpu_n) if a.cpu_n > 0 else None,
         "newton_solves_per_s": 1e3 / ms_per_step,
+        "elements_per_s_tangent_in_solver_operator": m.nel / ((r_ms + k_ms + tm["expand_ms"] / K) * 1e-3),
         "residual": {"ms": r_ms, "elements_per_s": m.nel / (r_ms * 1e-3), "gflops": res_gflops,
                      "frac_fp64_peak": res_gflops * 1e9 / FP64_PEAK},
         "stiffness": {"ms": k_ms, "elements_per_s": m.nel / (k_ms * 1e-3), "gflops": stf_gflops,
@@ -249,7 +254,7 @@ def main():
                 "dof_iterations_per_s": n_eq * its / (pcg_ms * 1e-3) if pcg_ms > 0 else None,
                 "expand_ms": tm["expand_ms"] / K},
         "timing_ms": tm,
-        "autotune_stiffness_ms": {k: {str(x): y for x, y in v.items()} for k, v in tuned["ms"].items()},
+        "autotune_tangent_ms": {k: {str(x): y for x, y in v.items()} for k, v in tuned["ms"].items()},
         "autotune_spmv_ms": {str(k): v for k, v in tuned_spmv["ms"].items()},
     }
     print(json.dumps(line), flush=True)

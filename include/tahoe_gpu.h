@@ -195,14 +195,13 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      6..8 for 16, 8 or 4 in the D*B form (D_q B shared
  *                      by an element's lanes through LDS);
  * TG_TUNE_NEWTON_ASSEMBLY  where tg_newton / tg_newton_batch /
- *                      tg_newton_correction put the tangent: 1 (default)
- *                      straight into the inner solve's block operator, 0
- *                      into the MSR and then expanded.  The operator is the
- *                      same either way (same sums in the same order); with
- *                      1 the MSR queries (tg_get_msr_values, tg_multx,
- *                      tg_copy_diagonal) see the matrix of the last
- *                      tg_form_lhs, as GlobalMatrixT's do between
- *                      assemblies. */
+ *                      tg_newton_correction put the tangent: 0 (default)
+ *                      into the MSR and then expanded into the inner solve's
+ *                      block operator, 1 straight into the block operator.
+ *                      The operator is the same either way (same sums in
+ *                      the same order); with 1 the MSR queries
+ *                      (tg_get_msr_values, tg_multx, tg_copy_diagonal) see
+ *                      the matrix of the last tg_form_lhs. */
 /* TG_TUNE_PCG_GRAPH    1 (default): the PCG's 16-iteration chunk is captured
  *                      once as a HIP graph and replayed (one launch instead
  *                      of 80, for small, launch-bound problems); 0: direct

@@ -7,6 +7,10 @@ arithmetic of an entry's contributions beyond their summation order, so
 every candidate gives a valid result.
 
 Knobs:
+  TUNE_NEWTON_ASSEMBLY -- Newton's tangent straight into the solver's block
+  operator (576 scattered values per element, component-major) or into the
+  MSR (300 values, row-contiguous clusters) and then expanded; the traffic of
+  the two differs by the cache-line reuse of the scatter.
   TUNE_STIFF_SHAPE -- elements per workgroup of the stiffness kernel
   (api.STIFF_SHAPE_EPB): fewer elements per workgroup means more workgroups
   resident per CU, which hides barrier and scatter latency, at the cost of
@@ -31,40 +35,54 @@ def stiffness_chunk_candidates(nel: int) -> list:
     return out
 
 
-def _time_stiffness(solid: api.Solid, reps: int) -> float:
-    solid.form_lhs()            # warm (builds the launch list)
+def _time_stiffness(solid: api.Solid, reps: int, run_correction=None) -> float:
+    """Mean device time of assembling the Newton tangent: the stiffness pass
+    plus what makes it the solver's operator (expansion, inverse diagonal).
+    With run_correction (a short Newton correction from a fixed state) the
+    Newton path is timed; otherwise tg_form_lhs (the MSR assembly)."""
+    step = run_correction or solid.form_lhs
+    step()                      # warm (builds launch lists, captures graphs)
     solid.synchronize()
     solid.timing_reset()
     for _ in range(reps):
-        solid.form_lhs()
+        step()
     solid.synchronize()
-    return solid.timing()["stiffness_ms"] / reps
+    t = solid.timing()
+    return (t["stiffness_ms"] + (t["expand_ms"] if run_correction else 0.0)) / reps
 
 
-def tune_stiffness(solid: api.Solid, shapes=None, chunks=None, reps: int = 2) -> dict:
-    """Best shape at the default chunk, then the best chunk for that shape.
-    Returns {"shape": s, "chunk": c, "ms": {"shape": {...}, "chunk": {...}}}."""
+def tune_stiffness(solid: api.Solid, shapes=None, chunks=None, reps: int = 2, run_correction=None) -> dict:
+    """Best (Newton assembly path, shape) at the default chunk, then the best
+    chunk.  With run_correction the Newton tangent is timed on both paths --
+    straight into the block operator, or into the MSR and expanded -- and
+    the faster is kept (TUNE_NEWTON_ASSEMBLY); without it the MSR assembly of
+    tg_form_lhs is tuned.  Returns {"assembly", "shape", "chunk", "ms"}."""
     if shapes is None:
         shapes = list(range(len(api.STIFF_SHAPE_EPB)))
     if chunks is None:
         chunks = stiffness_chunk_candidates(solid.nel)
+    modes = (0, 1) if run_correction else (solid.tuning(api.TUNE_NEWTON_ASSEMBLY),)
     t_shape, t_chunk = {}, {}
     solid.timing_enable(True)
     try:
         solid.set_tuning(api.TUNE_STIFF_CHUNK, -1)
-        for v in shapes:
-            solid.set_tuning(api.TUNE_STIFF_SHAPE, v)
-            t_shape[v] = _time_stiffness(solid, reps)
-        best_shape = min(t_shape, key=t_shape.get)
+        for m in modes:
+            solid.set_tuning(api.TUNE_NEWTON_ASSEMBLY, m)
+            for v in shapes:
+                solid.set_tuning(api.TUNE_STIFF_SHAPE, v)
+                t_shape[(m, v)] = _time_stiffness(solid, reps, run_correction)
+        best_mode, best_shape = min(t_shape, key=t_shape.get)
+        solid.set_tuning(api.TUNE_NEWTON_ASSEMBLY, best_mode)
         solid.set_tuning(api.TUNE_STIFF_SHAPE, best_shape)
         for v in chunks:
             solid.set_tuning(api.TUNE_STIFF_CHUNK, v)
-            t_chunk[v] = _time_stiffness(solid, reps)
+            t_chunk[v] = _time_stiffness(solid, reps, run_correction)
         best_chunk = min(t_chunk, key=t_chunk.get) if t_chunk else -1
         solid.set_tuning(api.TUNE_STIFF_CHUNK, best_chunk)
     finally:
         solid.timing_enable(False)
-    return {"shape": best_shape, "chunk": best_chunk, "ms": {"shape": t_shape, "chunk": t_chunk}}
+    return {"assembly": best_mode, "shape": best_shape, "chunk": best_chunk,
+            "ms": {"assembly_shape": {f"{m}/{v}": t for (m, v), t in t_shape.items()}, "chunk": t_chunk}}
 
 
 def tune_spmv_xcd(solid: api.Solid, run_pcg) -> dict:

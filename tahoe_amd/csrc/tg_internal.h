@@ -122,7 +122,7 @@ struct tg_ctx {
     // before range j+1 (0: whole colours)
     long stiff_chunk = -1;              // -1: chosen from the matrix footprint
     int stiff_variant = 0;              // block shape of the stiffness kernel
-    int newton_assembly = 1;            // 1: Newton's K straight into the solver operator, 0: via MSR
+    int newton_assembly = 0;            // 0: Newton's K into the MSR then expanded, 1: straight into the operator
     int precond = TG_PC_JACOBI;         // preconditioner of the inner PCG
     int pcg_graph = 1;                  // replay PCG chunks as a captured HIP graph
     int spmv_xcd = 0;                   // XCD-contiguous row ranges in the PCG SpMV
