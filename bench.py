@@ -306,11 +306,14 @@ def bench_mixed_steps(a):
     print(json.dumps(line), flush=True)
 
 
-def solve_alone(req, precision, opts):
-    """One request in its own context (the plain one-problem-at-a-time path)."""
+def solve_alone(req, precision, opts, problem=None):
+    """One request in its own context (the plain one-problem-at-a-time path).
+    Returns (u, Newton stats, seconds for pack + context + solve, seconds for
+    the solve alone)."""
     from tahoe_amd import batch
-    pk = batch.pack([req.problem()])
+    p = problem if problem is not None else req.problem()
     t0 = time.perf_counter()
+    pk = batch.pack([p])
     s = batch.make_solid(pk, formulation=req.formulation, precision=precision)
     t1 = time.perf_counter()
     us, st = batch.static_solve_batch(s, pk, 1, opts)
@@ -351,10 +354,11 @@ def bench_rve_batch(a):
     sample = reqs[::k][:a.plain_sample]
     solve_alone(sample[0], api.FP64, fp64)  # warm-up (library load, first context)
     plain_u, plain_wall, plain_solve, plain_el = {}, 0.0, 0.0, 0
+    probs = {r.rid: r.problem() for r in reqs}   # mesh generation: the client's, not timed
     for i, r in enumerate(sample):
         if i % 32 == 0:
             print(f"# plain FP64 RVE {i}/{len(sample)}", file=sys.stderr, flush=True)
-        u, st, wall, solve = solve_alone(r, api.FP64, fp64)
+        u, st, wall, solve = solve_alone(r, api.FP64, fp64, probs[r.rid])
         assert st.status == 0
         plain_u[r.rid] = u
         plain_wall += wall
@@ -369,7 +373,7 @@ def bench_rve_batch(a):
     errs = []
     newton_max = 0
     for b in batches:
-        pk = batch.pack([r.problem() for r in b])
+        pk = batch.pack([probs[r.rid] for r in b])
         s = batch.make_solid(pk, formulation=api.TOTAL_LAGRANGIAN, precision=api.MIXED)
         t1 = time.perf_counter()
         us, st = batch.static_solve_batch(s, pk, 1, mixed)
@@ -398,6 +402,9 @@ def bench_rve_batch(a):
         "plain_fp64": {"rves": len(sample), "elements": plain_el, "wall_s": plain_wall,
                        "elements_per_s": plain_rate, "solve_only_elements_per_s": plain_el / plain_solve,
                        "note": "every k-th request, one context per RVE, FP64 K + Jacobi-PCG to 1e-10"},
+        "timed": "both paths: pack + context setup (tg_initialize) + Newton to convergence; mesh generation "
+                 "excluded from both; solve_only: Newton alone",
+        "speedup_solve_only": (tot_el / solve_s) / (plain_el / plain_solve),
         "speedup_vs_plain_fp64": value / plain_rate,
         "max_rel_err_vs_fp64": max(errs) if errs else None, "compared_rves": len(errs),
         "newton_max": newton_max,
@@ -434,7 +441,9 @@ def bench_rve_serve(a):
     for i, r in enumerate(reqs[:n_plain]):
         if i % 32 == 0:
             print(f"# plain FP64 request {i}/{n_plain}", file=sys.stderr, flush=True)
-        u, stn, wall, _ = solve_alone(r, api.FP64, fp64)
+        t_r = time.perf_counter()
+        u, _, _ = serve.solve_request(r, api.FP64, fp64)
+        wall = time.perf_counter() - t_r
         clock = max(clock, r.arrival) + wall
         plain_lat.append(clock - r.arrival)
         ref = st.results[r.rid].u

@@ -4,8 +4,10 @@
 // ExceptionT::CodeT values.  Host<->device copies happen only where the
 // reference interface hands over host arrays; the Newton loop itself keeps
 // every vector on the device.
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -341,11 +343,25 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         for (int m = 0; m < nmat; m++)
             if (c->mu[m] < 0.0) fail(TG_BAD_INPUT, "element refers to an undefined material");
         c->initialized = false;
+        // TG_SETUP_TRACE=1 prints the host-side setup phases to stderr
+        static const bool trace = std::getenv("TG_SETUP_TRACE") != nullptr;
+        auto t_last = std::chrono::steady_clock::now();
+        auto phase = [&](const char* what) {
+            if (!trace) return;
+            TG_HIP(hipStreamSynchronize(c->stream));
+            const auto now = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "[tg_initialize] %-22s %8.1f ms (nel %d)\n", what,
+                         std::chrono::duration<double, std::milli>(now - t_last).count(), c->nel);
+            t_last = now;
+        };
         // equations, pattern, colours
         c->n_eq = equation_numbers(c->nnd, c->kbc_nodes, c->kbc_dofs, c->eqnos);
+        phase("equation numbers");
         msr_pattern(c->n_eq, c->nnd, c->nel, c->conn.data(), c->eqnos.data(), c->bindx);
         c->msr_len = (long)c->bindx.size();
+        phase("MSR pattern");
         c->ncolors = color_elements(c->nnd, c->nel, c->conn.data(), c->color_ptr, c->color_elems);
+        phase("colouring");
         c->stiff_chunk_built = -2;
         // device storage
         const size_t nn = 3 * (size_t)c->nnd;
@@ -387,8 +403,11 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
             TG_HIP(hipMemcpy(c->dkbc_idx.p, kidx.data(), sizeof(int) * kidx.size(), hipMemcpyHostToDevice));
             TG_HIP(hipMemcpy(c->dkbc_val.p, c->kbc_vals.data(), sizeof(double) * kidx.size(), hipMemcpyHostToDevice));
         }
+        phase("allocate + upload");
         launch_build_slots(c);
+        phase("slot map");
         build_bsell(c);
+        phase("block operator");
         c->op.d_diag.alloc((size_t)c->nnd);
         launch_build_operator_diag(c);
         if (c->formulation == TG_TOTAL_LAGRANGIAN) {
@@ -399,8 +418,10 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         }
         c->vbinv.alloc(c->precond == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
         c->op.d_val32.alloc(mixed ? (size_t)c->op.nblocks_padded * 9 : 0);
+        phase("operator slot map");
         setup_segments(c);
         TG_HIP(hipStreamSynchronize(c->stream));
+        phase("segments");
         c->initialized = true;
         if (n_eq) *n_eq = c->n_eq;
         if (msr_len) *msr_len = c->msr_len;
