@@ -195,7 +195,9 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      12, 8 or 4 elements per workgroup with element-major
  *                      lanes, 4..5 for 16 or 8 with block-major lanes,
  *                      6..8 for 16, 8 or 4 in the D*B form (D_q B shared
- *                      by an element's lanes through LDS);
+ *                      by an element's lanes through LDS), 9..11 for 12, 8
+ *                      or 4 with the MSR read-modify-write loads issued
+ *                      before the block products;
  * TG_TUNE_NEWTON_ASSEMBLY  where tg_newton / tg_newton_batch /
  *                      tg_newton_correction put the tangent: 0 (default)
  *                      into the MSR and then expanded into the inner solve's

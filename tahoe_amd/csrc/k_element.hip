@@ -166,9 +166,10 @@ __device__ __forceinline__ int upper_index(int m, int n) { return m * 24 - (m * 
 // barrier and scatter waits.  EPB (elements per block) and the phase-2 lane
 // order are template parameters; the autotuner picks among the compiled
 // shapes.
-constexpr int STIFF_EPB[] = {16, 12, 8, 4, 16, 8, 16, 8, 4};
-constexpr bool STIFF_PM[] = {false, false, false, false, true, true, false, false, false};
-constexpr bool STIFF_DB[] = {false, false, false, false, false, false, true, true, true};
+constexpr int STIFF_EPB[] = {16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4};
+constexpr bool STIFF_PM[] = {false, false, false, false, true, true, false, false, false, false, false, false};
+constexpr bool STIFF_DB[] = {false, false, false, false, false, false, true, true, true, false, false, false};
+constexpr bool STIFF_PF[] = {false, false, false, false, false, false, false, false, false, true, true, true};
 constexpr int N_STIFF_SHAPES = sizeof(STIFF_EPB) / sizeof(STIFF_EPB[0]);
 
 // Voigt index of (i,k): (11,22,33,23,13,12)
@@ -301,7 +302,11 @@ __device__ __forceinline__ void upper_block(int p, int& a, int& b)
 // same block p of many elements -- their scatter targets are neighbouring
 // rows -- and the LDS tables are element-minor so those lanes read
 // consecutive words.
-template <class VT, int EPB, bool TO_OP, bool PM>
+// PF (MSR path only): the slot indices and old values of the lane's 9
+// entries are loaded before the block products, so the read half of the
+// read-modify-write is in flight during the arithmetic (no other element of
+// the launch touches those entries).
+template <class VT, int EPB, bool TO_OP, bool PM, bool PF>
 __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict__ elems, int count,
                                                              const int* __restrict__ conn,
                                                              const int* __restrict__ emat,
@@ -356,6 +361,21 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
     if (base + le < count) {
         int a, b;
         upper_block(p, a, b);
+        constexpr bool PRE = PF && !TO_OP;
+        int sidx[PRE ? 9 : 1];
+        VT old[PRE ? 9 : 1];
+        if (PRE) {
+            const int* es = slot + 300L * elems[base + le];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int m = 3 * a + i, n = 3 * b + j;
+                    sidx[PRE ? 3 * i + j : 0] = n < m ? -1 : es[upper_index(m, n)];
+                }
+#pragma unroll
+            for (int k = 0; k < (PRE ? 9 : 1); k++) old[k] = sidx[k] >= 0 ? val[sidx[k]] : VT(0);
+        }
         VT K[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++)
@@ -395,7 +415,17 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
 #pragma unroll
             for (int i = 0; i < 3; i++) K[i][i] += geo;
         }
-        scatter_block<VT, TO_OP>(elems[base + le], a, b, p, K, conn, slot, eqn, val);
+        if (PRE) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int k = PRE ? 3 * i + j : 0;
+                    if (sidx[k] >= 0) val[sidx[k]] = old[k] + K[i][j];
+                }
+        } else {
+            scatter_block<VT, TO_OP>(elems[base + le], a, b, p, K, conn, slot, eqn, val);
+        }
     }
     if (bad) atomicOr(status, 7);
 }
@@ -559,7 +589,7 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_db(const int* __restrict
     if (bad) atomicOr(status, 7);
 }
 
-template <class VT, int EPB, bool TO_OP, bool PM, bool DB>
+template <class VT, int EPB, bool TO_OP, bool PM, bool DB, bool PF>
 static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u, VT* val)
 {
     const int blocks = (count + EPB - 1) / EPB;
@@ -568,7 +598,7 @@ static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u,
                            c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
                            TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
     else
-        hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
+        hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM, PF>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
                            c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
                            TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
 }
@@ -576,7 +606,8 @@ static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u,
 template <class VT, bool TO_OP>
 static void launch_stiffness_shape(tg_ctx* c, int v, int first, int count, const double* u, VT* val)
 {
-#define TG_SHAPE(i) launch_stiffness_v<VT, STIFF_EPB[i], TO_OP, STIFF_PM[i], STIFF_DB[i]>(c, first, count, u, val)
+#define TG_SHAPE(i) \
+    launch_stiffness_v<VT, STIFF_EPB[i], TO_OP, STIFF_PM[i], STIFF_DB[i], STIFF_PF[i]>(c, first, count, u, val)
     switch (v) {
     case 0: TG_SHAPE(0); break;
     case 1: TG_SHAPE(1); break;
@@ -586,7 +617,10 @@ static void launch_stiffness_shape(tg_ctx* c, int v, int first, int count, const
     case 5: TG_SHAPE(5); break;
     case 6: TG_SHAPE(6); break;
     case 7: TG_SHAPE(7); break;
-    default: TG_SHAPE(8); break;
+    case 8: TG_SHAPE(8); break;
+    case 9: TG_SHAPE(9); break;
+    case 10: TG_SHAPE(10); break;
+    default: TG_SHAPE(11); break;
     }
 #undef TG_SHAPE
 }
