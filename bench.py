@@ -145,7 +145,7 @@ def main():
           f"msr_upper_nnz={msr_len - 1}", file=sys.stderr, flush=True)
     opts = s.opts(lin_rtol=a.lin_rtol, lin_maxit=48 if a.profile else 200000)
     t_load = 0.2
-    short = s.opts(lin_rtol=a.lin_rtol, lin_maxit=8)
+    short = s.opts(lin_rtol=a.lin_rtol, lin_maxit=32)  # two full PCG chunks: no skipped launches
 
     def short_pcg():
         s.reset_step()
@@ -420,18 +420,22 @@ def bench_rve_serve(a):
     (one request at a time, in arrival order) on the same stream."""
     from tahoe_amd import api, rve, serve
     # capacity: everything arriving at once
-    cal = rve.stream_requests(min(128, a.serve_count), rate=1e9, seed=555)
+    cal = rve.stream_requests(min(48, a.serve_count), rate=1e9, seed=555)
     def progress(bid, size, form, sec):
-        print(f"# batch {bid}: {size} requests ({'Q1P0' if form == api.Q1P0 else 'Q1'}) in {sec:.2f} s",
+        what = "failed, re-solving one by one" if sec < 0 else f"in {sec:.2f} s"
+        print(f"# batch {bid}: {size} requests ({'Q1P0' if form == api.Q1P0 else 'Q1'}) {what}",
               file=sys.stderr, flush=True)
 
-    srv = serve.RVEServer(precision=api.MIXED, keep_u=False, on_batch=progress)
+    def log(msg):
+        print(f"# {msg}", file=sys.stderr, flush=True)
+
+    srv = serve.RVEServer(precision=api.MIXED, keep_u=False, on_batch=progress, log=log)
     st = srv.run(cal, realtime=False)
     cap = len(cal) / st.wall
     rate = a.serve_load * cap
     print(f"# capacity {cap:.1f} solves/s -> offered {rate:.1f}/s", file=sys.stderr, flush=True)
     reqs = rve.stream_requests(a.serve_count, rate=rate)
-    srv = serve.RVEServer(precision=api.MIXED, on_batch=progress)
+    srv = serve.RVEServer(precision=api.MIXED, on_batch=progress, log=log)
     st = srv.run(reqs, realtime=True)
     lat = serve.latency_summary(st)
     # plain FP64 on the same arrivals: one request at a time
@@ -443,7 +447,7 @@ def bench_rve_serve(a):
         if i % 32 == 0:
             print(f"# plain FP64 request {i}/{n_plain}", file=sys.stderr, flush=True)
         t_r = time.perf_counter()
-        u, _, _ = serve.solve_request(r, api.FP64, fp64)
+        u, _, _ = serve.solve_request(r, api.FP64, fp64, log=log)
         wall = time.perf_counter() - t_r
         clock = max(clock, r.arrival) + wall
         plain_lat.append(clock - r.arrival)

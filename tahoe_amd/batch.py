@@ -123,7 +123,7 @@ def static_solve_batch(s: api.Solid, pk: Packed, num_steps: int, opts=None):
     return unpack(pk, s.displacement()), stats
 
 
-def solve_adaptive(s: api.Solid, pk: Packed, opts=None, min_dt=1.0 / 1024):
+def solve_adaptive(s: api.Solid, pk: Packed, opts=None, min_dt=1.0 / 1024, log=None):
     """The static load path to t = 1 with step cutting: a step that fails
     (J <= 0 or Newton failure in any sub-problem) is reset and retried at
     half the increment, a converged step doubles the next one
@@ -147,6 +147,8 @@ def solve_adaptive(s: api.Solid, pk: Packed, opts=None, min_dt=1.0 / 1024):
             if e.code != 7:
                 raise
             ok, st = False, None
+        if log:
+            log(f"load step t={t:.4g}+{dt:.4g}: {'converged' if ok else 'cut'}")
         if ok:
             s.close_step()
             u = s.displacement()

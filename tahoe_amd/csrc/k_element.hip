@@ -776,6 +776,61 @@ void launch_build_operator_diag(tg_ctx* c)
     TG_HIP(hipGetLastError());
 }
 
+// Operator value -> MSR value it mirrors: for row node A (slice s, lane l)
+// and the column node of each slot, entry (i, j) is the upper MSR entry
+// (min(r, c), max(r, c)) of r = eq(A, i), c = eq(B, j); -1 when either dof is
+// prescribed or the slot is padding.
+__global__ void k_build_src(int nnd, const int* __restrict__ sbase, const int* __restrict__ swidth,
+                            const int* __restrict__ bcol, const int* __restrict__ eqn,
+                            const int* __restrict__ bindx, int* __restrict__ src)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = a >> 6, l = a & 63;
+    if (s >= (nnd + 63) / 64) return;
+    for (int k = 0; k < swidth[s]; k++) {
+        const long g = (long)sbase[s] + k;
+        const int B = a < nnd ? bcol[g * 64 + l] : -1;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                int v = -1;
+                if (B >= 0) {
+                    const int r = eqn[3L * a + i], cc = eqn[3L * B + j];
+                    if (r >= 0 && cc >= 0) {
+                        const int row = r < cc ? r : cc, col = r < cc ? cc : r;
+                        if (row == col)
+                            v = row;
+                        else {
+                            int lo = bindx[row], hi = bindx[row + 1] - 1;
+                            while (lo <= hi) {
+                                const int mid = (lo + hi) >> 1;
+                                const int x = bindx[mid];
+                                if (x == col) {
+                                    v = mid;
+                                    break;
+                                }
+                                if (x < col)
+                                    lo = mid + 1;
+                                else
+                                    hi = mid - 1;
+                            }
+                        }
+                    }
+                }
+                src[(g * 9 + 3 * i + j) * 64 + l] = v;
+            }
+    }
+}
+
+void launch_build_src(tg_ctx* c)
+{
+    const int bs = 256;
+    const long n = 64L * c->op.nslices;
+    hipLaunchKernelGGL(k_build_src, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nnd,
+                       c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->deqn.p, c->dbindx.p,
+                       c->op.d_src.p);
+    TG_HIP(hipGetLastError());
+}
+
 void launch_build_slots(tg_ctx* c)
 {
     const long n = 300L * c->nel;

@@ -357,7 +357,9 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         // equations, pattern, colours
         c->n_eq = equation_numbers(c->nnd, c->kbc_nodes, c->kbc_dofs, c->eqnos);
         phase("equation numbers");
-        msr_pattern(c->n_eq, c->nnd, c->nel, c->conn.data(), c->eqnos.data(), c->bindx);
+        node_graph(c->nnd, c->nel, c->conn.data(), c->nptr, c->nlist);
+        phase("node graph");
+        msr_pattern(c->n_eq, c->nnd, c->eqnos.data(), c->nptr, c->nlist, c->bindx);
         c->msr_len = (long)c->bindx.size();
         phase("MSR pattern");
         c->ncolors = color_elements(c->nnd, c->nel, c->conn.data(), c->color_ptr, c->color_elems);
@@ -419,6 +421,8 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         c->vbinv.alloc(c->precond == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
         c->op.d_val32.alloc(mixed ? (size_t)c->op.nblocks_padded * 9 : 0);
         phase("operator slot map");
+        std::vector<long>().swap(c->nptr);
+        std::vector<int>().swap(c->nlist);
         setup_segments(c);
         TG_HIP(hipStreamSynchronize(c->stream));
         phase("segments");

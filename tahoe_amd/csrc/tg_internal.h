@@ -117,6 +117,8 @@ struct tg_ctx {
     long msr_len = 0;
     std::vector<int> color_ptr, color_elems;
     int ncolors = 0;
+    std::vector<long> nptr;             // node adjacency (setup only, released after tg_initialize)
+    std::vector<int> nlist;
     // stiffness launch order: every segment's elements in ranges of
     // stiff_chunk (by rank inside the segment), all colours of range j
     // before range j+1 (0: whole colours)
@@ -171,7 +173,9 @@ namespace tg {
 
 // host setup (tg_setup.cpp)
 int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
-void msr_pattern(int n_eq, int nnd, int nel, const int* conn, const int* eqnos, std::vector<int>& bindx);
+void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, std::vector<int>& nlist);
+void msr_pattern(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const std::vector<int>& nlist,
+                 std::vector<int>& bindx);
 int color_elements(int nnd, int nel, const int* conn, std::vector<int>& ptr, std::vector<int>& elems);
 void build_bsell(tg_ctx* c);
 
@@ -180,6 +184,7 @@ void launch_residual(tg_ctx* c, const double* u, double* R_node);
 void launch_stiffness(tg_ctx* c, const double* u);
 void launch_stiffness_operator(tg_ctx* c, const double* u);
 void launch_build_operator_slots(tg_ctx* c);
+void launch_build_src(tg_ctx* c);
 void launch_operator_dinv(tg_ctx* c);
 void launch_block_inverse(tg_ctx* c);
 void launch_build_operator_diag(tg_ctx* c);

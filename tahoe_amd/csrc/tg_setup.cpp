@@ -74,37 +74,72 @@ static int node_neighbours(int a, const std::vector<int>& ptr, const std::vector
     return (int)(std::unique(out, out + n) - out);
 }
 
+// Node adjacency as CSR (sorted unique neighbours including the node itself),
+// built once and shared by the MSR pattern and the block operator.  Each
+// worker de-duplicates with a per-node stamp array, so only the unique
+// neighbours are sorted.
+void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, std::vector<int>& nlist)
+{
+    std::vector<int> ptr, list;
+    node_elements(nnd, nel, conn, ptr, list);
+    const int MAXNB = 8 * 64;
+    for (int a = 0; a < nnd; a++)
+        if (ptr[a + 1] - ptr[a] > MAXNB / 8) fail(TG_OUT_OF_RANGE, "a node is shared by more than 64 elements");
+    auto unique_neighbours = [&](int a, std::vector<int>& stamp, int* out) {
+        int n = 0;
+        for (int p = ptr[a]; p < ptr[a + 1]; p++) {
+            const int* c = conn + 8L * list[p];
+            for (int b = 0; b < 8; b++)
+                if (stamp[c[b]] != a) {
+                    stamp[c[b]] = a;
+                    out[n++] = c[b];
+                }
+        }
+        std::sort(out, out + n);
+        return n;
+    };
+    std::vector<int> deg((size_t)nnd);
+    parallel_for(nnd, [&](long b, long e) {
+        std::vector<int> nb(MAXNB), stamp((size_t)nnd, -1);
+        for (long a = b; a < e; a++) deg[a] = unique_neighbours((int)a, stamp, nb.data());
+    });
+    nptr.assign((size_t)nnd + 1, 0);
+    for (int a = 0; a < nnd; a++) nptr[a + 1] = nptr[a] + deg[a];
+    nlist.resize((size_t)nptr[nnd]);
+    parallel_for(nnd, [&](long b, long e) {
+        std::vector<int> stamp((size_t)nnd, -1);
+        for (long a = b; a < e; a++) unique_neighbours((int)a, stamp, nlist.data() + nptr[a]);
+    });
+}
+
 // Upper-triangle MSR pattern of GraphT::MakeGraph(active, add_self, upper_only)
 // (toolbox/src/graph/GraphT.cpp:376-456) packed by MSRBuilderT::GenerateMSR
 // (toolbox/src/graph/MSRBuilderT.cpp:134-181): row i lists every active
 // equation j > i sharing an element with i, ascending; bindx[0] = n_eq + 1.
-void msr_pattern(int n_eq, int nnd, int nel, const int* conn, const int* eqnos, std::vector<int>& bindx)
+void msr_pattern(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const std::vector<int>& nlist,
+                 std::vector<int>& bindx)
 {
-    std::vector<int> ptr, list;
-    node_elements(nnd, nel, conn, ptr, list);
     std::vector<int> rowlen((size_t)n_eq, 0);
-    auto columns = [&](int a, int row, int* nb, int nnb, int* cols) {
+    auto columns = [&](long a, int row, int* cols) {
         int m = 0;
-        for (int k = 0; k < nnb; k++)
+        for (long k = nptr[a]; k < nptr[a + 1]; k++)
             for (int j = 0; j < 3; j++) {
-                int col = eqnos[3L * nb[k] + j];
+                int col = eqnos[3L * nlist[k] + j];
                 if (col > row) cols[m++] = col - 1;
             }
-        // equation numbers need not follow node order in general
-        std::sort(cols, cols + m);
-        (void)a;
+        // equation numbers need not follow node order in general (they do
+        // for the node-major numbering, and then the columns come sorted)
+        if (!std::is_sorted(cols, cols + m)) std::sort(cols, cols + m);
         return m;
     };
     const int MAXNB = 8 * 64;
     parallel_for(nnd, [&](long b, long e) {
-        std::vector<int> nb(MAXNB), cols(3 * MAXNB);
-        for (long a = b; a < e; a++) {
-            int nnb = node_neighbours((int)a, ptr, list, conn, nb.data());
+        std::vector<int> cols(3 * MAXNB);
+        for (long a = b; a < e; a++)
             for (int i = 0; i < 3; i++) {
                 int row = eqnos[3 * a + i];
-                if (row > 0) rowlen[row - 1] = columns((int)a, row, nb.data(), nnb, cols.data());
+                if (row > 0) rowlen[row - 1] = columns(a, row, cols.data());
             }
-        }
     });
     long total = 0;
     for (int r = 0; r < n_eq; r++) total += rowlen[r];
@@ -113,16 +148,14 @@ void msr_pattern(int n_eq, int nnd, int nel, const int* conn, const int* eqnos, 
     bindx[0] = n_eq + 1;
     for (int r = 0; r < n_eq; r++) bindx[r + 1] = bindx[r] + rowlen[r];
     parallel_for(nnd, [&](long b, long e) {
-        std::vector<int> nb(MAXNB), cols(3 * MAXNB);
-        for (long a = b; a < e; a++) {
-            int nnb = node_neighbours((int)a, ptr, list, conn, nb.data());
+        std::vector<int> cols(3 * MAXNB);
+        for (long a = b; a < e; a++)
             for (int i = 0; i < 3; i++) {
                 int row = eqnos[3 * a + i];
                 if (row <= 0) continue;
-                int m = columns((int)a, row, nb.data(), nnb, cols.data());
+                int m = columns(a, row, cols.data());
                 std::memcpy(&bindx[(size_t)bindx[row - 1]], cols.data(), sizeof(int) * (size_t)m);
             }
-        }
     });
 }
 
@@ -152,40 +185,27 @@ int color_elements(int nnd, int nel, const int* conn, std::vector<int>& cptr, st
     return ncol;
 }
 
-static long msr_find(const std::vector<int>& bindx, int row, int col)
-{
-    if (row == col) return row;
-    const int* b = bindx.data();
-    const int* lo = b + b[row];
-    const int* hi = b + b[row + 1];
-    const int* p = std::lower_bound(lo, hi, col);
-    if (p == hi || *p != col) return -1;
-    return (long)(p - b);
-}
-
 // Solver operator structure: block row = node, block columns = neighbour
 // nodes; 64 block rows per slice, each slice padded to its widest row.
-// d_src maps each stored value to the MSR value it mirrors (upper entry
-// (min(r,c), max(r,c))), -1 for inactive dofs and padding.
+// d_src (built on the device, launch_build_src) maps each stored value to
+// the MSR value it mirrors (upper entry (min(r,c), max(r,c))), -1 for
+// inactive dofs and padding.
 void build_bsell(tg_ctx* c)
 {
     Bsell& B = c->op;
     const int nnd = c->nnd;
-    std::vector<int> ptr, list;
-    node_elements(nnd, c->nel, c->conn.data(), ptr, list);
-    std::vector<int> deg((size_t)nnd);
-    const int MAXNB = 8 * 64;
-    parallel_for(nnd, [&](long b, long e) {
-        std::vector<int> nb(MAXNB);
-        for (long a = b; a < e; a++) deg[a] = node_neighbours((int)a, ptr, list, c->conn.data(), nb.data());
-    });
+    const std::vector<long>& nptr = c->nptr;
+    const std::vector<int>& nlist = c->nlist;
     B.nslices = (nnd + 63) / 64;
     B.slice_base.assign((size_t)B.nslices, 0);
     B.slice_width.assign((size_t)B.nslices, 0);
     long base = 0;
     for (int s = 0; s < B.nslices; s++) {
         int w = 0;
-        for (int l = 0; l < 64 && 64 * s + l < nnd; l++) w = std::max(w, deg[64 * s + l]);
+        for (int l = 0; l < 64 && 64 * s + l < nnd; l++) {
+            const long a = 64L * s + l;
+            w = std::max(w, (int)(nptr[a + 1] - nptr[a]));
+        }
         B.slice_base[s] = (int)base;
         B.slice_width[s] = w;
         base += w;
@@ -193,26 +213,13 @@ void build_bsell(tg_ctx* c)
     B.nblocks_padded = base * 64;
     if (B.nblocks_padded * 9 > 0x7fffffffL) fail(TG_OUT_OF_RANGE, "block operator exceeds 32-bit indexing");
     std::vector<int> col((size_t)B.nblocks_padded, -1);
-    std::vector<int> src((size_t)B.nblocks_padded * 9, -1);
-    const int* eq = c->eqnos.data();
     parallel_for(B.nslices, [&](long sb, long se) {
-        std::vector<int> nb(MAXNB);
         for (long s = sb; s < se; s++)
             for (int l = 0; l < 64; l++) {
-                long a = 64 * s + l;
+                const long a = 64 * s + l;
                 if (a >= nnd) break;
-                int nnb = node_neighbours((int)a, ptr, list, c->conn.data(), nb.data());
-                for (int k = 0; k < nnb; k++) {
-                    long slot = ((long)B.slice_base[s] + k) * 64 + l;
-                    col[(size_t)slot] = nb[k];
-                    for (int i = 0; i < 3; i++)
-                        for (int j = 0; j < 3; j++) {
-                            int r = eq[3 * a + i], cc = eq[3L * nb[k] + j];
-                            long v = -1;
-                            if (r > 0 && cc > 0) v = msr_find(c->bindx, std::min(r, cc) - 1, std::max(r, cc) - 1);
-                            src[(size_t)((((long)B.slice_base[s] + k) * 9 + 3 * i + j) * 64 + l)] = (int)v;
-                        }
-                }
+                for (long k = nptr[a]; k < nptr[a + 1]; k++)
+                    col[(size_t)(((long)B.slice_base[s] + (k - nptr[a])) * 64 + l)] = nlist[k];
             }
     });
     B.d_slice_base.alloc(B.nslices);
@@ -223,7 +230,7 @@ void build_bsell(tg_ctx* c)
     TG_HIP(hipMemcpy(B.d_slice_base.p, B.slice_base.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice));
     TG_HIP(hipMemcpy(B.d_slice_width.p, B.slice_width.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice));
     TG_HIP(hipMemcpy(B.d_col.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
-    TG_HIP(hipMemcpy(B.d_src.p, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice));
+    launch_build_src(c);
 }
 
 }  // namespace tg

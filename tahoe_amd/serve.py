@@ -68,7 +68,7 @@ class ServerStats:
     wall: float = 0.0
 
 
-def solve_request(req, precision, opts, device=0):
+def solve_request(req, precision, opts, device=0, log=None):
     """One request on its own context: one load step from the affine
     predictor, and when that fails the adaptive load path (batch.solve_adaptive).
     Returns (u, Newton corrections, load steps).  A request's result does not
@@ -85,7 +85,8 @@ def solve_request(req, precision, opts, device=0):
         s.close()
     s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device)
     try:
-        us, steps, newton = batch.solve_adaptive(s, pk, opts)
+        us, steps, newton = batch.solve_adaptive(
+            s, pk, opts, log=(lambda m: log(f"request {req.rid} (L={req.L}): {m}")) if log else None)
     finally:
         s.close()
     return us[0], int(newton[0]), steps
@@ -93,14 +94,16 @@ def solve_request(req, precision, opts, device=0):
 
 class RVEServer:
     def __init__(self, precision=api.MIXED, workers=4, max_batch=256, budget_bytes=64 << 30, lin_rtol=None,
-                 device=0, keep_u=True, on_batch=None):
+                 device=0, keep_u=True, on_batch=None, log=None):
         self.precision = precision
         self.workers = workers
         self.max_batch = max_batch
         self.budget = budget_bytes
         self.device = device
         self.keep_u = keep_u
-        self.on_batch = on_batch     # called as on_batch(batch_id, size, formulation, seconds) after each batch
+        self.log = log               # log(message) for load-step cutting of re-solved requests
+        self.on_batch = on_batch     # on_batch(batch_id, size, formulation, seconds) after each batch
+                                     # (seconds = -1 when the batch failed and is re-solved request by request)
         rt = lin_rtol if lin_rtol is not None else (1e-6 if precision == api.MIXED else 1e-12)
         self.opts = api.Solid.opts(lin_rtol=rt)
         self._lock = threading.Condition()
@@ -161,10 +164,12 @@ class RVEServer:
         try:
             us, its = self._solve_group(reqs)
             done = {r.rid: (u, n, 0, 1) for r, u, n in zip(reqs, us, its)}
-        except api.TahoeGpuError:
+        except api.TahoeGpuError as e:
+            if self.on_batch:
+                self.on_batch(bid, len(reqs), reqs[0].formulation, -1.0)  # -1: batch failed, solving one by one
             for r in reqs:
                 try:
-                    u, n, steps = solve_request(r, self.precision, self.opts, self.device)
+                    u, n, steps = solve_request(r, self.precision, self.opts, self.device, self.log)
                     done[r.rid] = (u, n, 0, steps)
                 except api.TahoeGpuError as e:
                     done[r.rid] = (None, 0, e.code, 0)
