@@ -821,6 +821,32 @@ __global__ void k_build_src(int nnd, const int* __restrict__ sbase, const int* _
     }
 }
 
+// Lower blocks (column node B < row node A) are the transposes of upper
+// blocks (B, A): the slot k' of column A in row B gives the partner's value
+// index, so the expansion copies them from the operator itself instead of
+// gathering them from MSR rows of other nodes.
+__global__ void k_build_tpos(int nnd, const int* __restrict__ sbase, const int* __restrict__ swidth,
+                             const int* __restrict__ bcol, int* __restrict__ tpos)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = a >> 6, l = a & 63;
+    if (s >= (nnd + 63) / 64) return;
+    for (int k = 0; k < swidth[s]; k++) {
+        const long g = (long)sbase[s] + k;
+        const int B = a < nnd ? bcol[g * 64 + l] : -1;
+        int out = -1;
+        if (B >= 0 && B < a) {
+            const int sB = B >> 6, lB = B & 63;
+            for (int kk = 0; kk < swidth[sB]; kk++)
+                if (bcol[((long)sbase[sB] + kk) * 64 + lB] == a) {
+                    out = (int)((((long)sbase[sB] + kk) * 9) * 64 + lB);
+                    break;
+                }
+        }
+        tpos[g * 64 + l] = out;
+    }
+}
+
 void launch_build_src(tg_ctx* c)
 {
     const int bs = 256;
@@ -828,6 +854,8 @@ void launch_build_src(tg_ctx* c)
     hipLaunchKernelGGL(k_build_src, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nnd,
                        c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->deqn.p, c->dbindx.p,
                        c->op.d_src.p);
+    hipLaunchKernelGGL(k_build_tpos, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nnd,
+                       c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->op.d_tpos.p);
     TG_HIP(hipGetLastError());
 }
 

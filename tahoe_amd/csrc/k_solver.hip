@@ -82,12 +82,14 @@ __device__ __forceinline__ void ordered_total(const double* part, int nparts, do
 // ---------------------------------------------------------------------------
 
 template <class VT>
-__global__ void k_expand(long nb, const int* __restrict__ src, const VT* __restrict__ val, VT* __restrict__ out)
+__global__ void k_expand(long nb, const int* __restrict__ src, const int* __restrict__ tpos,
+                         const VT* __restrict__ val, VT* __restrict__ out)
 {
-    // one thread per block slot (lane l of slot blk): its 9 components are
-    // read in turn, so the three entries of one MSR row segment come from
-    // the same cache line
+    // pass 1: one thread per block slot (lane l of slot blk) with column >=
+    // row (and padding): its 9 components through the MSR map, the three
+    // entries of one MSR row segment from the same cache line
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (long)gridDim.x * blockDim.x) {
+        if (tpos[i] >= 0) continue;
         const long blk = i >> 6, l = i & 63;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
@@ -95,6 +97,22 @@ __global__ void k_expand(long nb, const int* __restrict__ src, const VT* __restr
             const int s = src[o];
             out[o] = s >= 0 ? val[s] : VT(0);
         }
+    }
+}
+
+// pass 2: lower blocks as the transposes of their upper partners, written
+// by pass 1 (consecutive rows read consecutive lanes of the partner slot)
+template <class VT>
+__global__ void k_expand_lower(long nb, const int* __restrict__ tpos, VT* __restrict__ out)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (long)gridDim.x * blockDim.x) {
+        const int t = tpos[i];
+        if (t < 0) continue;
+        const long blk = i >> 6, l = i & 63;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) out[(blk * 9 + 3 * r + q) * 64 + l] = out[t + (3 * q + r) * 64];
     }
 }
 
@@ -275,7 +293,9 @@ void launch_expand(tg_ctx* c)
 {
     TimedRegion tr(c, T_EXPAND);
     const long n = c->op.nblocks_padded;
-    hipLaunchKernelGGL(k_expand<double>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p, c->dval.p,
+    hipLaunchKernelGGL(k_expand<double>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p,
+                       c->op.d_tpos.p, c->dval.p, c->op.d_val.p);
+    hipLaunchKernelGGL(k_expand_lower<double>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_tpos.p,
                        c->op.d_val.p);
     const long nn = 3L * c->nnd;
     hipLaunchKernelGGL(k_dinv<double>, dim3(grid_for(nn)), dim3(VB), 0, c->stream, nn, c->deqn.p, c->dval.p,
@@ -288,7 +308,9 @@ void launch_expand32(tg_ctx* c)
     TimedRegion tr(c, T_EXPAND);
     const long n = c->op.nblocks_padded;
     hipLaunchKernelGGL(k_expand<float>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p,
-                       c->dval32.p, c->op.d_val32.p);
+                       c->op.d_tpos.p, c->dval32.p, c->op.d_val32.p);
+    hipLaunchKernelGGL(k_expand_lower<float>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_tpos.p,
+                       c->op.d_val32.p);
     const long nn = 3L * c->nnd;
     hipLaunchKernelGGL(k_dinv<float>, dim3(grid_for(nn)), dim3(VB), 0, c->stream, nn, c->deqn.p, c->dval32.p,
                        c->vdinv.p);

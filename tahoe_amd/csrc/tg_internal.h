@@ -58,6 +58,8 @@ struct Bsell {
     DBuf<int> d_slice_base, d_slice_width;
     DBuf<int> d_col;                       // [nblocks_padded] column node, -1 = padding
     DBuf<int> d_src;                       // [nblocks_padded * 9] MSR val index, -1 = zero
+    DBuf<int> d_tpos;                      // [nblocks_padded] for a lower block (column < row): value
+                                           // index of its transpose partner, else -1
     DBuf<double> d_val;                    // [nblocks_padded * 9]
     DBuf<float> d_val32;                   // FP32 copy for the mixed-precision inner solve
     DBuf<int> d_oslot;                     // [nel][36][2] element block (a<=b) -> value index of

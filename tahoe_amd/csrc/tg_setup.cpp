@@ -226,6 +226,7 @@ void build_bsell(tg_ctx* c)
     B.d_slice_width.alloc(B.nslices);
     B.d_col.alloc((size_t)B.nblocks_padded);
     B.d_src.alloc((size_t)B.nblocks_padded * 9);
+    B.d_tpos.alloc((size_t)B.nblocks_padded);
     B.d_val.alloc((size_t)B.nblocks_padded * 9);
     TG_HIP(hipMemcpy(B.d_slice_base.p, B.slice_base.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice));
     TG_HIP(hipMemcpy(B.d_slice_width.p, B.slice_width.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice));
