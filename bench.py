@@ -70,7 +70,7 @@ def parse_args():
     ap.add_argument("--assembly", type=int, default=None,
                     help="Newton tangent: 0 into the MSR and expanded, 1 straight into the operator (default: autotuned)")
     ap.add_argument("--stiff-shape", type=int, default=None,
-                    help="stiffness workgroup shape 0..11 (default: autotuned)")
+                    help="stiffness workgroup shape 0..15 (default: autotuned)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the plain copy kernel (1 GiB) used to calibrate HBM counters")
     return ap.parse_args()

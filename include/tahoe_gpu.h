@@ -197,7 +197,10 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      6..8 for 16, 8 or 4 in the D*B form (D_q B shared
  *                      by an element's lanes through LDS), 9..11 for 12, 8
  *                      or 4 with the MSR read-modify-write loads issued
- *                      before the block products;
+ *                      before the block products, 12..14 for 12, 8 or 4
+ *                      and 15 for 16 block-major with the additions issued
+ *                      as memory-side atomics (order per entry unchanged:
+ *                      colours);
  * TG_TUNE_NEWTON_ASSEMBLY  where tg_newton / tg_newton_batch /
  *                      tg_newton_correction put the tangent: 0 (default)
  *                      into the MSR and then expanded into the inner solve's

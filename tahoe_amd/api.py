@@ -18,10 +18,10 @@ TOTAL_LAGRANGIAN, Q1P0 = 0, 1
 FP64, MIXED = 0, 1
 TUNE_STIFF_CHUNK, TUNE_STIFF_SHAPE, TUNE_NEWTON_ASSEMBLY, TUNE_PCG_GRAPH, TUNE_SPMV_XCD = 1, 2, 3, 4, 5
 PC_JACOBI, PC_BLOCK_JACOBI = 0, 1
-STIFF_SHAPE_EPB = (16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4)
-STIFF_SHAPE_BLOCK_MAJOR = (False, False, False, False, True, True, False, False, False, False, False, False)
-STIFF_SHAPE_DB_FORM = (False,) * 6 + (True,) * 3 + (False,) * 3
-STIFF_SHAPE_PREFETCH = (False,) * 9 + (True,) * 3
+STIFF_SHAPE_EPB = (16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4, 12, 8, 4, 16)
+STIFF_SHAPE_BLOCK_MAJOR = (False,) * 4 + (True, True) + (False,) * 9 + (True,)
+STIFF_SHAPE_DB_FORM = (False,) * 6 + (True,) * 3 + (False,) * 7
+STIFF_SHAPE_SCATTER = (0,) * 9 + (1,) * 3 + (2,) * 4   # 0 load-add-store, 1 loads first, 2 atomic adds
 
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)

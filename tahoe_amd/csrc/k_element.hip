@@ -166,10 +166,12 @@ __device__ __forceinline__ int upper_index(int m, int n) { return m * 24 - (m * 
 // barrier and scatter waits.  EPB (elements per block) and the phase-2 lane
 // order are template parameters; the autotuner picks among the compiled
 // shapes.
-constexpr int STIFF_EPB[] = {16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4};
-constexpr bool STIFF_PM[] = {false, false, false, false, true, true, false, false, false, false, false, false};
-constexpr bool STIFF_DB[] = {false, false, false, false, false, false, true, true, true, false, false, false};
-constexpr bool STIFF_PF[] = {false, false, false, false, false, false, false, false, false, true, true, true};
+constexpr int STIFF_EPB[] = {16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4, 12, 8, 4, 16};
+constexpr bool STIFF_PM[] = {false, false, false, false, true, true, false, false,
+                             false, false, false, false, false, false, false, true};
+constexpr bool STIFF_DB[] = {false, false, false, false, false, false, true, true,
+                             true, false, false, false, false, false, false, false};
+constexpr int STIFF_SC[] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 2, 2};
 constexpr int N_STIFF_SHAPES = sizeof(STIFF_EPB) / sizeof(STIFF_EPB[0]);
 
 // Voigt index of (i,k): (11,22,33,23,13,12)
@@ -238,7 +240,20 @@ __device__ __forceinline__ bool stiffness_ip(int q, bool valid, double2 mk, XU x
 // MSRMatrixT::Assemble) or the block and its transpose into the solver
 // operator through the operator slot map (TO_OP = true, prescribed rows and
 // columns left zero).
-template <class VT, bool TO_OP>
+// ATOM: the additions are issued as memory-side atomic adds with no return
+// value, so the lane does not wait for the old value; within one launch no
+// two lanes touch one entry (colouring), so the summation order per entry is
+// still the launch order.
+template <class VT>
+__device__ __forceinline__ void add_to(VT* p, VT v, bool atom)
+{
+    if (atom)
+        unsafeAtomicAdd(p, v);
+    else
+        *p += v;
+}
+
+template <class VT, bool TO_OP, bool ATOM = false>
 __device__ __forceinline__ void scatter_block(int e, int a, int b, int p, const VT (&K)[3][3],
                                               const int* __restrict__ conn, const int* __restrict__ slot,
                                               const int* __restrict__ eqn, VT* __restrict__ val)
@@ -260,10 +275,10 @@ __device__ __forceinline__ void scatter_block(int e, int a, int b, int p, const 
                 if (!(ra[i] && cb[j])) continue;
                 if (a == b) {
                     // both triangles of a diagonal block from the upper value
-                    val[b1 + (3 * i + j) * 64] += K[i < j ? i : j][i < j ? j : i];
+                    add_to(&val[b1 + (3 * i + j) * 64], K[i < j ? i : j][i < j ? j : i], ATOM);
                 } else {
-                    val[b1 + (3 * i + j) * 64] += K[i][j];
-                    if (b2 >= 0) val[b2 + (3 * j + i) * 64] += K[i][j];
+                    add_to(&val[b1 + (3 * i + j) * 64], K[i][j], ATOM);
+                    if (b2 >= 0) add_to(&val[b2 + (3 * j + i) * 64], K[i][j], ATOM);
                 }
             }
     } else {
@@ -275,7 +290,7 @@ __device__ __forceinline__ void scatter_block(int e, int a, int b, int p, const 
                 const int m = 3 * a + i, n = 3 * b + j;
                 if (n < m) continue;  // lower half of a diagonal block
                 const int s = es[upper_index(m, n)];
-                if (s >= 0) val[s] += K[i][j];
+                if (s >= 0) add_to(&val[s], K[i][j], ATOM);
             }
     }
 }
@@ -302,11 +317,12 @@ __device__ __forceinline__ void upper_block(int p, int& a, int& b)
 // same block p of many elements -- their scatter targets are neighbouring
 // rows -- and the LDS tables are element-minor so those lanes read
 // consecutive words.
-// PF (MSR path only): the slot indices and old values of the lane's 9
-// entries are loaded before the block products, so the read half of the
-// read-modify-write is in flight during the arithmetic (no other element of
-// the launch touches those entries).
-template <class VT, int EPB, bool TO_OP, bool PM, bool PF>
+// SC selects the scatter: 0 load-add-store after the block products; 1 (MSR
+// path) the slot indices and old values of the lane's 9 entries are loaded
+// before the block products, so the read half of the read-modify-write is in
+// flight during the arithmetic (no other element of the launch touches those
+// entries); 2 memory-side atomic adds with no return value.
+template <class VT, int EPB, bool TO_OP, bool PM, int SC>
 __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict__ elems, int count,
                                                              const int* __restrict__ conn,
                                                              const int* __restrict__ emat,
@@ -361,7 +377,7 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
     if (base + le < count) {
         int a, b;
         upper_block(p, a, b);
-        constexpr bool PRE = PF && !TO_OP;
+        constexpr bool PRE = SC == 1 && !TO_OP;
         int sidx[PRE ? 9 : 1];
         VT old[PRE ? 9 : 1];
         if (PRE) {
@@ -424,7 +440,7 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
                     if (sidx[k] >= 0) val[sidx[k]] = old[k] + K[i][j];
                 }
         } else {
-            scatter_block<VT, TO_OP>(elems[base + le], a, b, p, K, conn, slot, eqn, val);
+            scatter_block<VT, TO_OP, SC == 2>(elems[base + le], a, b, p, K, conn, slot, eqn, val);
         }
     }
     if (bad) atomicOr(status, 7);
@@ -589,7 +605,7 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_db(const int* __restrict
     if (bad) atomicOr(status, 7);
 }
 
-template <class VT, int EPB, bool TO_OP, bool PM, bool DB, bool PF>
+template <class VT, int EPB, bool TO_OP, bool PM, bool DB, int SC>
 static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u, VT* val)
 {
     const int blocks = (count + EPB - 1) / EPB;
@@ -598,7 +614,7 @@ static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u,
                            c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
                            TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
     else
-        hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM, PF>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
+        hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM, SC>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
                            c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
                            TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
 }
@@ -607,7 +623,7 @@ template <class VT, bool TO_OP>
 static void launch_stiffness_shape(tg_ctx* c, int v, int first, int count, const double* u, VT* val)
 {
 #define TG_SHAPE(i) \
-    launch_stiffness_v<VT, STIFF_EPB[i], TO_OP, STIFF_PM[i], STIFF_DB[i], STIFF_PF[i]>(c, first, count, u, val)
+    launch_stiffness_v<VT, STIFF_EPB[i], TO_OP, STIFF_PM[i], STIFF_DB[i], STIFF_SC[i]>(c, first, count, u, val)
     switch (v) {
     case 0: TG_SHAPE(0); break;
     case 1: TG_SHAPE(1); break;
@@ -620,7 +636,11 @@ static void launch_stiffness_shape(tg_ctx* c, int v, int first, int count, const
     case 8: TG_SHAPE(8); break;
     case 9: TG_SHAPE(9); break;
     case 10: TG_SHAPE(10); break;
-    default: TG_SHAPE(11); break;
+    case 11: TG_SHAPE(11); break;
+    case 12: TG_SHAPE(12); break;
+    case 13: TG_SHAPE(13); break;
+    case 14: TG_SHAPE(14); break;
+    default: TG_SHAPE(15); break;
     }
 #undef TG_SHAPE
 }

@@ -813,7 +813,7 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
             if (value < -1) fail(TG_BAD_INPUT, "stiffness chunk must be -1, 0 or positive");
             c->stiff_chunk = value;
         } else if (knob == TG_TUNE_STIFF_SHAPE) {
-            if (value < 0 || value > 11) fail(TG_OUT_OF_RANGE, "stiffness shape must be 0..11");
+            if (value < 0 || value > 15) fail(TG_OUT_OF_RANGE, "stiffness shape must be 0..15");
             c->stiff_variant = (int)value;
         } else if (knob == TG_TUNE_NEWTON_ASSEMBLY) {
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "Newton assembly must be 0 or 1");

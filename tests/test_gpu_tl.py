@@ -163,7 +163,7 @@ def test_large_mesh_against_oracle_and_invariants():
     assert abs(y @ s.multx(x) - x @ s.multx(y)) <= 1e-10 * np.linalg.norm(x) * np.linalg.norm(y) * np.abs(vo).max()
 
 
-@pytest.mark.parametrize("shape", list(range(12)))
+@pytest.mark.parametrize("shape", list(range(16)))
 @pytest.mark.parametrize("chunk", [0, 100])
 def test_stiffness_every_launch_shape(shape, chunk):
     """Every compiled workgroup shape and launch-range split assembles the
