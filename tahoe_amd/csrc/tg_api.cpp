@@ -125,11 +125,18 @@ static void setup_segments(tg_ctx* c)
     S.d_out.alloc((size_t)S.nseg);
     S.d_ndone.alloc(1);
     std::vector<int> ones((size_t)S.nseg, 1);
-    TG_HIP(hipMemcpy(S.d_slice_seg.p, slice_seg.data(), sizeof(int) * slice_seg.size(), hipMemcpyHostToDevice));
-    TG_HIP(hipMemcpy(S.d_seg_slice.p, seg_slice.data(), sizeof(int) * seg_slice.size(), hipMemcpyHostToDevice));
-    TG_HIP(hipMemcpy(S.d_active.p, ones.data(), sizeof(int) * ones.size(), hipMemcpyHostToDevice));
-    TG_HIP(hipMemcpy(S.d_elem_seg.p, S.elem_seg.data(), sizeof(int) * S.elem_seg.size(), hipMemcpyHostToDevice));
-    TG_HIP(hipMemset(S.d_scal.p, 0, sizeof(double) * S_NSCAL * S.nseg));
+    auto up = [&](int* dst, const std::vector<int>& v) {
+        TG_HIP(hipMemcpyAsync(dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, c->stream));
+    };
+    up(S.d_slice_seg.p, slice_seg);
+    up(S.d_seg_slice.p, seg_slice);
+    up(S.d_active.p, ones);
+    up(S.d_elem_seg.p, S.elem_seg);
+    TG_HIP(hipMemsetAsync(S.d_scal.p, 0, sizeof(double) * S_NSCAL * S.nseg, c->stream));
+    // every copy and clear of the context goes through its own stream (a
+    // null-stream operation is not ordered with a non-blocking stream), and
+    // the host vectors copied from must outlive the copies
+    TG_HIP(hipStreamSynchronize(c->stream));
 }
 
 static void check_status(tg_ctx* c)
@@ -185,9 +192,10 @@ int tg_create(tg_ctx** out, int device)
         TG_HIP(hipHostMalloc((void**)&c->hscal, sizeof(double) * 16, hipHostMallocDefault));
         TG_HIP(hipHostMalloc((void**)&c->hstatus, sizeof(int) * 4, hipHostMallocDefault));
         c->dstatus.alloc(4);
-        TG_HIP(hipMemset(c->dstatus.p, 0, sizeof(int) * 4));
+        TG_HIP(hipMemsetAsync(c->dstatus.p, 0, sizeof(int) * 4, c->stream));
         c->dscal.alloc(16);
-        TG_HIP(hipMemset(c->dscal.p, 0, sizeof(double) * 16));
+        TG_HIP(hipMemsetAsync(c->dscal.p, 0, sizeof(double) * 16, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
     });
     if (rc != TG_OK) {
         tg_destroy(c);
@@ -392,18 +400,24 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         for (size_t k = 0; k < kidx.size(); k++) kidx[k] = 3 * c->kbc_nodes[k] + c->kbc_dofs[k];
         c->dkbc_idx.alloc(kidx.size());
         c->dkbc_val.alloc(kidx.size());
-        TG_HIP(hipMemcpy(c->dX.p, c->X.data(), sizeof(double) * nn, hipMemcpyHostToDevice));
-        TG_HIP(hipMemset(c->du.p, 0, sizeof(double) * nn));
-        TG_HIP(hipMemset(c->du_last.p, 0, sizeof(double) * nn));
-        TG_HIP(hipMemcpy(c->dconn.p, c->conn.data(), sizeof(int) * c->conn.size(), hipMemcpyHostToDevice));
-        TG_HIP(hipMemcpy(c->dmat.p, c->mat.data(), sizeof(int) * c->mat.size(), hipMemcpyHostToDevice));
-        TG_HIP(hipMemcpy(c->deqn.p, eqn0.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
-        TG_HIP(hipMemcpy(c->dcolor_elems.p, c->color_elems.data(), sizeof(int) * c->nel, hipMemcpyHostToDevice));
-        TG_HIP(hipMemcpy(c->dmatprops.p, props.data(), sizeof(double2) * props.size(), hipMemcpyHostToDevice));
-        TG_HIP(hipMemcpy(c->dbindx.p, c->bindx.data(), sizeof(int) * c->msr_len, hipMemcpyHostToDevice));
+        TG_HIP(hipMemcpyAsync(c->dX.p, c->X.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+        TG_HIP(hipMemsetAsync(c->du.p, 0, sizeof(double) * nn, c->stream));
+        TG_HIP(hipMemsetAsync(c->du_last.p, 0, sizeof(double) * nn, c->stream));
+        TG_HIP(hipMemcpyAsync(c->dconn.p, c->conn.data(), sizeof(int) * c->conn.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        TG_HIP(hipMemcpyAsync(c->dmat.p, c->mat.data(), sizeof(int) * c->mat.size(), hipMemcpyHostToDevice, c->stream));
+        TG_HIP(hipMemcpyAsync(c->deqn.p, eqn0.data(), sizeof(int) * nn, hipMemcpyHostToDevice, c->stream));
+        TG_HIP(hipMemcpyAsync(c->dcolor_elems.p, c->color_elems.data(), sizeof(int) * c->nel, hipMemcpyHostToDevice,
+                              c->stream));
+        TG_HIP(hipMemcpyAsync(c->dmatprops.p, props.data(), sizeof(double2) * props.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        TG_HIP(hipMemcpyAsync(c->dbindx.p, c->bindx.data(), sizeof(int) * c->msr_len, hipMemcpyHostToDevice,
+                              c->stream));
         if (!kidx.empty()) {
-            TG_HIP(hipMemcpy(c->dkbc_idx.p, kidx.data(), sizeof(int) * kidx.size(), hipMemcpyHostToDevice));
-            TG_HIP(hipMemcpy(c->dkbc_val.p, c->kbc_vals.data(), sizeof(double) * kidx.size(), hipMemcpyHostToDevice));
+            TG_HIP(hipMemcpyAsync(c->dkbc_idx.p, kidx.data(), sizeof(int) * kidx.size(), hipMemcpyHostToDevice,
+                                  c->stream));
+            TG_HIP(hipMemcpyAsync(c->dkbc_val.p, c->kbc_vals.data(), sizeof(double) * kidx.size(),
+                                  hipMemcpyHostToDevice, c->stream));
         }
         phase("allocate + upload");
         launch_build_slots(c);

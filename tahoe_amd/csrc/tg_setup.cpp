@@ -228,10 +228,11 @@ void build_bsell(tg_ctx* c)
     B.d_src.alloc((size_t)B.nblocks_padded * 9);
     B.d_tpos.alloc((size_t)B.nblocks_padded);
     B.d_val.alloc((size_t)B.nblocks_padded * 9);
-    TG_HIP(hipMemcpy(B.d_slice_base.p, B.slice_base.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice));
-    TG_HIP(hipMemcpy(B.d_slice_width.p, B.slice_width.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice));
-    TG_HIP(hipMemcpy(B.d_col.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice));
+    TG_HIP(hipMemcpyAsync(B.d_slice_base.p, B.slice_base.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice, c->stream));
+    TG_HIP(hipMemcpyAsync(B.d_slice_width.p, B.slice_width.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice, c->stream));
+    TG_HIP(hipMemcpyAsync(B.d_col.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice, c->stream));
     launch_build_src(c);
+    TG_HIP(hipStreamSynchronize(c->stream));  // col (host) outlives its copy
 }
 
 }  // namespace tg

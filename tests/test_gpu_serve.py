@@ -17,7 +17,10 @@ def test_served_equals_alone():
     srv = serve.RVEServer(precision=api.MIXED, workers=3, max_batch=6)
     st = srv.run(reqs, realtime=True)
     assert len(st.results) == len(reqs)
-    assert all(x.status == 0 for x in st.results.values()), [(x.rid, x.status) for x in st.results.values()]
+    by_id = {r.rid: r for r in reqs}
+    assert all(x.status == 0 for x in st.results.values()), \
+        [(x.rid, by_id[x.rid].L, by_id[x.rid].formulation, x.status, x.load_steps, x.batch_size)
+         for x in st.results.values() if x.status != 0]
     assert max(b[1] for b in st.batches) > 1, "no request was batched with another"
     fp64 = api.Solid.opts(lin_rtol=1e-10)
     for r in reqs:
