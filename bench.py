@@ -35,7 +35,7 @@ METRIC = "elements/s assembled (R+K) + Newton solves/s; p50/p99 RVE latency vs p
 HBM_PEAK = 8.0e12        # B/s, MI355X HBM3E
 FP64_PEAK = 78.6e12      # flop/s, MI355X FP64 vector and matrix (dense)
 
-# algorithmic work per unit (see DESIGN.md "Measurement")
+# algorithmic work per element (flop counts of the kernels' arithmetic, not of the launch)
 RES_FLOP_PER_EL = 6.2e3          # residual: shape derivatives + kinematics + stress + B^T P
 STIFF_FLOP_PER_EL = 83.0e3       # stiffness: 8 x (2*24*6*24 + 2*6*6*24) material + geometric + kinematics
 
@@ -118,8 +118,8 @@ def main():
                           "launches": 11, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
         return
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU layout is "
-              "described in DESIGN.md and is not measured here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); a multi-GPU partition of the "
+              "mesh is not built or measured here.", file=sys.stderr)
         sys.exit(2)
     if a.workload == "mixed-steps":
         return bench_mixed_steps(a)
@@ -457,16 +457,20 @@ def bench_rve_serve(a):
     plain_wall = time.perf_counter() - t0
     pl = np.array(plain_lat)
     els = sum(r.elements for r in reqs)
+    plain_els = sum(r.elements for r in reqs[:n_plain])
     line = base_line(els / st.wall, "elements/s", len(reqs), 0, st.wall * 1e3 / len(reqs), "f32/f64",
                      {"workload": f"{len(reqs)} RVE requests, Poisson arrivals at {rate:.1f}/s "
                                   f"({a.serve_load:g} of measured capacity), edge log-uniform 4-32, 50% Q1 nu=0.3 "
-                                  "(mixed precision) / 50% Q1P0 nu=0.499 (FP64), Newton to 1e-10",
+                                  "/ 50% Q1P0 nu=0.499 (isochoric macro F), two-phase ratio 10, |H_ij| <= 0.02, "
+                                  "mixed precision (FP32 K, FP64 R and Newton), Newton to 1e-10",
                       "requests": len(reqs), "elements": els})
     line.update({"latency": lat, "capacity_solves_per_s": cap,
                  "plain_fp64": {"requests": n_plain, "p50_s": float(np.percentile(pl, 50)),
                                 "p99_s": float(np.percentile(pl, 99)), "wall_s": plain_wall,
-                                "solves_per_s": n_plain / plain_wall,
-                                "note": "same arrival times, one request at a time, FP64 throughout"},
+                                "solves_per_s": n_plain / plain_wall, "elements_per_s": plain_els / plain_wall,
+                                "note": "the first requests of the same stream at the same arrival times, one "
+                                        "request at a time on its own context, FP64 throughout"},
+                 "speedup_vs_plain_fp64": (els / st.wall) / (plain_els / plain_wall),
                  "max_rel_err_vs_fp64": max(errs) if errs else None,
                  "load_steps": {str(k): v for k, v in sorted(collections.Counter(
                      x.load_steps for x in st.results.values()).items())}})

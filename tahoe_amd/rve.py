@@ -5,9 +5,19 @@ Every request is a unit cube of L^3 voxel hexahedra, two-phase
 neo-Hookean (phase 2 with probability p2 per element, stiffness ratio
 `ratio`), loaded by the affine boundary displacement u = H X on all outer
 nodes with H_ij ~ U(-hmax, hmax).  The Newton predictor is the affine field
-u = H X on every node (the Taylor assumption of FE^2 homogenisation), which
-keeps the near-incompressible Q1P0 requests robust in a single load step.  Draws come from the counter-based
-SplitMix64 of tahoe_amd.mesh with one stream per request.
+u = H X on every node (the Taylor assumption of FE^2 homogenisation).
+
+Near-incompressible requests (Q1P0, nu = 0.499) take the isochoric part of
+the macro deformation, F = det(I+H)^(-1/3) (I+H): a macro material point of a
+near-incompressible solid does not change volume.  Imposing the raw F there
+(volumetric strain tr H up to 0.06 against kappa/mu = 500) drives the mean
+pressure to ~100 E, far above the shear modulus, where the tangent is
+indefinite (loss of material stability: Jacobi-PCG needs ~16k iterations on
+a 7^3 cube and the first correction inverts elements).  With the isochoric F
+each such request converges quadratically in one load step.
+
+Draws come from the counter-based SplitMix64 of tahoe_amd.mesh with one
+stream per request.
 """
 from __future__ import annotations
 
@@ -26,6 +36,7 @@ class Request:
     nu: float
     seed: int
     arrival: float = 0.0
+    isochoric: bool = False  # impose det(I+H)^(-1/3) (I+H) instead of I+H
 
     @property
     def elements(self) -> int:
@@ -37,6 +48,9 @@ class Request:
         m.mat = ph.astype(np.int32)
         hu = M.uniform01(self.seed, np.uint64(1 << 40) + np.arange(9, dtype=np.uint64))
         H = ((2.0 * hu - 1.0) * hmax).reshape(3, 3)
+        if self.isochoric:
+            F = np.eye(3) + H
+            H = F / np.cbrt(np.linalg.det(F)) - np.eye(3)
         # predictor: the affine field u = H X everywhere (exact for one phase)
         return batch.Problem(m, M.kbc_affine(m, H), [E1, ratio * E1], [self.nu, self.nu], m.X @ H.T)
 
@@ -56,7 +70,7 @@ def batch_requests(n=4096, lo=4, hi=24, seed=4242, nu=0.3):
 def stream_requests(n, rate, lo=4, hi=32, seed=777, q1p0_fraction=0.5):
     """Poisson arrivals (exponential gaps at `rate` per second), edge
     log-uniform in lo-hi, a fraction q1p0_fraction of Q1P0 mixed-pressure
-    requests at nu = 0.499 and the rest Q1 at nu = 0.3."""
+    requests at nu = 0.499 (isochoric macro F) and the rest Q1 at nu = 0.3."""
     L = log_uniform_edges(n, lo, hi, seed)
     ug = M.uniform01(seed, np.uint64(1 << 42) + np.arange(n, dtype=np.uint64))
     uk = M.uniform01(seed, np.uint64(1 << 43) + np.arange(n, dtype=np.uint64))
@@ -65,5 +79,5 @@ def stream_requests(n, rate, lo=4, hi=32, seed=777, q1p0_fraction=0.5):
     for i in range(n):
         q1p0 = uk[i] < q1p0_fraction
         out.append(Request(i, int(L[i]), api.Q1P0 if q1p0 else api.TOTAL_LAGRANGIAN, 0.499 if q1p0 else 0.3,
-                           seed * 100003 + i, float(t[i])))
+                           seed * 100003 + i, float(t[i]), isochoric=bool(q1p0)))
     return out

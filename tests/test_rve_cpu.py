@@ -1,0 +1,116 @@
+"""Host side of the batched and served RVE solves (no GPU): the request
+generator, the disjoint-union packing of sub-problems and the admission
+footprint; the CPU oracle checks that a near-incompressible request with
+its isochoric macro deformation converges in one load step from the affine
+predictor."""
+import numpy as np
+import pytest
+
+from tahoe_amd import api, batch, rve, serve
+from tahoe_amd import mesh as M
+
+
+def test_edges_log_uniform_in_range():
+    L = rve.log_uniform_edges(4000, 4, 24, seed=5)
+    assert L.min() == 4 and L.max() == 24
+    # log-uniform: the median edge is near sqrt(4 * 24) ~ 9.8
+    assert 8 <= np.median(L) <= 12
+
+
+def test_stream_requests_arrivals_and_mix():
+    reqs = rve.stream_requests(400, rate=50.0, lo=4, hi=32, seed=3)
+    t = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(t) >= 0)
+    assert abs(t[-1] / len(reqs) - 1 / 50.0) < 0.3 / 50.0  # mean gap 1/rate
+    q = [r for r in reqs if r.formulation == api.Q1P0]
+    assert 0.35 < len(q) / len(reqs) < 0.65
+    assert all(r.nu == 0.499 and r.isochoric for r in q)
+    assert all(r.nu == 0.3 and not r.isochoric for r in reqs if r.formulation == api.TOTAL_LAGRANGIAN)
+
+
+@pytest.mark.parametrize("iso", [False, True])
+def test_request_boundary_is_affine(iso):
+    r = rve.Request(0, 5, api.Q1P0, 0.499, seed=123, isochoric=iso)
+    p = r.problem()
+    m = p.mesh
+    # the prescribed values are u = H X on the outer nodes, and the predictor is the same field
+    u = np.zeros((m.nnd, 3))
+    u[p.kbc.nodes, p.kbc.dofs] = p.kbc.values
+    bn = np.unique(p.kbc.nodes)
+    assert np.array_equal(bn, M.boundary_nodes(m))
+    np.testing.assert_allclose(p.u0[bn], u[bn], rtol=0, atol=1e-15)
+    # recover H from the predictor: u0 = X H^T
+    H = np.linalg.lstsq(m.X, p.u0, rcond=None)[0].T
+    if iso:
+        # the same draw, scaled to unit volume: F = det(F_raw)^(-1/3) F_raw
+        raw = rve.Request(0, 5, api.Q1P0, 0.499, seed=123).problem()
+        Fr = np.eye(3) + np.linalg.lstsq(m.X, raw.u0, rcond=None)[0].T
+        np.testing.assert_allclose(np.eye(3) + H, Fr / np.cbrt(np.linalg.det(Fr)), rtol=0, atol=1e-13)
+        assert abs(np.linalg.det(np.eye(3) + H) - 1.0) < 1e-13
+    else:
+        assert np.abs(H).max() <= 0.02
+        assert abs(np.linalg.det(np.eye(3) + H) - 1.0) > 1e-6
+
+
+def test_pack_segments_on_64_node_boundaries():
+    reqs = [rve.Request(i, L, api.TOTAL_LAGRANGIAN, 0.3, seed=10 + i) for i, L in enumerate((3, 5, 4))]
+    probs = [r.problem() for r in reqs]
+    pk = batch.pack(probs)
+    assert np.all(pk.node_begin % 64 == 0)
+    assert list(pk.node_count) == [p.mesh.nnd for p in probs]
+    # every node outside a sub-problem's real range is padding with all three dofs fixed at zero
+    fixed = np.zeros((pk.X.shape[0], 3), dtype=bool)
+    fixed[pk.kbc.nodes, pk.kbc.dofs] = True
+    used = np.zeros(pk.X.shape[0], dtype=bool)
+    used[pk.conn.ravel()] = True
+    for b, n, e in zip(pk.node_begin[:-1], pk.node_count, pk.node_begin[1:]):
+        assert used[b:b + n].all() and not used[b + n:e].any()
+        assert fixed[b + n:e].all()
+    pad_cards = pk.kbc.values[~used[pk.kbc.nodes]]
+    assert np.all(pad_cards == 0)
+    # materials: one global table, both phases of the three requests share two entries
+    assert pk.materials == [(1000.0, 0.3), (10000.0, 0.3)]
+    # unpack returns each sub-problem's real nodes
+    u = np.arange(pk.X.shape[0] * 3, dtype=float).reshape(-1, 3)
+    parts = batch.unpack(pk, u)
+    for part, b, n in zip(parts, pk.node_begin, pk.node_count):
+        assert np.array_equal(part, u[b:b + n])
+
+
+def test_footprint_grows_with_size_and_precision():
+    small = rve.Request(0, 4, api.TOTAL_LAGRANGIAN, 0.3, 1)
+    big = rve.Request(1, 32, api.TOTAL_LAGRANGIAN, 0.3, 1)
+    assert serve.footprint_bytes(big) > 100 * serve.footprint_bytes(small) > 0
+    assert serve.footprint_bytes(big, api.MIXED) > serve.footprint_bytes(big, api.FP64)
+    # a 32^3 request stays a small fraction of one GPU's memory
+    assert serve.footprint_bytes(big) < 1 << 30
+
+
+def test_isochoric_q1p0_request_converges_in_one_step_on_oracle():
+    """Newton from the affine predictor at the full load (the served
+    single-step path) on the CPU oracle: quadratic convergence to 1e-10."""
+    from oracle import oracle as O
+    r = rve.Request(7, 4, api.Q1P0, 0.499, seed=777 * 100003 + 5, isochoric=True)
+    p = r.problem()
+    m = p.mesh
+    eq, n_eq = M.equation_numbers(m.nnd, p.kbc)
+    bindx = O.msr_pattern(n_eq, m.conn, eq)
+    mu = [api.mu_kappa(E, nu)[0] for E, nu in zip(p.E, p.nu)]
+    ka = [api.mu_kappa(E, nu)[1] for E, nu in zip(p.E, p.nu)]
+    u = p.u0.copy()
+    u[p.kbc.nodes, p.kbc.dofs] = p.kbc.values
+    R = O.residual(api.Q1P0, m.X, m.conn, m.mat, mu, ka, u, eq, n_eq)
+    e0, hist = np.linalg.norm(R), []
+    free = eq > 0
+    for _ in range(8):
+        val = O.stiffness(api.Q1P0, m.X, m.conn, m.mat, mu, ka, u, eq, n_eq, bindx)
+        du, its, rr, rc = O.pcg_jacobi(bindx, val, R, rtol=1e-12)
+        assert rc == 0 and its < 2000
+        u[free] += du[eq[free] - 1]
+        R = O.residual(api.Q1P0, m.X, m.conn, m.mat, mu, ka, u, eq, n_eq)
+        hist.append(np.linalg.norm(R) / e0)
+        if hist[-1] < 1e-10:
+            break
+    assert hist[-1] < 1e-10 and len(hist) <= 6, hist
+    # quadratic convergence before the round-off floor: a correction gains at least two digits
+    assert hist[-2] < 1e-2 * hist[-3], hist
