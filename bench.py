@@ -49,9 +49,10 @@ def parse_args():
     ap.add_argument("--lin-rtol", type=float, default=1e-10)
     ap.add_argument("--cpu-n", type=int, default=64, help="edge of the CPU-baseline sample cube (0: skip)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--precond", choices=["jacobi", "block"], default="jacobi",
-                    help="inner PCG preconditioner: point Jacobi (the reference's DiagonalMatrixT) or 3x3 nodal "
-                         "block Jacobi")
+    ap.add_argument("--precond", choices=["jacobi", "block", "amg"], default="jacobi",
+                    help="inner PCG preconditioner: point Jacobi (the reference's DiagonalMatrixT), 3x3 nodal "
+                         "block Jacobi, or one V-cycle of aggregation multigrid")
+    ap.add_argument("--mixed-n", type=int, default=24, help="cube edge of the mixed-steps workload")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: 2 Newton corrections with 48 PCG iterations each")
     ap.add_argument("--workload", choices=["cube", "mixed-steps", "rve-batch", "rve-serve"], default="cube",
@@ -74,6 +75,10 @@ def parse_args():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only the plain copy kernel (1 GiB) used to calibrate HBM counters")
     return ap.parse_args()
+
+
+PRECOND = {"jacobi": 0, "block": 1, "amg": 2}   # api.PC_JACOBI, PC_BLOCK_JACOBI, PC_AMG
+PC_NAME = {"jacobi": "Jacobi", "block": "block-Jacobi", "amg": "multigrid"}
 
 
 def traffic_from_profiles():
@@ -139,7 +144,7 @@ def main():
     s.set_formulation(api.TOTAL_LAGRANGIAN)
     s.set_precision(api.MIXED if a.precision == "mixed" else api.FP64)
     s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
-    s.set_preconditioner(api.PC_BLOCK_JACOBI if a.precond == "block" else api.PC_JACOBI)
+    s.set_preconditioner(PRECOND[a.precond])
     n_eq, msr_len = s.initialize()
     print(f"# setup {time.perf_counter() - t_setup:.1f} s: nel={m.nel} nnd={m.nnd} n_eq={n_eq} "
           f"msr_upper_nnz={msr_len - 1}", file=sys.stderr, flush=True)
@@ -218,7 +223,7 @@ def main():
     res_gflops = RES_FLOP_PER_EL * m.nel / (r_ms * 1e-3) / 1e9
     stf_gflops = STIFF_FLOP_PER_EL * m.nel / (k_ms * 1e-3) / 1e9
     its = float(np.mean(lin_its))
-    pcg_ms = (tm["spmv_ms"] + tm["vector_ms"]) / K
+    pcg_ms = (tm["spmv_ms"] + tm["vector_ms"] + tm["precond_ms"]) / K
     line = {
         "metric": METRIC,
         "value": value,
@@ -234,7 +239,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{n}^3 Q1 hex TL neo-Hookean (Simo_isotropic E=1000 nu=0.3), +-5% perturbed, "
                                "uniaxial 5% load step 1/5, one Newton correction: R + K into the solver operator + "
-                               f"{'block-' if a.precond == 'block' else ''}Jacobi-PCG to {a.lin_rtol:g}",
+                               f"{PC_NAME[a.precond]}-PCG to {a.lin_rtol:g}",
                    "elements": m.nel, "nodes": m.nnd, "equations": n_eq, "msr_upper_nnz": msr_len - 1,
                    "newton_assembly": "operator" if tuned["assembly"] == 1 else "msr+expansion",
                    "stiffness_shape_elements_per_workgroup": api.STIFF_SHAPE_EPB[tuned["shape"]],
@@ -253,7 +258,8 @@ def main():
                       "frac_fp64_peak": stf_gflops * 1e9 / FP64_PEAK},
         "pcg": {"iterations_per_step": its, "ms_per_step": pcg_ms,
                 "dof_iterations_per_s": n_eq * its / (pcg_ms * 1e-3) if pcg_ms > 0 else None,
-                "expand_ms": tm["expand_ms"] / K},
+                "expand_ms": tm["expand_ms"] / K, "preconditioner_ms": tm["precond_ms"] / K,
+                "preconditioner": a.precond},
         "timing_ms": tm,
         "autotune_tangent_ms": {k: {str(x): y for x, y in v.items()} for k, v in tuned["ms"].items()},
         "autotune_spmv_ms": {str(k): v for k, v in tuned_spmv["ms"].items()},
@@ -272,7 +278,7 @@ def bench_mixed_steps(a):
     residual every step), solved in FP64 and in mixed precision on the same
     mesh; relative displacement error of mixed vs FP64 after every step."""
     from tahoe_amd import api, mesh as M
-    n = a.n if a.n != 128 else 24
+    n = a.mixed_n
     m = M.cube(n, seed=2024)
     kbc = M.kbc_uniaxial(m, 0.2)
     res = {}
@@ -282,25 +288,40 @@ def bench_mixed_steps(a):
         s.set_material(0, *api.mu_kappa(1000.0, 0.3))
         s.set_precision(api.MIXED if prec == "mixed" else api.FP64)
         s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
+        s.set_preconditioner(PRECOND[a.precond])
         n_eq, _ = s.initialize()
         opts = s.opts(lin_rtol=1e-10 if prec == "fp64" else 1e-6)
         s.timing_enable(True)
         t0 = time.perf_counter()
-        us, its = api.static_solve(s, 10, opts)
+        lin = []
+
+        def on_step(k, st, prec=prec, t0=t0, lin=lin):
+            lin.append(st.lin_iterations)
+            print(f"# {prec} step {k}: Newton {st.iterations}, PCG {st.lin_iterations}, "
+                  f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+        def log(msg, prec=prec):
+            print(f"# {prec} {msg}", file=sys.stderr, flush=True)
+
+        us, its, subs = api.static_solve_adaptive(s, 10, opts, on_step=on_step, log=log)
         s.synchronize()
         wall = time.perf_counter() - t0
-        res[prec] = {"u": us, "newton": its, "wall_s": wall, "timing": s.timing()}
+        res[prec] = {"u": us, "newton": its, "substeps": subs, "pcg": lin, "wall_s": wall, "timing": s.timing()}
         print(f"# {prec}: {wall:.2f} s, Newton per step {its}", file=sys.stderr, flush=True)
+        s.close()
     err = [float(np.abs(um - ud).max() / np.abs(ud).max()) for um, ud in zip(res["mixed"]["u"], res["fp64"]["u"])]
     tot = lambda r: r["timing"]["residual_ms"] + r["timing"]["stiffness_ms"]
     line = base_line(m.nel * sum(res["mixed"]["newton"]) / (tot(res["mixed"]) * 1e-3), "elements/s", 10, 0,
                      res["mixed"]["wall_s"] * 1e3 / 10, "f32/f64",
                      {"workload": f"{n}^3 Q1 hex TL neo-Hookean, +-5% perturbed, uniaxial 20% in 10 load steps, "
-                                  "Newton to 1e-10; mixed: FP32 K + FP32 Jacobi-PCG to 1e-6 inside FP64 "
-                                  "residual/Newton; FP64: Jacobi-PCG to 1e-10",
-                      "elements": m.nel, "equations": n_eq})
+                                  "(a load step that fails is cut in half and retried), "
+                                  "Newton to 1e-10; mixed: FP32 K and operator, PCG to 1e-6 inside FP64 "
+                                  f"residual/Newton; FP64: PCG to 1e-10; preconditioner {a.precond}",
+                      "elements": m.nel, "equations": n_eq, "preconditioner": a.precond})
     line.update({"rel_err_vs_fp64_per_step": err, "max_rel_err": max(err),
                  "newton_per_step": {"mixed": res["mixed"]["newton"], "fp64": res["fp64"]["newton"]},
+                 "pcg_per_step": {"mixed": res["mixed"]["pcg"], "fp64": res["fp64"]["pcg"]},
+                 "substeps_per_step": {"mixed": res["mixed"]["substeps"], "fp64": res["fp64"]["substeps"]},
                  "wall_s": {"mixed": res["mixed"]["wall_s"], "fp64": res["fp64"]["wall_s"]},
                  "speedup_wall": res["fp64"]["wall_s"] / res["mixed"]["wall_s"],
                  "timing_ms": {k: res[k]["timing"] for k in res}})

@@ -80,6 +80,7 @@ typedef struct {
     double expand_ms;    long expand_launches;
     double spmv_ms;      long spmv_launches;
     double vector_ms;    long vector_launches;
+    double precond_ms;   long precond_launches;   /* multigrid V-cycles (TG_PC_AMG) */
 } tg_timing;
 
 /* ---- handle ---------------------------------------------------------- */
@@ -100,9 +101,13 @@ int tg_set_formulation(tg_ctx* ctx, int formulation);
 int tg_set_precision(tg_ctx* ctx, int precision);
 /* Preconditioner of the inner PCG: TG_PC_JACOBI (default) is the
  * reference's DiagonalMatrixT (1/diag, DiagonalMatrixT.cpp:267-324);
- * TG_PC_BLOCK_JACOBI inverts each node's 3x3 diagonal block (no counterpart
- * in the reference).  Either may be set before or after tg_initialize. */
-enum { TG_PC_JACOBI = 0, TG_PC_BLOCK_JACOBI = 1 };
+ * TG_PC_BLOCK_JACOBI inverts each node's 3x3 diagonal block; TG_PC_AMG is
+ * one V-cycle of aggregation multigrid (rigid-body modes, Chebyshev
+ * smoothing, Galerkin coarse operators re-formed with every tangent) for a
+ * single problem (no segments).  The last two have no counterpart in the
+ * reference.  Any may be set before or after tg_initialize; the multigrid
+ * structure is built on the host when first needed. */
+enum { TG_PC_JACOBI = 0, TG_PC_BLOCK_JACOBI = 1, TG_PC_AMG = 2 };
 int tg_set_preconditioner(tg_ctx* ctx, int pc);
 /* Kinematic BC cards: prescribed value = values[k] * t (KBC_CardT::Value,
  * tahoe/src/primitives/KBC_CardT.cpp:53-64, linear schedule). */
@@ -218,8 +223,14 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      contiguous eighth of the rows instead of the hardware
  *                      order; bit 1: branch-free inner loop unrolled by 3
  *                      (padding slots multiply zeros).  0 (default) .. 3. */
+/* TG_TUNE_AMG_DEGREE   multigrid: Chebyshev degree of each pre- and
+ *                      post-smoothing pass, 2 (default) .. 4. */
+/* TG_TUNE_AMG_WEIGHT   multigrid: weight of the coarse-grid correction in
+ *                      percent, 100 .. 190 (default 150: unsmoothed
+ *                      aggregation under-corrects; any weight in (0, 2)
+ *                      keeps the V-cycle symmetric positive definite). */
 enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3, TG_TUNE_PCG_GRAPH = 4,
-       TG_TUNE_SPMV_XCD = 5 };
+       TG_TUNE_SPMV_XCD = 5, TG_TUNE_AMG_DEGREE = 6, TG_TUNE_AMG_WEIGHT = 7 };
 int tg_set_tuning(tg_ctx* ctx, int knob, long value);
 int tg_get_tuning(tg_ctx* ctx, int knob, long* value);
 

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "libtahoe_gpu.so")
 TOTAL_LAGRANGIAN, Q1P0 = 0, 1
 FP64, MIXED = 0, 1
 TUNE_STIFF_CHUNK, TUNE_STIFF_SHAPE, TUNE_NEWTON_ASSEMBLY, TUNE_PCG_GRAPH, TUNE_SPMV_XCD = 1, 2, 3, 4, 5
-PC_JACOBI, PC_BLOCK_JACOBI = 0, 1
+TUNE_AMG_DEGREE, TUNE_AMG_WEIGHT = 6, 7
+PC_JACOBI, PC_BLOCK_JACOBI, PC_AMG = 0, 1, 2
 STIFF_SHAPE_EPB = (16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4, 12, 8, 4, 16)
 STIFF_SHAPE_BLOCK_MAJOR = (False,) * 4 + (True, True) + (False,) * 9 + (True,)
 STIFF_SHAPE_DB_FORM = (False,) * 6 + (True,) * 3 + (False,) * 7
@@ -57,7 +58,8 @@ class Timing(C.Structure):
                 ("stiffness_ms", C.c_double), ("stiffness_launches", C.c_long),
                 ("expand_ms", C.c_double), ("expand_launches", C.c_long),
                 ("spmv_ms", C.c_double), ("spmv_launches", C.c_long),
-                ("vector_ms", C.c_double), ("vector_launches", C.c_long)]
+                ("vector_ms", C.c_double), ("vector_launches", C.c_long),
+                ("precond_ms", C.c_double), ("precond_launches", C.c_long)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -335,6 +337,52 @@ def static_solve(solid: Solid, num_steps: int, opts=None, on_step=None):
         if on_step:
             on_step(s, st)
     return np.array(out_u), iters
+
+
+def static_solve_adaptive(solid: Solid, num_steps: int, opts=None, on_step=None, min_dt=1.0 / 4096, log=None):
+    """The static time loop to the checkpoints t = s/num_steps with step
+    cutting inside each interval: a sub-step whose Newton iteration fails
+    (J <= 0, divergence, iteration limit) is reset and retried at half the
+    increment, and a converged sub-step doubles the next one, never past the
+    next checkpoint (FEManagerT's time-step cutting, FEManagerT.cpp:243-266,
+    with ResetStep).  Returns the converged displacement at every checkpoint,
+    the Newton corrections summed per checkpoint and the sub-step counts."""
+    out_u, iters, subs = [], [], []
+    t, dt = 0.0, 1.0 / num_steps
+    for k in range(1, num_steps + 1):
+        tk = k / num_steps
+        n_newton, n_sub, lin = 0, 0, 0
+        while t < tk - 1e-12:
+            dt = min(dt, tk - t)
+            solid.apply_kbc(t + dt)
+            try:
+                st = solid.newton(opts)
+                ok = st.status == 0
+            except TahoeGpuError as e:
+                if e.code != 7:
+                    raise
+                ok = False
+            if log:
+                log(f"t={t:.5g}+{dt:.5g}: {'converged' if ok else 'cut'}")
+            if ok:
+                solid.close_step()
+                t += dt
+                n_newton += st.iterations
+                lin += st.lin_iterations
+                n_sub += 1
+                dt *= 2.0
+            else:
+                solid.reset_step()
+                dt *= 0.5
+                if dt < min_dt:
+                    raise TahoeGpuError(1, f"load step below {min_dt} at t = {t}")
+        t = tk
+        iters.append(n_newton)
+        subs.append(n_sub)
+        out_u.append(solid.displacement())
+        if on_step:
+            on_step(k, NewtonStats(0, n_newton, 0.0, 0.0, lin))
+    return np.array(out_u), iters, subs
 
 
 def mu_kappa(E: float, nu: float):

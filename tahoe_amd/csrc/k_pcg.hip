@@ -32,11 +32,15 @@ __device__ __forceinline__ double* sf(double* s, int field, int nseg, int seg) {
 
 // z = M^-1 r at node a.  PC = 0: Jacobi, m = 1/diag [nnd][3]
 // (DiagonalMatrixT); PC = 1: 3x3 nodal block Jacobi, m = inverse diagonal
-// blocks [nnd][9] (row-major, prescribed rows and columns zero).
+// blocks [nnd][9] (row-major, prescribed rows and columns zero); PC = 2:
+// multigrid, m = z itself, computed beforehand by a V-cycle.
 template <int PC>
 __device__ __forceinline__ void precondition(const double* __restrict__ m, long a, const double (&r)[3], double (&z)[3])
 {
-    if (PC == 0) {
+    if (PC == 2) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) z[i] = m[3 * a + i];
+    } else if (PC == 0) {
 #pragma unroll
         for (int i = 0; i < 3; i++) z[i] = m[3 * a + i] * r[i];
     } else {
@@ -215,7 +219,8 @@ __global__ void __launch_bounds__(FB) k_seg_fin_alpha(const double* __restrict__
     }
 }
 
-// x += alpha p, r -= alpha q; per-slice partials of r.M^-1 r and r.r
+// x += alpha p, r -= alpha q; per-slice partials of r.M^-1 r and r.r (with
+// the multigrid, r.r only: r.z follows the V-cycle)
 template <int PC>
 __global__ void __launch_bounds__(PB) k_seg_update(int nnd, const int* __restrict__ slice_seg,
                                                    const double* __restrict__ p, const double* __restrict__ q,
@@ -239,17 +244,17 @@ __global__ void __launch_bounds__(PB) k_seg_update(int nnd, const int* __restric
             ri[i] = r[k] - alpha * q[k];
             r[k] = ri[i];
         }
-        precondition<PC>(m, a, ri, zi);
+        if (PC != 2) precondition<PC>(m, a, ri, zi);
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            v0 += ri[i] * zi[i];
+            if (PC != 2) v0 += ri[i] * zi[i];
             v1 += ri[i] * ri[i];
         }
     }
     v0 = wsum(v0);
     v1 = wsum(v1);
     if ((a & 63) == 0) {
-        part[s] = v0;
+        if (PC != 2) part[s] = v0;
         part[nslices + s] = v1;
     }
 }
@@ -352,6 +357,8 @@ void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, doub
     (void)nactive;
     if (c->precond == TG_PC_BLOCK_JACOBI)
         pcg_loop<MT, 1>(c, val, c->vbinv.p, b, x, r, p, q, rtol, maxit, d_active, st);
+    else if (c->precond == TG_PC_AMG)
+        pcg_loop<MT, 2>(c, val, c->amg.lv[0]->d_x.p, b, x, r, p, q, rtol, maxit, d_active, st);
     else
         pcg_loop<MT, 0>(c, val, dinv, b, x, r, p, q, rtol, maxit, d_active, st);
 }
@@ -365,13 +372,19 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
     const int nseg = S.nseg;
     const unsigned nb = node_blocks(c);
     TG_HIP(hipMemsetAsync(S.d_ndone.p, 0, sizeof(int), c->stream));
+    // multigrid: dinv is the z vector; *done of the single segment gates the V-cycle
+    const double* done = S.d_scal.p + S_DONE * nseg;
+    if (PC == 2) {
+        TimedRegion tr(c, T_PRECOND);
+        amg_vcycle<MT>(c, val, b, (double*)dinv, nullptr);
+    }
     {
         TimedRegion tr(c, T_VECTOR);
         hipLaunchKernelGGL(k_seg_init<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, b, dinv, x, r, p, S.d_part.p, ns);
         hipLaunchKernelGGL(k_seg_fin_init, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns, S.d_seg_slice.p,
                            d_active, nseg, S.d_scal.p, rtol, maxit, S.d_ndone.p);
     }
-    const int CHUNK = 16;
+    const int CHUNK = PC == 2 ? 8 : 16;
     auto issue_chunk = [&]() {
         for (int k = 0; k < CHUNK; k++) {
             {
@@ -394,6 +407,14 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
                                    S.d_seg_slice.p, nseg, S.d_scal.p);
                 hipLaunchKernelGGL(k_seg_update<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, p, q,
                                    dinv, x, r, S.d_part.p, ns, nseg, S.d_scal.p);
+            }
+            if (PC == 2) {
+                TimedRegion tr(c, T_PRECOND);
+                amg_vcycle<MT>(c, val, r, (double*)dinv, done);
+                hipLaunchKernelGGL(k_seg_dot, dim3(nb), dim3(PB), 0, c->stream, c->nnd, r, dinv, S.d_part.p, ns);
+            }
+            {
+                TimedRegion tr(c, T_VECTOR);
                 hipLaunchKernelGGL(k_seg_fin_beta, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
                                    S.d_seg_slice.p, nseg, S.d_scal.p, S.d_ndone.p);
                 hipLaunchKernelGGL(k_seg_direction<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, r,
@@ -408,8 +429,8 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
     const bool graph = c->pcg_graph && !c->timing.enabled;
     if (graph) {
         const void* key[12] = {val, dinv, x, r, p, q, S.d_part.p, S.d_scal.p, S.d_ndone.p,
-                               (const void*)(long)(PC + 2 * c->spmv_xcd), (const void*)(long)sizeof(MT),
-                               (const void*)(long)c->nnd};
+                               (const void*)(long)(PC + 4 * c->spmv_xcd + 16L * c->amg.gen),
+                               (const void*)(long)sizeof(MT), (const void*)(long)c->nnd};
         if (!c->pg.exec || std::memcmp(key, c->pg.key, sizeof key) != 0) {
             if (c->pg.exec) TG_HIP(hipGraphExecDestroy(c->pg.exec));
             if (c->pg.graph) TG_HIP(hipGraphDestroy(c->pg.graph));

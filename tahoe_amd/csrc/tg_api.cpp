@@ -78,6 +78,7 @@ static void drain_events(tg_ctx* c)
             case T_STIFFNESS: a.stiffness_ms += ms; a.stiffness_launches++; break;
             case T_EXPAND: a.expand_ms += ms; a.expand_launches++; break;
             case T_SPMV: a.spmv_ms += ms; a.spmv_launches++; break;
+            case T_PRECOND: a.precond_ms += ms; a.precond_launches++; break;
             default: a.vector_ms += ms; a.vector_launches++; break;
         }
         (void)hipEventDestroy(p.e0);
@@ -318,9 +319,14 @@ int tg_set_preconditioner(tg_ctx* c, int pc)
 {
     if (!c) return TG_BAD_INPUT;
     return guarded(c, [&] {
-        if (pc != TG_PC_JACOBI && pc != TG_PC_BLOCK_JACOBI) fail(TG_BAD_INPUT, "unknown preconditioner");
+        if (pc != TG_PC_JACOBI && pc != TG_PC_BLOCK_JACOBI && pc != TG_PC_AMG) fail(TG_BAD_INPUT, "unknown preconditioner");
+        if (pc == TG_PC_AMG && c->seg.nseg != 1)
+            fail(TG_BAD_INPUT, "the multigrid preconditioner needs a single problem (no segments)");
         c->precond = pc;
-        if (c->initialized) c->vbinv.alloc(pc == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
+        if (c->initialized) {
+            c->vbinv.alloc(pc == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
+            if (pc == TG_PC_AMG && c->amg.lv.empty()) amg_setup(c);
+        }
     });
 }
 
@@ -435,6 +441,12 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         c->vbinv.alloc(c->precond == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
         c->op.d_val32.alloc(mixed ? (size_t)c->op.nblocks_padded * 9 : 0);
         phase("operator slot map");
+        if (c->precond == TG_PC_AMG) {
+            amg_setup(c);
+            phase("multigrid structure");
+        } else {
+            c->amg.clear();
+        }
         std::vector<long>().swap(c->nptr);
         std::vector<int>().swap(c->nlist);
         setup_segments(c);
@@ -550,6 +562,13 @@ static void fp64_values(tg_ctx* c)
     if (mixed_values(c)) widen_values(c);
 }
 
+// the preconditioner's data for the current operator values
+static void prepare_preconditioner(tg_ctx* c)
+{
+    launch_block_inverse(c);
+    if (c->precond == TG_PC_AMG) amg_numeric(c);
+}
+
 // operator of the inner linear solve, in the solve's precision
 static void prepare_operator(tg_ctx* c)
 {
@@ -557,7 +576,7 @@ static void prepare_operator(tg_ctx* c)
         launch_expand32(c);
     else
         launch_expand(c);
-    launch_block_inverse(c);
+    prepare_preconditioner(c);
 }
 
 // Newton's tangent in the inner solve's operator: straight into the block
@@ -567,7 +586,7 @@ static void newton_operator(tg_ctx* c)
     if (c->formulation == TG_TOTAL_LAGRANGIAN && c->newton_assembly == 1 && c->op.d_oslot.p) {
         launch_stiffness_operator(c, c->du.p);
         launch_operator_dinv(c);
-        launch_block_inverse(c);
+        prepare_preconditioner(c);
     } else {
         launch_stiffness(c, c->du.p);
         prepare_operator(c);
@@ -838,6 +857,14 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
         } else if (knob == TG_TUNE_SPMV_XCD) {
             if (value < 0 || value > 3) fail(TG_OUT_OF_RANGE, "SpMV variant must be 0..3");
             c->spmv_xcd = (int)value;
+        } else if (knob == TG_TUNE_AMG_DEGREE) {
+            if (value < 2 || value > 4) fail(TG_OUT_OF_RANGE, "multigrid smoother degree must be 2..4");
+            c->amg.degree = (int)value;
+            c->amg.gen++;
+        } else if (knob == TG_TUNE_AMG_WEIGHT) {
+            if (value < 100 || value > 190) fail(TG_OUT_OF_RANGE, "multigrid coarse weight must be 100..190");
+            c->amg.weight = (int)value;
+            c->amg.gen++;
         } else
             fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
     });
@@ -857,6 +884,10 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
             *value = c->pcg_graph;
         else if (knob == TG_TUNE_SPMV_XCD)
             *value = c->spmv_xcd;
+        else if (knob == TG_TUNE_AMG_DEGREE)
+            *value = c->amg.degree;
+        else if (knob == TG_TUNE_AMG_WEIGHT)
+            *value = c->amg.weight;
         else
             fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
     });

@@ -72,6 +72,49 @@ struct Timing {
     tg_timing acc{};
 };
 
+// One level of the aggregation multigrid preconditioner.  Level 0 is the
+// solver's block operator (3 dofs per node); every coarser level has 6 dofs
+// per node (the rigid-body modes of an aggregate) and its matrix is a block
+// CSR of 6x6 FP32 blocks, row-major inside a block.
+struct AmgLevel {
+    int n = 0, bs = 3;                       // nodes, dofs per node
+    std::vector<int> rowptr, col;            // pattern (levels >= 1; sorted, diagonal included)
+    DBuf<int> d_rowptr, d_col, d_diagblk;
+    DBuf<float> d_val;                       // [nnz blocks][36]
+    DBuf<double> d_dinv_own;                 // [n*bs] inverse diagonal (levels >= 1)
+    double* dinv = nullptr;                  // level 0: the PCG's Jacobi vector
+    DBuf<double> d_f, d_x, d_r, d_d0, d_d1;  // V-cycle vectors [n*bs]
+    // transfer to level + 1
+    int nnext = 0;
+    DBuf<int> d_agg;                         // [n] aggregate (node of level + 1), -1 none
+    DBuf<int> d_mptr, d_mem;                 // members of each aggregate (CSR)
+    DBuf<float> d_P;                         // [n][bs][6] tentative prolongator rows
+    long nup = 0;                            // upper blocks (A <= B) of level + 1
+    DBuf<int> d_up;                          // [nup][2] block index of (A,B) and of (B,A) (-1 when A == B)
+    DBuf<long> d_gptr;                       // [nup+1]
+    DBuf<int> d_glist;                       // [entries][2] (block index in this level, block row)
+    double lmax = 0.0;
+};
+
+struct Amg {
+    int gen = 0;                             // bumped on every rebuild or knob change (captured PCG graphs key on it)
+    int degree = 2;                          // Chebyshev degree per smoothing pass
+    int weight = 150;                        // coarse-grid correction weight, percent
+    std::vector<AmgLevel*> lv;               // owned
+    int ncd = 0;                             // dofs of the coarsest level (dense solve)
+    DBuf<double> d_cinv;                     // [ncd][ncd] inverse of the coarsest matrix
+    DBuf<double> d_coef;                     // [levels][8] Chebyshev coefficients
+    DBuf<double> d_red;                      // reduction scratch
+    DBuf<double> d_part;                     // reduction partials
+    void clear()
+    {
+        for (auto* p : lv) delete p;
+        lv.clear();
+        ncd = 0;
+    }
+    ~Amg() { clear(); }
+};
+
 // device scalar slots of an iteration (per segment for the PCG)
 enum { S_RZ = 0, S_PQ, S_ALPHA, S_RZNEW, S_RR, S_BETA, S_BB, S_DONE, S_ITER, S_THRESH, S_MAXIT, S_DOT, S_NSCAL };
 
@@ -169,6 +212,7 @@ struct tg_ctx {
 
     tg::Timing timing;
     tg::Segments seg;
+    tg::Amg amg;                        // aggregation multigrid (precond == TG_PC_AMG)
 };
 
 namespace tg {
@@ -202,6 +246,11 @@ template <class MT>
 void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, double* x, double* r, double* p,
              double* q, double rtol, int maxit, const int* d_active, int nactive, tg_lin_stats* st);
 void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out);
+// aggregation multigrid (tg_amg.cpp: structure on the host; k_amg.hip: values and cycles)
+void amg_setup(tg_ctx* c);
+void amg_numeric(tg_ctx* c);
+template <class MT>
+void amg_vcycle(tg_ctx* c, const MT* val0, const double* f, double* z, const double* done);
 void eq_to_node(tg_ctx* c, const double* src_eq, double* dst_node);
 void node_to_eq(tg_ctx* c, const double* src_node, double* dst_eq);
 void launch_axpy(tg_ctx* c, double* y, const double* x, double a, long n);
@@ -215,6 +264,6 @@ struct TimedRegion {
     TimedRegion(tg_ctx* ctx, int w);
     ~TimedRegion();
 };
-enum { T_RESIDUAL = 0, T_STIFFNESS, T_EXPAND, T_SPMV, T_VECTOR };
+enum { T_RESIDUAL = 0, T_STIFFNESS, T_EXPAND, T_SPMV, T_VECTOR, T_PRECOND };
 
 }  // namespace tg

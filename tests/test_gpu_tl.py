@@ -251,3 +251,78 @@ def test_spmv_xcd_mapping_bitwise(precision):
     for o in out[1:]:
         assert o[1] == out[0][1]
         assert np.array_equal(o[0], out[0][0])
+
+
+@pytest.mark.parametrize("name,precision", [("tl_n8", 0), ("tl_n16", 0), ("rve_n4", 0), ("q1p0_n4", 0),
+                                            ("tl_n16", 1)])
+def test_amg_newton(name, precision):
+    """The aggregation-multigrid preconditioner changes only the inner
+    solve: the static solve reaches the reference's displacements (1e-9)
+    with the reference's Newton counts, in far fewer PCG iterations than
+    point Jacobi on the larger meshes."""
+    from tahoe_amd import api
+    g = load_golden(name)
+    its = {}
+    for pc in (api.PC_JACOBI, api.PC_AMG):
+        s = make_solid(g)
+        if precision:
+            s.set_precision(api.MIXED)
+            s.initialize()
+        s.set_preconditioner(pc)
+        lin = []
+        us, iters = api.static_solve(s, int(g["num_steps"]), s.opts(lin_rtol=1e-12),
+                                     on_step=lambda k, st: lin.append(st.lin_iterations))
+        print(f"# {name} precision {precision} pc {pc}: PCG iterations per step {lin}", flush=True)
+        for k in range(us.shape[0]):
+            ref = g["step_u"][k]
+            assert np.abs(us[k] - ref).max() <= TOL_U * np.abs(ref).max(), (name, pc, k)
+        assert list(iters) == g["step_iters"].tolist()
+        its[pc] = sum(lin)
+        s.close()
+    if g["X"].shape[0] > 500:
+        assert its[api.PC_AMG] * 3 < its[api.PC_JACOBI], its
+
+
+def test_amg_set_after_initialize_and_rejects_segments():
+    """The multigrid structure is built when the preconditioner is chosen
+    after tg_initialize; a batch of segments is refused."""
+    from tahoe_amd import api
+    g = load_golden("tl_n8")
+    s = make_solid(g)
+    s.set_preconditioner(api.PC_AMG)
+    s.apply_kbc(1.0 / int(g["num_steps"]))
+    st = s.newton_correction(s.opts(lin_rtol=1e-10))
+    assert st.lin_iterations > 0
+    s.close()
+    from tahoe_amd import batch, rve
+    pk = batch.pack([rve.Request(k, 3, api.TOTAL_LAGRANGIAN, 0.3, seed=5 + k).problem() for k in range(2)])
+    s = api.Solid(0)
+    s.set_mesh(pk.X, pk.conn, pk.mat)
+    for m, (E, nu) in enumerate(pk.materials):
+        s.set_material(m, *api.mu_kappa(E, nu))
+    s.set_kbc(pk.kbc.nodes, pk.kbc.dofs, pk.kbc.values)
+    s.set_segments(pk.node_begin)
+    with pytest.raises(api.TahoeGpuError):
+        s.set_preconditioner(api.PC_AMG)
+    s.close()
+
+
+@pytest.mark.parametrize("degree,weight", [(2, 100), (3, 150), (4, 190)])
+def test_amg_knobs(degree, weight):
+    """Chebyshev degree and coarse-correction weight change the iteration
+    count, never the converged displacements."""
+    from tahoe_amd import api
+    g = load_golden("tl_n8")
+    s = make_solid(g)
+    s.set_preconditioner(api.PC_AMG)
+    s.set_tuning(api.TUNE_AMG_DEGREE, degree)
+    s.set_tuning(api.TUNE_AMG_WEIGHT, weight)
+    assert s.tuning(api.TUNE_AMG_DEGREE) == degree and s.tuning(api.TUNE_AMG_WEIGHT) == weight
+    us, iters = api.static_solve(s, int(g["num_steps"]), s.opts(lin_rtol=1e-12))
+    for k in range(us.shape[0]):
+        ref = g["step_u"][k]
+        assert np.abs(us[k] - ref).max() <= TOL_U * np.abs(ref).max(), (degree, weight, k)
+    assert list(iters) == g["step_iters"].tolist()
+    with pytest.raises(api.TahoeGpuError):
+        s.set_tuning(api.TUNE_AMG_WEIGHT, 200)
+    s.close()
