@@ -94,8 +94,20 @@ const char* tg_last_error(const tg_ctx* ctx);
 int tg_set_mesh(tg_ctx* ctx, int nnd, int nel, const double* X, const int* conn, const int* mat);
 /* Simo_isotropic moduli for material id (IsotropicT.cpp:32-45 gives mu,kappa). */
 int tg_set_material(tg_ctx* ctx, int id, double mu, double kappa);
-/* TG_TOTAL_LAGRANGIAN or TG_Q1P0 */
+/* TG_TOTAL_LAGRANGIAN or TG_Q1P0.  The reference's updated_lagrangian
+ * element (UpdatedLagrangianT.cpp:83-92: shape functions of the current
+ * configuration, weights det(dx/dxi)) integrates the same forms as
+ * TotalLagrangianT (det(dx/dxi) = J det(dX/dxi)) and maps to
+ * TG_TOTAL_LAGRANGIAN. */
 int tg_set_formulation(tg_ctx* ctx, int formulation);
+/* Material model of every material of the context, whose moduli come from
+ * tg_set_material: TG_MAT_SIMO_ISOTROPIC (default, Simo_isotropic,
+ * SimoIso3D.cpp:16-52) or TG_MAT_STVENANT (large_strain_StVenant: FDKStV
+ * with FDHookeanMatT::s_ij/c_ijkl, FDHookeanMatT.cpp, and
+ * IsotropicT::ComputeModuli, IsotropicT.cpp:153-169; total-Lagrangian element
+ * only). */
+enum { TG_MAT_SIMO_ISOTROPIC = 0, TG_MAT_STVENANT = 1 };
+int tg_set_material_model(tg_ctx* ctx, int model);
 /* TG_FP64 (default) or TG_MIXED; no counterpart in the reference, which is
  * FP64 only.  TG_MIXED applies to the total-Lagrangian element. */
 int tg_set_precision(tg_ctx* ctx, int precision);
@@ -112,6 +124,12 @@ int tg_set_preconditioner(tg_ctx* ctx, int pc);
 /* Kinematic BC cards: prescribed value = values[k] * t (KBC_CardT::Value,
  * tahoe/src/primitives/KBC_CardT.cpp:53-64, linear schedule). */
 int tg_set_kbc(tg_ctx* ctx, int nkbc, const int* nodes, const int* dofs, const double* values);
+/* Nodal force cards (force_BC, FBC_CardT): the residual gains
+ * values[k] * scale at (nodes[k], dofs[k]) for active dofs (FieldT::FormRHS,
+ * FieldT.cpp:390-401, with FBC_CardT::CurrentValue, FBC_CardT.cpp:57-63:
+ * scale = the card's schedule value, set by tg_apply_forces; 0 until then). */
+int tg_set_forces(tg_ctx* ctx, int nforce, const int* nodes, const int* dofs, const double* values);
+int tg_apply_forces(tg_ctx* ctx, double scale);
 
 /* Batched problems: the mesh of tg_set_mesh is the disjoint union of nseg
  * independent sub-meshes; sub-mesh g owns nodes [node_begin[g],

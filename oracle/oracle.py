@@ -18,7 +18,7 @@ _lib = None
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
 
-TL, Q1P0 = 0, 1
+TL, Q1P0, TL_STVENANT = 0, 1, 2
 
 
 class NewtonOpts(C.Structure):
@@ -37,6 +37,11 @@ def lib():
         L.or_simo_moduli.argtypes = [C.c_double, C.c_double, D, D]
         L.or_simo_energy.argtypes = [C.c_double, C.c_double, D]
         L.or_simo_energy.restype = C.c_double
+        L.or_stvenant_cauchy.argtypes = [C.c_double, C.c_double, D, D]
+        L.or_stvenant_moduli.argtypes = [C.c_double, C.c_double, D, D]
+        L.or_stvenant_energy.argtypes = [C.c_double, C.c_double, D]
+        L.or_stvenant_energy.restype = C.c_double
+        L.or_tl_stvenant_element.argtypes = [D, D, C.c_double, C.c_double, D, D]
         L.or_hex8_parent.argtypes = [D, D, D]
         L.or_tl_element.argtypes = [D, D, C.c_double, C.c_double, D, D]
         L.or_q1p0_element.argtypes = [D, D, C.c_double, C.c_double, D, D]
@@ -99,6 +104,30 @@ def simo_energy(mu, kappa, F):
     return lib().or_simo_energy(mu, kappa, _d(Fc))
 
 
+def stvenant_cauchy(mu, kappa, F):
+    """large_strain_StVenant Cauchy stress, Voigt (11,22,33,23,13,12)."""
+    Fc = f64(np.asarray(F).T.ravel())
+    s = np.zeros(6)
+    rc = lib().or_stvenant_cauchy(mu, kappa, _d(Fc), _d(s))
+    if rc:
+        raise ArithmeticError(rc)
+    return s
+
+
+def stvenant_moduli(mu, kappa, F):
+    Fc = f64(np.asarray(F).T.ravel())
+    c = np.zeros(36)
+    rc = lib().or_stvenant_moduli(mu, kappa, _d(Fc), _d(c))
+    if rc:
+        raise ArithmeticError(rc)
+    return c.reshape(6, 6).T.copy()
+
+
+def stvenant_energy(mu, kappa, F):
+    Fc = f64(np.asarray(F).T.ravel())
+    return lib().or_stvenant_energy(mu, kappa, _d(Fc))
+
+
 def hex8_parent():
     Na = np.zeros((8, 8)); DNa = np.zeros((8, 3, 8)); w = np.zeros(8)
     lib().or_hex8_parent(_d(Na), _d(DNa), _d(w))
@@ -109,7 +138,7 @@ def element(kind, Xe, ue, mu, kappa, want_R=True, want_K=True):
     Xe = f64(Xe).ravel(); ue = f64(ue).ravel()
     R = np.zeros(24) if want_R else None
     K = np.zeros((24, 24)) if want_K else None
-    fn = lib().or_q1p0_element if kind == Q1P0 else lib().or_tl_element
+    fn = {Q1P0: lib().or_q1p0_element, TL_STVENANT: lib().or_tl_stvenant_element}.get(kind, lib().or_tl_element)
     rc = fn(_d(Xe), _d(ue), mu, kappa, _d(R) if want_R else None, _d(K) if want_K else None)
     if rc:
         raise ArithmeticError(rc)

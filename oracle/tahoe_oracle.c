@@ -141,6 +141,84 @@ double or_simo_energy(double mu, double kappa, const double F[9])
     return 0.5 * kappa * (0.5 * (J * J - 1.0) - log(J)) + 0.5 * mu * (bb[0] + bb[1] + bb[2] - 3.0);
 }
 
+/* large_strain_StVenant (FDKStV, FDKStV.cpp): FDHookeanMatT::s_ij and c_ijkl
+ * (FDHookeanMatT.cpp: Compute_E, HookeanStress, PushForward, / det F) with
+ * the isotropic moduli of IsotropicT::ComputeModuli (IsotropicT.cpp:153-169),
+ * lambda = kappa - 2 mu / 3 (IsotropicT::Set_E_nu gives mu and kappa).
+ * Restated in index form: E_IJ = (F_kI F_kJ - delta_IJ)/2, S_IJ = C_IJKL E_KL,
+ * sigma_ij = F_iI S_IJ F_jJ / J, c_ijkl = F_iI F_jJ F_kK F_lL C_IJKL / J. */
+static double iso_C(double lam, double mu, int I, int J, int K, int L)
+{
+    return lam * (I == J) * (K == L) + mu * ((I == K) * (J == L) + (I == L) * (J == K));
+}
+
+int or_stvenant_cauchy(double mu, double kappa, const double F[9], double sig[6])
+{
+    const double lam = kappa - 2.0 * mu / 3.0;
+    const double J = det3(F);
+    if (!(J > 0.0)) return 7;
+    double E[9], S[9];
+    for (int I = 0; I < 3; I++)
+        for (int Jx = 0; Jx < 3; Jx++) {
+            double c = 0.0;
+            for (int k = 0; k < 3; k++) c += M3(F, k, I) * M3(F, k, Jx);
+            M3(E, I, Jx) = 0.5 * (c - (I == Jx ? 1.0 : 0.0));
+        }
+    for (int I = 0; I < 3; I++)
+        for (int Jx = 0; Jx < 3; Jx++) {
+            double v = 0.0;
+            for (int K = 0; K < 3; K++)
+                for (int L = 0; L < 3; L++) v += iso_C(lam, mu, I, Jx, K, L) * M3(E, K, L);
+            M3(S, I, Jx) = v;
+        }
+    for (int v = 0; v < 6; v++) {
+        const int i = VI[v], j = VJ[v];
+        double x = 0.0;
+        for (int I = 0; I < 3; I++)
+            for (int Jx = 0; Jx < 3; Jx++) x += M3(F, i, I) * M3(S, I, Jx) * M3(F, j, Jx);
+        sig[v] = x / J;
+    }
+    return 0;
+}
+
+int or_stvenant_moduli(double mu, double kappa, const double F[9], double c[36])
+{
+    const double lam = kappa - 2.0 * mu / 3.0;
+    const double J = det3(F);
+    if (!(J > 0.0)) return 7;
+    for (int v = 0; v < 6; v++)
+        for (int w = 0; w < 6; w++) {
+            const int i = VI[v], j = VJ[v], k = VI[w], l = VJ[w];
+            double x = 0.0;
+            for (int I = 0; I < 3; I++)
+                for (int Jx = 0; Jx < 3; Jx++)
+                    for (int K = 0; K < 3; K++)
+                        for (int L = 0; L < 3; L++)
+                            x += M3(F, i, I) * M3(F, j, Jx) * M3(F, k, K) * M3(F, l, L) * iso_C(lam, mu, I, Jx, K, L);
+            c[v + 6 * w] = x / J;
+        }
+    return 0;
+}
+
+/* W = lambda/2 tr(E)^2 + mu E:E (HookeanMatT::HookeanEnergy with the
+ * isotropic moduli) */
+double or_stvenant_energy(double mu, double kappa, const double F[9])
+{
+    const double lam = kappa - 2.0 * mu / 3.0;
+    double E[9], tr = 0.0, ee = 0.0;
+    for (int I = 0; I < 3; I++)
+        for (int Jx = 0; Jx < 3; Jx++) {
+            double c = 0.0;
+            for (int k = 0; k < 3; k++) c += M3(F, k, I) * M3(F, k, Jx);
+            M3(E, I, Jx) = 0.5 * (c - (I == Jx ? 1.0 : 0.0));
+        }
+    for (int I = 0; I < 3; I++) tr += M3(E, I, I);
+    for (int q = 0; q < 9; q++) ee += E[q] * E[q];
+    return 0.5 * lam * tr * tr + mu * ee;
+}
+
+typedef int (*stress_fn)(double, double, const double*, double*);
+
 /* ------------------------------------------------------------------------ */
 /* A3: 8-node hexahedron, 2x2x2 Gauss (toolbox/src/geometry/HexahedronT.cpp) */
 /* ------------------------------------------------------------------------ */
@@ -238,8 +316,8 @@ static void set_B(const double dN[3][8], double B[6][24])
  * SolidElementT.cpp:1240), i.e. minus the internal force; Ke = FormStiffness(1)
  * (TotalLagrangianT.cpp:40-104) with both triangles filled.  Either output may
  * be NULL. */
-int or_tl_element(const double Xe[24], const double ue[24], double mu, double kappa,
-                  double Re[24], double Ke[576])
+static int tl_element(stress_fn cauchy, stress_fn moduli, const double Xe[24], const double ue[24], double mu,
+                      double kappa, double Re[24], double Ke[576])
 {
     const parent_t* P = parent();
     double det0[8], dNX[8][3][8];
@@ -251,7 +329,7 @@ int or_tl_element(const double Xe[24], const double ue[24], double mu, double ka
     for (int q = 0; q < 8; q++) {
         double F[9], Fi[9], sig[6];
         def_grad(ue, dNX[q], F);
-        if (or_simo_cauchy(mu, kappa, F, sig)) return 7;
+        if (cauchy(mu, kappa, F, sig)) return 7;
         double J = det3(F);
         if (J <= 0.0) return 7; /* TotalLagrangianT.cpp:127-130 */
         inv3(F, Fi);
@@ -289,7 +367,7 @@ int or_tl_element(const double Xe[24], const double ue[24], double mu, double ka
                     G[a][b] += s;
                 }
             double c[36], B[6][24], DB[6][24];
-            if (or_simo_moduli(mu, kappa, F, c)) return 7;
+            if (moduli(mu, kappa, F, c)) return 7;
             set_B(dnx, B);
             for (int i = 0; i < 6; i++)
                 for (int n = 0; n < 24; n++) {
@@ -313,6 +391,19 @@ int or_tl_element(const double Xe[24], const double ue[24], double mu, double ka
         memcpy(Ke, K, sizeof K);
     }
     return 0;
+}
+
+int or_tl_element(const double Xe[24], const double ue[24], double mu, double kappa,
+                  double Re[24], double Ke[576])
+{
+    return tl_element(or_simo_cauchy, or_simo_moduli, Xe, ue, mu, kappa, Re, Ke);
+}
+
+/* TotalLagrangianT with large_strain_StVenant */
+int or_tl_stvenant_element(const double Xe[24], const double ue[24], double mu, double kappa,
+                           double Re[24], double Ke[576])
+{
+    return tl_element(or_stvenant_cauchy, or_stvenant_moduli, Xe, ue, mu, kappa, Re, Ke);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -527,6 +618,11 @@ long or_msr_pattern(int n_eq, int nnd, int nel, const int* conn, const int* eqno
 
 typedef int (*element_fn)(const double*, const double*, double, double, double*, double*);
 
+static element_fn element_for(int type)
+{
+    return type == OR_Q1P0 ? or_q1p0_element : (type == OR_TL_STVENANT ? or_tl_stvenant_element : or_tl_element);
+}
+
 static void gather(const double* A, const int* conn, int e, double* ae)
 {
     for (int a = 0; a < 8; a++)
@@ -540,7 +636,7 @@ int or_residual(int type, int nnd, int nel, const double* X, const int* conn, co
                 int n_eq, double* R)
 {
     (void)nnd;
-    element_fn fn = (type == OR_Q1P0) ? or_q1p0_element : or_tl_element;
+    element_fn fn = element_for(type);
     for (int k = 0; k < n_eq; k++) R[k] = 0.0;
     for (int e = 0; e < nel; e++) {
         double Xe[24], ue[24], Re[24];
@@ -602,7 +698,7 @@ int or_stiffness(int type, int nnd, int nel, const double* X, const int* conn, c
                  int n_eq, const int* bindx, double* val)
 {
     (void)nnd;
-    element_fn fn = (type == OR_Q1P0) ? or_q1p0_element : or_tl_element;
+    element_fn fn = element_for(type);
     long len = bindx[n_eq];
     for (long k = 0; k < len; k++) val[k] = 0.0;
     double* Ke = (double*)malloc(576 * sizeof(double));

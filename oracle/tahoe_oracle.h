@@ -25,13 +25,17 @@
 extern "C" {
 #endif
 
-enum { OR_TL = 0, OR_Q1P0 = 1 };
+enum { OR_TL = 0, OR_Q1P0 = 1, OR_TL_STVENANT = 2 };
 
 /* ---- material (SimoIso3D) ---- */
 void or_mu_kappa(double E, double nu, double* mu, double* kappa);
 int or_simo_cauchy(double mu, double kappa, const double F[9], double sig[6]);
 int or_simo_moduli(double mu, double kappa, const double F[9], double c[36]);
 double or_simo_energy(double mu, double kappa, const double F[9]);
+/* ---- material (large_strain_StVenant) ---- */
+int or_stvenant_cauchy(double mu, double kappa, const double F[9], double sig[6]);
+int or_stvenant_moduli(double mu, double kappa, const double F[9], double c[36]);
+double or_stvenant_energy(double mu, double kappa, const double F[9]);
 
 /* ---- parent domain ---- */
 void or_hex8_parent(double Na[8][8], double DNa[8][3][8], double w[8]);
@@ -41,6 +45,8 @@ int or_tl_element(const double Xe[24], const double ue[24], double mu, double ka
                   double Re[24], double Ke[576]);
 int or_q1p0_element(const double Xe[24], const double ue[24], double mu, double kappa,
                     double Re[24], double Ke[576]);
+int or_tl_stvenant_element(const double Xe[24], const double ue[24], double mu, double kappa,
+                           double Re[24], double Ke[576]);
 
 /* ---- equation system ---- */
 int or_equation_numbers(int nnd, int nkbc, const int* kbc_nodes, const int* kbc_dofs, int* eqnos);

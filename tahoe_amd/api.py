@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtahoe_gpu.so")
 
 TOTAL_LAGRANGIAN, Q1P0 = 0, 1
+MAT_SIMO_ISOTROPIC, MAT_STVENANT = 0, 1
 FP64, MIXED = 0, 1
 TUNE_STIFF_CHUNK, TUNE_STIFF_SHAPE, TUNE_NEWTON_ASSEMBLY, TUNE_PCG_GRAPH, TUNE_SPMV_XCD = 1, 2, 3, 4, 5
 TUNE_AMG_DEGREE, TUNE_AMG_WEIGHT = 6, 7
@@ -30,7 +31,8 @@ I = C.POINTER(C.c_int)
 # exported symbols, in declaration order of include/tahoe_gpu.h
 SYMBOLS = [
     "tg_create", "tg_destroy", "tg_last_error", "tg_set_mesh", "tg_set_material", "tg_set_formulation",
-    "tg_set_precision", "tg_set_preconditioner", "tg_set_kbc", "tg_set_segments", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
+    "tg_set_material_model", "tg_set_precision", "tg_set_preconditioner", "tg_set_kbc", "tg_set_forces",
+    "tg_apply_forces", "tg_set_segments", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
     "tg_get_displacement", "tg_apply_kbc", "tg_close_step", "tg_reset_step", "tg_form_rhs", "tg_form_lhs",
     "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_batch", "tg_newton_correction",
     "tg_synchronize", "tg_timing_enable", "tg_timing_reset", "tg_timing_get", "tg_calib_copy", "tg_set_tuning",
@@ -95,6 +97,9 @@ def lib():
         L.tg_set_mesh.argtypes = [vp, C.c_int, C.c_int, D, I, I]
         L.tg_set_material.argtypes = [vp, C.c_int, C.c_double, C.c_double]
         L.tg_set_formulation.argtypes = [vp, C.c_int]
+        L.tg_set_material_model.argtypes = [vp, C.c_int]
+        L.tg_set_forces.argtypes = [vp, C.c_int, I, I, D]
+        L.tg_apply_forces.argtypes = [vp, C.c_double]
         L.tg_set_precision.argtypes = [vp, C.c_int]
         L.tg_set_preconditioner.argtypes = [vp, C.c_int]
         L.tg_set_kbc.argtypes = [vp, C.c_int, I, I, D]
@@ -184,6 +189,18 @@ class Solid:
 
     def set_formulation(self, f):
         self._ok(lib().tg_set_formulation(self._h, f))
+
+    def set_material_model(self, model):
+        self._ok(lib().tg_set_material_model(self._h, int(model)))
+
+    def set_forces(self, nodes, dofs, values):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        self._ok(lib().tg_set_forces(self._h, len(nodes), _i(nodes), _i(dofs), _d(values)))
+
+    def apply_forces(self, scale):
+        self._ok(lib().tg_apply_forces(self._h, float(scale)))
 
     def set_precision(self, p):
         self._ok(lib().tg_set_precision(self._h, p))

@@ -550,13 +550,14 @@ static void amg_numeric_t(tg_ctx* c, const MT* val0)
     TG_HIP(hipStreamSynchronize(c->stream));  // host vectors outlive their copies
 }
 
+// The multigrid works on the FP32 copy of the fine operator in either
+// precision (the solver's own operator when it is FP32; refreshed from the
+// FP64 one by the caller otherwise): a preconditioner needs no more, and the
+// smoother's four fine-level products per cycle move half the bytes.
 void amg_numeric(tg_ctx* c)
 {
     if (c->amg.lv.empty()) amg_setup(c);
-    if (c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN)
-        amg_numeric_t<float>(c, c->op.d_val32.p);
-    else
-        amg_numeric_t<double>(c, c->op.d_val.p);
+    amg_numeric_t<float>(c, c->op.d_val32.p);
 }
 
 template <class Op>
@@ -591,9 +592,10 @@ static void smooth_up(tg_ctx* c, const Op& op, const double* dinv, const double*
 
 // z = M^-1 f by one V-cycle; every kernel is a no-op once *done != 0 (the
 // solve has converged), so a captured chunk of iterations stays valid.
-template <class MT>
-void amg_vcycle(tg_ctx* c, const MT* val0, const double* f0, double* z, const double* done)
+void amg_vcycle(tg_ctx* c, const double* f0, double* z, const double* done)
 {
+    using MT = float;
+    const MT* val0 = c->op.d_val32.p;
     Amg& G = c->amg;
     const int L = (int)G.lv.size();
     for (int l = 0; l + 1 < L; l++) {
@@ -629,7 +631,5 @@ void amg_vcycle(tg_ctx* c, const MT* val0, const double* f0, double* z, const do
     TG_HIP(hipGetLastError());
 }
 
-template void amg_vcycle<double>(tg_ctx*, const double*, const double*, double*, const double*);
-template void amg_vcycle<float>(tg_ctx*, const float*, const double*, double*, const double*);
 
 }  // namespace tg

@@ -48,6 +48,7 @@ __device__ __forceinline__ void reduce_scatter8(const double r[24], int q, doubl
 // (TotalLagrangianT::FormKd, TotalLagrangianT.cpp:107-143, called with
 // constK = -1 from SolidElementT::ElementRHSDriver, SolidElementT.cpp:1240):
 //   R_a,i += -J w det0 sum_k (sigma F^-T)_ik dN_a/dX_k
+template <int MAT>
 __global__ void __launch_bounds__(256) k_residual_tl(const int* __restrict__ elems, int count,
                                                      const int* __restrict__ conn, const int* __restrict__ emat,
                                                      const double2* __restrict__ props,
@@ -87,8 +88,7 @@ __global__ void __launch_bounds__(256) k_residual_tl(const int* __restrict__ ele
         const double J = det3(F);
         bad |= !(J > 0.0);
         double Jb, bb[6], sig[6];
-        bad |= !simo_kinematics(F, Jb, bb);
-        simo_cauchy(mk.x, mk.y, Jb, bb, sig);
+        bad |= !material_stress<MAT>(mk.x, mk.y, F, J, Jb, bb, sig);
         inv3(F, J, Fi);
         // S = sigma F^-T ; S(i,k) = sum_m sigma(i,m) Fi(k,m)
         const double sm[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
@@ -121,10 +121,31 @@ __global__ void __launch_bounds__(256) k_residual_tl(const int* __restrict__ ele
 
 void launch_q1p0(tg_ctx* c, const double* u, double* R_node, bool stiff);
 
+// R += applied nodal forces at active dofs (FieldT::FormRHS, FieldT.cpp:390-401)
+__global__ void __launch_bounds__(256) k_add_forces(long n3, const int* __restrict__ eqn,
+                                                    const double* __restrict__ f, double* __restrict__ R)
+{
+    const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    if (k < n3 && eqn[k] >= 0) R[k] += f[k];
+}
+
+static void element_residual(tg_ctx* c, const double* u, double* R_node);
+
 void launch_residual(tg_ctx* c, const double* u, double* R_node)
 {
     TimedRegion tr(c, T_RESIDUAL);
     TG_HIP(hipMemsetAsync(R_node, 0, sizeof(double) * 3 * (size_t)c->nnd, c->stream));
+    element_residual(c, u, R_node);
+    if (c->dfext.p) {
+        const long n3 = 3L * c->nnd;
+        hipLaunchKernelGGL(k_add_forces, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, c->stream, n3, c->deqn.p,
+                           c->dfext.p, R_node);
+        TG_HIP(hipGetLastError());
+    }
+}
+
+static void element_residual(tg_ctx* c, const double* u, double* R_node)
+{
     if (c->formulation == TG_Q1P0) {
         launch_q1p0(c, u, R_node, false);
         return;
@@ -133,10 +154,14 @@ void launch_residual(tg_ctx* c, const double* u, double* R_node)
         const int first = c->color_ptr[col], count = c->color_ptr[col + 1] - first;
         if (count == 0) continue;
         const int blocks = (count + 31) / 32;
-        if (c->formulation == TG_TOTAL_LAGRANGIAN)
-            hipLaunchKernelGGL(k_residual_tl, dim3(blocks), dim3(256), 0, c->stream, c->dcolor_elems.p + first,
-                               count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->deqn.p, R_node,
-                               c->dstatus.p);
+        if (c->material_model == TG_MAT_STVENANT)
+            hipLaunchKernelGGL(k_residual_tl<MAT_STVENANT>, dim3(blocks), dim3(256), 0, c->stream,
+                               c->dcolor_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                               c->deqn.p, R_node, c->dstatus.p);
+        else
+            hipLaunchKernelGGL(k_residual_tl<MAT_SIMO>, dim3(blocks), dim3(256), 0, c->stream,
+                               c->dcolor_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                               c->deqn.p, R_node, c->dstatus.p);
     }
     TG_HIP(hipGetLastError());
 }
@@ -191,7 +216,7 @@ __device__ __forceinline__ constexpr int sym6(int v, int w)
 // s sigma with s = det0 * J * w (w = 1), all in FP64; xu(a, c) reads node a's
 // X (c < 3) or u (c >= 3).  Returns false on J <= 0 or det0 <= 0 (the data
 // are then zero).  An invalid slot (no element) gives zeros.
-template <class XU>
+template <int MAT, class XU>
 __device__ __forceinline__ bool stiffness_ip(int q, bool valid, double2 mk, XU xu, double (&g)[3][8],
                                              double (&D)[21], double (&T)[6])
 {
@@ -208,8 +233,7 @@ __device__ __forceinline__ bool stiffness_ip(int q, bool valid, double2 mk, XU x
         deformation_gradient([&](int a, int c) { return xu(a, 3 + c); }, g0, F);
         const double J = det3(F);
         bad |= !(J > 0.0);
-        bad |= !simo_kinematics(F, Jb, bb);
-        simo_cauchy(mk.x, mk.y, Jb, bb, sig);
+        bad |= !material_stress<MAT>(mk.x, mk.y, F, J, Jb, bb, sig);
         inv3(F, J, Fi);
         scale = bad ? 0.0 : det0 * J;  // weight 1
         // dN/dx_j = sum_k Fi(k,j) dN/dX_k
@@ -225,7 +249,7 @@ __device__ __forceinline__ bool stiffness_ip(int q, bool valid, double2 mk, XU x
             for (int j = 0; j < 3; j++) g[j][a] = 0.0;
     }
     double cm[6][6];
-    simo_moduli(mk.x, mk.y, Jb, bb, cm);
+    material_moduli<MAT>(mk.x, mk.y, Jb, bb, cm);
 #pragma unroll
     for (int v = 0; v < 6; v++)
 #pragma unroll
@@ -322,7 +346,7 @@ __device__ __forceinline__ void upper_block(int p, int& a, int& b)
 // before the block products, so the read half of the read-modify-write is in
 // flight during the arithmetic (no other element of the launch touches those
 // entries); 2 memory-side atomic adds with no return value.
-template <class VT, int EPB, bool TO_OP, bool PM, int SC>
+template <class VT, int EPB, bool TO_OP, bool PM, int SC, int MAT = MAT_SIMO>
 __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict__ elems, int count,
                                                              const int* __restrict__ conn,
                                                              const int* __restrict__ emat,
@@ -361,7 +385,7 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
         const bool valid = base + le < count;
         const double2 mk = valid ? props[emat[elems[base + le]]] : make_double2(0.0, 0.0);
         double g[3][8], D[21], T[6];
-        bad |= !stiffness_ip(q, valid, mk, [&](int a, int c) { return sxu[le][a][c]; }, g, D, T);
+        bad |= !stiffness_ip<MAT>(q, valid, mk, [&](int a, int c) { return sxu[le][a][c]; }, g, D, T);
 #pragma unroll
         for (int a = 0; a < 8; a++)
 #pragma unroll
@@ -508,7 +532,7 @@ static const std::vector<std::pair<int, int>>& stiffness_launches(tg_ctx* c)
 // workgroup, phase 1 on 8 lanes per element, lane (le, p) owns upper block
 // p.  Voigt rows of B for node b (11,22,33,23,13,12): u1 -> (g1,0,0,0,g3,g2),
 // u2 -> (0,g2,0,g3,0,g1), u3 -> (0,0,g3,g2,g1,0) (Hughes 2.8.21).
-template <class VT, int EPB, bool TO_OP>
+template <class VT, int EPB, bool TO_OP, int MAT = MAT_SIMO>
 __global__ void __launch_bounds__(36 * EPB) k_stiffness_db(const int* __restrict__ elems, int count,
                                                              const int* __restrict__ conn,
                                                              const int* __restrict__ emat,
@@ -545,7 +569,7 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_db(const int* __restrict
         const bool valid = base + le < count;
         const double2 mk = valid ? props[emat[elems[base + le]]] : make_double2(0.0, 0.0);
         double g[3][8], D[21], T[6];
-        bad |= !stiffness_ip(q, valid, mk, [&](int a, int c) { return sxu[le][a][c]; }, g, D, T);
+        bad |= !stiffness_ip<MAT>(q, valid, mk, [&](int a, int c) { return sxu[le][a][c]; }, g, D, T);
 #pragma unroll
         for (int j = 0; j < 3; j++)
 #pragma unroll
@@ -605,23 +629,29 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_db(const int* __restrict
     if (bad) atomicOr(status, 7);
 }
 
-template <class VT, int EPB, bool TO_OP, bool PM, bool DB, int SC>
+template <class VT, int EPB, bool TO_OP, bool PM, bool DB, int SC, int MAT = MAT_SIMO>
 static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u, VT* val)
 {
     const int blocks = (count + EPB - 1) / EPB;
     if (DB)
-        hipLaunchKernelGGL((k_stiffness_db<VT, EPB, TO_OP>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
+        hipLaunchKernelGGL((k_stiffness_db<VT, EPB, TO_OP, MAT>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
                            c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
                            TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
     else
-        hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM, SC>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
-                           c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
-                           TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
+        hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM, SC, MAT>), dim3(blocks), dim3(36 * EPB), 0,
+                           c->stream, c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p,
+                           c->dX.p, u, TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
 }
 
 template <class VT, bool TO_OP>
 static void launch_stiffness_shape(tg_ctx* c, int v, int first, int count, const double* u, VT* val)
 {
+    if (c->material_model == TG_MAT_STVENANT) {
+        // one shape (16 elements per workgroup, load-add-store scatter) for
+        // the St.Venant material
+        launch_stiffness_v<VT, 16, TO_OP, false, false, 0, MAT_STVENANT>(c, first, count, u, val);
+        return;
+    }
 #define TG_SHAPE(i) \
     launch_stiffness_v<VT, STIFF_EPB[i], TO_OP, STIFF_PM[i], STIFF_DB[i], STIFF_SC[i]>(c, first, count, u, val)
     switch (v) {

@@ -376,7 +376,7 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
     const double* done = S.d_scal.p + S_DONE * nseg;
     if (PC == 2) {
         TimedRegion tr(c, T_PRECOND);
-        amg_vcycle<MT>(c, val, b, (double*)dinv, nullptr);
+        amg_vcycle(c, b, (double*)dinv, nullptr);
     }
     {
         TimedRegion tr(c, T_VECTOR);
@@ -410,7 +410,7 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
             }
             if (PC == 2) {
                 TimedRegion tr(c, T_PRECOND);
-                amg_vcycle<MT>(c, val, r, (double*)dinv, done);
+                amg_vcycle(c, r, (double*)dinv, done);
                 hipLaunchKernelGGL(k_seg_dot, dim3(nb), dim3(PB), 0, c->stream, c->nnd, r, dinv, S.d_part.p, ns);
             }
             {

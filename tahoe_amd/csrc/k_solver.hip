@@ -353,6 +353,16 @@ void widen_values(tg_ctx* c)
     TG_HIP(hipGetLastError());
 }
 
+// FP32 copy of an FP64 operator (the multigrid smoother's operator in FP64 runs)
+void launch_narrow_operator(tg_ctx* c)
+{
+    TimedRegion tr(c, T_PRECOND);
+    const long n = c->op.nblocks_padded * 9;
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_val.p,
+                       c->op.d_val32.p);
+    TG_HIP(hipGetLastError());
+}
+
 void launch_spmv(tg_ctx* c, const double* x, double* y)
 {
     TimedRegion tr(c, T_SPMV);

@@ -289,6 +289,15 @@ int tg_set_formulation(tg_ctx* c, int f)
     });
 }
 
+int tg_set_material_model(tg_ctx* c, int model)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (model != TG_MAT_SIMO_ISOTROPIC && model != TG_MAT_STVENANT) fail(TG_BAD_INPUT, "unknown material model");
+        c->material_model = model;
+    });
+}
+
 int tg_set_precision(tg_ctx* c, int prec)
 {
     if (!c) return TG_BAD_INPUT;
@@ -325,6 +334,7 @@ int tg_set_preconditioner(tg_ctx* c, int pc)
         c->precond = pc;
         if (c->initialized) {
             c->vbinv.alloc(pc == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
+            if (pc == TG_PC_AMG && !c->op.d_val32.p) c->op.d_val32.alloc((size_t)c->op.nblocks_padded * 9);
             if (pc == TG_PC_AMG && c->amg.lv.empty()) amg_setup(c);
         }
     });
@@ -356,7 +366,12 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         if ((int)c->mu.size() < nmat) fail(TG_BAD_INPUT, "element refers to an undefined material");
         for (int m = 0; m < nmat; m++)
             if (c->mu[m] < 0.0) fail(TG_BAD_INPUT, "element refers to an undefined material");
+        if (c->material_model == TG_MAT_STVENANT && c->formulation != TG_TOTAL_LAGRANGIAN)
+            fail(TG_BAD_INPUT, "the St.Venant material is on the total-Lagrangian element only");
+        for (int k : c->fbc_idx)
+            if (k >= 3 * c->nnd) fail(TG_OUT_OF_RANGE, "force card out of range");
         c->initialized = false;
+        c->dfext.release();
         // TG_SETUP_TRACE=1 prints the host-side setup phases to stderr
         static const bool trace = std::getenv("TG_SETUP_TRACE") != nullptr;
         auto t_last = std::chrono::steady_clock::now();
@@ -439,7 +454,7 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
             c->op.d_oslot.release();
         }
         c->vbinv.alloc(c->precond == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
-        c->op.d_val32.alloc(mixed ? (size_t)c->op.nblocks_padded * 9 : 0);
+        c->op.d_val32.alloc(mixed || c->precond == TG_PC_AMG ? (size_t)c->op.nblocks_padded * 9 : 0);
         phase("operator slot map");
         if (c->precond == TG_PC_AMG) {
             amg_setup(c);
@@ -491,6 +506,38 @@ int tg_get_displacement(tg_ctx* c, double* u)
         require_init(c);
         TG_HIP(hipMemcpyAsync(u, c->du.p, sizeof(double) * 3 * (size_t)c->nnd, hipMemcpyDeviceToHost, c->stream));
         TG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int tg_set_forces(tg_ctx* c, int n, const int* nodes, const int* dofs, const double* values)
+{
+    if (!c || n < 0 || (n > 0 && (!nodes || !dofs || !values))) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        c->fbc_idx.resize((size_t)n);
+        c->fbc_vals.assign(values, values + n);
+        for (int k = 0; k < n; k++) {
+            if (nodes[k] < 0 || nodes[k] >= c->nnd || dofs[k] < 0 || dofs[k] > 2)
+                fail(TG_OUT_OF_RANGE, "force card out of range");
+            c->fbc_idx[k] = 3 * nodes[k] + dofs[k];
+        }
+        c->dfext.release();
+    });
+}
+
+int tg_apply_forces(tg_ctx* c, double scale)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        if (c->fbc_idx.empty()) {
+            c->dfext.release();
+            return;
+        }
+        std::vector<double> f(3 * (size_t)c->nnd, 0.0);
+        for (size_t k = 0; k < c->fbc_idx.size(); k++) f[c->fbc_idx[k]] += c->fbc_vals[k] * scale;
+        if (!c->dfext.p) c->dfext.alloc(f.size());
+        TG_HIP(hipMemcpyAsync(c->dfext.p, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));  // f outlives its copy
     });
 }
 
@@ -566,7 +613,10 @@ static void fp64_values(tg_ctx* c)
 static void prepare_preconditioner(tg_ctx* c)
 {
     launch_block_inverse(c);
-    if (c->precond == TG_PC_AMG) amg_numeric(c);
+    if (c->precond == TG_PC_AMG) {
+        if (!mixed_values(c)) launch_narrow_operator(c);
+        amg_numeric(c);
+    }
 }
 
 // operator of the inner linear solve, in the solve's precision

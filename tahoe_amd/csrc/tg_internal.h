@@ -150,7 +150,11 @@ struct tg_ctx {
     std::vector<int> conn, mat;
     std::vector<double> mu, kappa;
     int formulation = TG_TOTAL_LAGRANGIAN;
+    int material_model = TG_MAT_SIMO_ISOTROPIC;
     int precision = TG_FP64;
+    std::vector<int> fbc_idx;           // force cards: 3*node+dof
+    std::vector<double> fbc_vals;
+    tg::DBuf<double> dfext;             // [nnd*3] applied nodal forces (allocated once a card exists)
     std::vector<int> kbc_nodes, kbc_dofs;
     std::vector<double> kbc_vals;
 
@@ -238,6 +242,7 @@ void launch_build_slots(tg_ctx* c);
 void launch_expand(tg_ctx* c);
 void launch_spmv(tg_ctx* c, const double* x, double* y);
 void launch_expand32(tg_ctx* c);
+void launch_narrow_operator(tg_ctx* c);
 void widen_values(tg_ctx* c);
 void pcg_solve_mixed(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
 double device_dot(tg_ctx* c, const double* a, const double* b, long n);
@@ -249,8 +254,7 @@ void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out)
 // aggregation multigrid (tg_amg.cpp: structure on the host; k_amg.hip: values and cycles)
 void amg_setup(tg_ctx* c);
 void amg_numeric(tg_ctx* c);
-template <class MT>
-void amg_vcycle(tg_ctx* c, const MT* val0, const double* f, double* z, const double* done);
+void amg_vcycle(tg_ctx* c, const double* f, double* z, const double* done);
 void eq_to_node(tg_ctx* c, const double* src_eq, double* dst_node);
 void node_to_eq(tg_ctx* c, const double* src_node, double* dst_eq);
 void launch_axpy(tg_ctx* c, double* y, const double* x, double a, long n);

@@ -170,6 +170,91 @@ __device__ __forceinline__ void simo_moduli(double mu, double kappa, double J, c
         }
 }
 
+// Kirchhoff-St.Venant (large_strain_StVenant, FDKStV.cpp with
+// FDHookeanMatT::s_ij/c_ijkl, FDHookeanMatT.cpp, and
+// IsotropicT::ComputeModuli, IsotropicT.cpp:153-169): S = lambda tr(E) 1 +
+// 2 mu E with E = (F^T F - 1)/2, pushed forward, sigma = F S F^T / J.  With
+// b = F F^T (Voigt 11,22,33,23,13,12) and F E F^T = (b b - b)/2 this is
+//   sigma = (lambda tr(E) b + mu (b b - b)) / J,  tr(E) = (tr b - 3)/2,
+// and the pushed-forward moduli of lambda 1x1 + 2 mu I are
+//   c_ijkl = (lambda b_ij b_kl + mu (b_ik b_jl + b_il b_jk)) / J.
+// Material parameters arrive as (mu, kappa): lambda = kappa - 2 mu / 3.
+__device__ __forceinline__ void stvenant_b(const double* f, double* b)
+{
+    b[0] = f[0] * f[0] + f[3] * f[3] + f[6] * f[6];
+    b[1] = f[1] * f[1] + f[4] * f[4] + f[7] * f[7];
+    b[2] = f[2] * f[2] + f[5] * f[5] + f[8] * f[8];
+    b[3] = f[1] * f[2] + f[4] * f[5] + f[7] * f[8];
+    b[4] = f[0] * f[2] + f[3] * f[5] + f[6] * f[8];
+    b[5] = f[0] * f[1] + f[3] * f[4] + f[6] * f[7];
+}
+
+// full 3x3 index of Voigt component v: (row, column)
+__device__ __forceinline__ int vrow(int v) { return v < 3 ? v : (v == 3 ? 1 : 0); }
+__device__ __forceinline__ int vcol(int v) { return v < 3 ? v : (v == 5 ? 1 : 2); }
+__device__ __forceinline__ int vidx(int i, int j)
+{
+    return i == j ? i : (i + j == 3 ? 3 : (i + j == 2 ? 4 : 5));
+}
+
+__device__ __forceinline__ void stvenant_cauchy(double mu, double kappa, double J, const double* b, double* sig)
+{
+    const double lam = kappa - 2.0 * mu / 3.0;
+    const double trE = 0.5 * (b[0] + b[1] + b[2] - 3.0);
+#pragma unroll
+    for (int v = 0; v < 6; v++) {
+        const int i = vrow(v), j = vcol(v);
+        double bb = 0.0;  // (b b)_ij
+#pragma unroll
+        for (int k = 0; k < 3; k++) bb += b[vidx(i, k)] * b[vidx(k, j)];
+        sig[v] = (lam * trE * b[v] + mu * (bb - b[v])) / J;
+    }
+}
+
+__device__ __forceinline__ void stvenant_moduli(double mu, double kappa, double J, const double* b, double c[6][6])
+{
+    const double lam = kappa - 2.0 * mu / 3.0;
+#pragma unroll
+    for (int v = 0; v < 6; v++)
+#pragma unroll
+        for (int w = 0; w < 6; w++) {
+            const int i = vrow(v), j = vcol(v), k = vrow(w), l = vcol(w);
+            c[v][w] = (lam * b[v] * b[w] + mu * (b[vidx(i, k)] * b[vidx(j, l)] + b[vidx(i, l)] * b[vidx(j, k)])) / J;
+        }
+}
+
+// Material models of the large-strain elements: the template parameter of
+// the element kernels.
+enum { MAT_SIMO = 0, MAT_STVENANT = 1 };
+
+// Cauchy stress of model MAT at F (J = det F > 0 checked by the caller);
+// kin holds what the tangent of the same model needs (J and b_bar for
+// Simo, J and b for St.Venant).  Returns false when det b <= 0.
+template <int MAT>
+__device__ __forceinline__ bool material_stress(double mu, double kappa, const double* F, double J, double& Jk,
+                                                double* kin, double* sig)
+{
+    if (MAT == MAT_STVENANT) {
+        stvenant_b(F, kin);
+        Jk = J;
+        stvenant_cauchy(mu, kappa, J, kin, sig);
+        return true;
+    } else {
+        const bool ok = simo_kinematics(F, Jk, kin);
+        simo_cauchy(mu, kappa, Jk, kin, sig);
+        return ok;
+    }
+}
+
+template <int MAT>
+__device__ __forceinline__ void material_moduli(double mu, double kappa, double Jk, const double* kin, double c[6][6])
+{
+    if (MAT == MAT_STVENANT)
+        stvenant_moduli(mu, kappa, Jk, kin, c);
+    else
+        simo_moduli(mu, kappa, Jk, kin, c);
+}
+
 // B-matrix structure (SolidElementT::Set_B 3D, SolidElementT.cpp:890-918):
 // B[v][3a+i] = g[bcomp(v,i)][a] or 0 when bcomp < 0.
 __device__ __forceinline__ int bcomp(int v, int i)

@@ -18,6 +18,7 @@ FEManagerHIP::FEManagerHIP(const InputDeck& deck, const Options& opt)
     for (size_t m = 0; m < deck.materials.size(); m++)
         fElements.SetMaterial((int)m, deck.materials[m].first, deck.materials[m].second);
     fElements.SetPrecision(opt.precision);
+    fElements.SetMaterialModel(deck.material_model);
     // one schedule drives every scheduled card: value_k * schedule(t)
     std::vector<int> nodes, dofs;
     std::vector<double> vals;
@@ -32,6 +33,14 @@ FEManagerHIP::FEManagerHIP(const InputDeck& deck, const Options& opt)
         vals.push_back(k.schedule >= 0 ? k.value : 0.0);
     }
     fElements.SetKinematicBC(nodes, dofs, vals);
+    std::vector<int> fn, fd;
+    std::vector<double> fv;
+    for (const FBCCard& f : deck.fbc) {
+        fn.push_back(f.node);
+        fd.push_back(f.dof);
+        fv.push_back(0.0);
+    }
+    fElements.SetForces(fn, fd, fv);
     const int n_eq = fElements.InitializeEquations();
     fLHS.Initialize(n_eq, n_eq, 1);
 }
@@ -52,6 +61,12 @@ const std::vector<StepRecord>& FEManagerHIP::Solve(std::ostream* log)
         const double t = n * fDeck.time_step;
         const double s = fSchedule >= 0 ? fDeck.schedules[(size_t)fSchedule].Value(t) : 0.0;
         fElements.InitStep(s);
+        if (!fDeck.fbc.empty()) {
+            std::vector<double> f;  // FBC_CardT::CurrentValue
+            for (const FBCCard& c : fDeck.fbc)
+                f.push_back(c.schedule >= 0 ? c.value * fDeck.schedules[(size_t)c.schedule].Value(t) : c.value);
+            fElements.ApplyForces(f);
+        }
         const NLSolverHIP::SolutionStatusT st = solver.Solve(-1);
         if (st != NLSolverHIP::kConverged) {
             fElements.ResetStep();

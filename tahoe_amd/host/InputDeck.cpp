@@ -381,7 +381,7 @@ InputDeck ReadInputDeck(const std::string& xml_path)
     const auto fields = nodes->Children("field");
     if (fields.size() != 1) ExceptionT::BadInputValue(caller, "exactly one field (displacement)");
     const XMLNode& field = *fields[0];
-    reject_unknown(field, {"dof_labels", "kinematic_BC", "mapped_nodes"});
+    reject_unknown(field, {"dof_labels", "kinematic_BC", "force_BC", "mapped_nodes"});
     auto node_set = [&](const std::string& id) -> const std::vector<int>& {
         auto it = d.geom.node_sets.find(to_int(id, "node_ID"));
         if (it == d.geom.node_sets.end()) ExceptionT::BadInputValue(caller, "no node set " + id);
@@ -406,6 +406,15 @@ InputDeck ReadInputDeck(const std::string& xml_path)
             ExceptionT::BadInputValue(caller, "kinematic_BC type \"" + type + "\" is not on the path");
         for (int n : node_set(k->Attr("node_ID"))) d.kbc.push_back({n, dof, value, sched});
     }
+    // FieldT's force cards (FieldT.cpp:1370, FBC_CardT::SetValues): value
+    // times the schedule's value, or the value itself without a schedule
+    for (const XMLNode* f : field.Children("force_BC")) {
+        const int dof = f->Int("dof", 0) - 1;
+        if (dof < 0 || dof > 2) ExceptionT::BadInputValue(caller, "force_BC dof must be 1..3");
+        const double value = f->Real("value", 0.0);
+        const int sched = f->Has("schedule") ? schedule_index(f->Int("schedule", 0)) : -1;
+        for (int n : node_set(f->Attr("node_ID"))) d.fbc.push_back({n, dof, value, sched});
+    }
     for (const XMLNode* m : field.Children("mapped_nodes")) {
         reject_unknown(*m, {"Matrix_3x3", "mapped_node_ID_list"});
         const int sched = schedule_index(m->Int("schedule", 0));
@@ -428,7 +437,10 @@ InputDeck ReadInputDeck(const std::string& xml_path)
     const XMLNode& grp = *el->children[0];
     std::string ename = grp.name;
     if (ename.size() > 4 && ename.compare(ename.size() - 4, 4, "_hip") == 0) ename.resize(ename.size() - 4);
-    if (ename == "total_lagrangian")
+    // updated_lagrangian integrates the same forms over the current
+    // configuration (UpdatedLagrangianT.cpp:83-92); it runs on the
+    // total-Lagrangian kernels
+    if (ename == "total_lagrangian" || ename == "updated_lagrangian")
         d.formulation = TG_TOTAL_LAGRANGIAN;
     else if (ename == "updated_lagrangian_Q1P0")
         d.formulation = TG_Q1P0;
@@ -441,12 +453,24 @@ InputDeck ReadInputDeck(const std::string& xml_path)
     if (!hex) ExceptionT::BadInputValue(caller, "only <hexahedron> geometry is on the path");
     if (hex->Int("num_ip", 8) != 8) ExceptionT::BadInputValue(caller, "hexahedron num_ip must be 8");
     std::map<int, int> block_material;  // element set ID -> material index
+    int model = -1;
     for (const XMLNode* b : grp.Children("large_strain_element_block")) {
         const XMLNode* m3 = b->Child("large_strain_material_3D");
-        const XMLNode* simo = m3 ? m3->Child("Simo_isotropic") : nullptr;
-        if (!simo) ExceptionT::BadInputValue(caller, "only Simo_isotropic materials are on the path");
-        const XMLNode* en = simo->Child("E_and_nu");
-        if (!en) ExceptionT::BadInputValue(caller, "Simo_isotropic needs <E_and_nu>");
+        const XMLNode* mat = nullptr;
+        int this_model = -1;
+        if (m3 && (mat = m3->Child("Simo_isotropic")))
+            this_model = TG_MAT_SIMO_ISOTROPIC;
+        else if (m3 && (mat = m3->Child("large_strain_StVenant")))
+            this_model = TG_MAT_STVENANT;
+        else
+            ExceptionT::BadInputValue(caller, "only Simo_isotropic and large_strain_StVenant materials are on the path");
+        if (model >= 0 && this_model != model)
+            ExceptionT::BadInputValue(caller, "all blocks of the group must use one material model");
+        model = this_model;
+        if (model == TG_MAT_STVENANT && d.formulation != TG_TOTAL_LAGRANGIAN)
+            ExceptionT::BadInputValue(caller, "large_strain_StVenant is on the path with the Lagrangian elements only");
+        const XMLNode* en = mat->Child("E_and_nu");
+        if (!en) ExceptionT::BadInputValue(caller, mat->name + " needs <E_and_nu>");
         const int mid = (int)d.materials.size();
         d.materials.emplace_back(en->Real("Young_modulus", 0.0), en->Real("Poisson_ratio", 0.0));
         for (int id : string_list(b->Child("block_ID_list"))) block_material[id] = mid;
@@ -459,6 +483,7 @@ InputDeck ReadInputDeck(const std::string& xml_path)
         d.elem_block.insert(d.elem_block.end(), c.size() / 8, it->second);
     }
     if (d.conn.empty()) ExceptionT::BadInputValue(caller, "the element group has no elements");
+    d.material_model = model < 0 ? TG_MAT_SIMO_ISOTROPIC : model;
 
     // ---- nonlinear solver (NLSolver) ----
     const XMLNode* ns = root->Child("nonlinear_solver");

@@ -8,13 +8,15 @@
 //         <OrderedPair x y/>...                        (ScheduleT, one or more)
 //     <nodes><field field_name="displacement">
 //         <kinematic_BC dof node_ID [type="u"|"fixed"] [schedule] [value]/>
+//         <force_BC dof node_ID [schedule] value/>        (FBC_CardT)
 //         <mapped_nodes schedule> <Matrix_3x3 A_i_j.../> <mapped_node_ID_list>
 //                                                      (MappedPeriodicT without
 //                                                       leader/follower pairs)
-//     <element_list><total_lagrangian | updated_lagrangian_Q1P0 (also *_hip)>
+//     <element_list><total_lagrangian | updated_lagrangian |
+//                    updated_lagrangian_Q1P0 (also *_hip)>
 //         <hexahedron [num_ip="8"]/>
 //         <large_strain_element_block><block_ID_list><String value/>
-//             <large_strain_material_3D><Simo_isotropic>
+//             <large_strain_material_3D><Simo_isotropic | large_strain_StVenant>
 //                 <E_and_nu Young_modulus Poisson_ratio/>
 //     <nonlinear_solver rel_tolerance abs_tolerance divergence_tolerance
 //                       max_iterations [min_iterations]>
@@ -74,18 +76,26 @@ struct KBCCard {
     int schedule;        // 0-based schedule, -1 for fixed
 };
 
+struct FBCCard {
+    int node, dof;       // 0-based
+    double value;        // multiplies the schedule value
+    int schedule;        // 0-based schedule, -1 for none (constant value)
+};
+
 struct InputDeck {
     std::string xml_path, geometry_path, title;
     int num_steps = 1;
     double time_step = 1.0;
     std::vector<ScheduleT> schedules;
-    std::string element;                 // "total_lagrangian" / "updated_lagrangian_Q1P0"
+    std::string element;                 // "total_lagrangian" / "updated_lagrangian" / "updated_lagrangian_Q1P0"
     int formulation = 0;                 // TG_TOTAL_LAGRANGIAN / TG_Q1P0
     Geometry geom;
     std::vector<int> conn;               // all element blocks [nel][8] 0-based
     std::vector<int> elem_block;         // material index of every element
     std::vector<std::pair<double, double>> materials;  // (E, nu) per material index
     std::vector<KBCCard> kbc;
+    std::vector<FBCCard> fbc;
+    int material_model = 0;              // TG_MAT_SIMO_ISOTROPIC / TG_MAT_STVENANT
     // nonlinear_solver
     double rel_tolerance = 1e-10, abs_tolerance = 1e-14, divergence_tolerance = 10.0;
     int max_iterations = 20, min_iterations = 0;

@@ -46,6 +46,27 @@ void TotalLagrangianHIPT::SetKinematicBC(const std::vector<int>& nodes, const st
                "TotalLagrangianHIPT::SetKinematicBC");
 }
 
+void TotalLagrangianHIPT::SetForces(const std::vector<int>& nodes, const std::vector<int>& dofs,
+                                    const std::vector<double>& values)
+{
+    if (nodes.size() != dofs.size() || nodes.size() != values.size())
+        ExceptionT::SizeMismatch("TotalLagrangianHIPT::SetForces");
+    fForceNodes = nodes;
+    fForceDofs = dofs;
+    fCtx.Check(tg_set_forces(fCtx.Handle(), (int)nodes.size(), nodes.data(), dofs.data(), values.data()),
+               "TotalLagrangianHIPT::SetForces");
+}
+
+void TotalLagrangianHIPT::ApplyForces(const std::vector<double>& current_values)
+{
+    if (current_values.size() != fForceNodes.size()) ExceptionT::SizeMismatch("TotalLagrangianHIPT::ApplyForces");
+    if (fForceNodes.empty()) return;
+    fCtx.Check(tg_set_forces(fCtx.Handle(), (int)fForceNodes.size(), fForceNodes.data(), fForceDofs.data(),
+                             current_values.data()),
+               "TotalLagrangianHIPT::ApplyForces");
+    fCtx.Check(tg_apply_forces(fCtx.Handle(), 1.0), "TotalLagrangianHIPT::ApplyForces");
+}
+
 int TotalLagrangianHIPT::InitializeEquations()
 {
     long len = 0;

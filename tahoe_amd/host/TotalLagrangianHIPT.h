@@ -43,6 +43,15 @@ public:
     void SetKinematicBC(const std::vector<int>& nodes, const std::vector<int>& dofs,
                         const std::vector<double>& values);
     void SetPrecision(int precision) { fCtx.Check(tg_set_precision(fCtx.Handle(), precision), "SetPrecision"); }
+    // material model of every block: TG_MAT_SIMO_ISOTROPIC or TG_MAT_STVENANT
+    void SetMaterialModel(int model)
+    {
+        fCtx.Check(tg_set_material_model(fCtx.Handle(), model), "SetMaterialModel");
+    }
+    // nodal force cards (FieldT's FBC_CardT list); ApplyForces sets their
+    // current values (value x schedule value, FBC_CardT::CurrentValue)
+    void SetForces(const std::vector<int>& nodes, const std::vector<int>& dofs, const std::vector<double>& values);
+    void ApplyForces(const std::vector<double>& current_values);
 
     // equation numbering and device storage (FEManagerT::SetEquationSystem)
     int InitializeEquations();
@@ -78,6 +87,7 @@ private:
     int fNumNodes = 0, fNumElements = 0, fNumEq = 0;
     std::map<int, std::pair<double, double>> fMaterials;  // block -> (E, nu)
     std::vector<int> fEqnos;
+    std::vector<int> fForceNodes, fForceDofs;
 };
 
 }  // namespace tahoe_hip

@@ -90,8 +90,11 @@ int main(int argc, char** argv)
         for (const KBCCard& c : deck.kbc) scheduled += c.schedule >= 0;
         std::cout << " deck " << deck_path << ": " << deck.element << ", " << deck.geom.NumNodes() << " nodes, "
                   << deck.conn.size() / 8 << " elements, " << deck.materials.size() << " materials, "
-                  << deck.kbc.size() << " kinematic cards (" << scheduled << " scheduled), " << deck.num_steps
-                  << " steps, matrix " << (deck.matrix.empty() ? "-" : deck.matrix) << std::endl;
+                  << deck.kbc.size() << " kinematic cards (" << scheduled << " scheduled), " << deck.fbc.size()
+                  << " force cards, " << (deck.material_model == TG_MAT_STVENANT ? "large_strain_StVenant"
+                                                                                  : "Simo_isotropic")
+                  << ", " << deck.num_steps << " steps, matrix " << (deck.matrix.empty() ? "-" : deck.matrix)
+                  << std::endl;
         if (check) return 0;
         FEManagerHIP fe(deck, opt);
         std::cout << " equations " << fe.Elements().NumEquations() << ", MSR length " << fe.Matrix().MSRLength()

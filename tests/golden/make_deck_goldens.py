@@ -2,11 +2,16 @@
 """Expected nodal displacements for the reference's own input decks.
 
 Runs only in the development container: reads the reference's committed
-benchmark output (benchmark_XML/level.0/enhanced.strain/benchmark/
-Q1P0.3D.b.io0.run, written by the reference's OutputBaseT) and keeps the
-D_X, D_Y, D_Z columns of every print step as tests/golden/q1p0_3d_b_run.npz
-(time, u[step][node][3]).  The deck and its geometry are the reference's
-input files, copied unchanged to tests/golden/decks/.
+benchmark output (.run files under benchmark_XML/level.0/*/benchmark/,
+written by the reference's OutputBaseT) and keeps the D_X, D_Y, D_Z columns
+of every print step as tests/golden/<name>_run.npz (time, u[step][node][3]):
+  Q1P0.3D.b          enhanced.strain (updated_lagrangian_Q1P0, Simo_isotropic)
+  beam.Newton        3D.elastostatic (updated_lagrangian, large_strain_StVenant,
+                     force_BC, nonlinear_solver)
+  beam.Newton.TotLag 3D.elastostatic (total_lagrangian, otherwise the same)
+  beam.PCG           3D.elastostatic (updated_lagrangian, PCG_solver)
+The decks and their geometry are the reference's input files, copied
+unchanged to tests/golden/decks/ and tests/golden/geometry/.
 """
 import os
 import re
@@ -14,7 +19,13 @@ import sys
 
 import numpy as np
 
-REF = "/root/reference/benchmark_XML/level.0/enhanced.strain/benchmark/Q1P0.3D.b.io0.run"
+LEVEL0 = "/root/reference/benchmark_XML/level.0"
+RUNS = {
+    "q1p0_3d_b": "enhanced.strain/benchmark/Q1P0.3D.b.io0.run",
+    "beam_newton": "3D.elastostatic/benchmark/beam.Newton.io0.run",
+    "beam_newton_totlag": "3D.elastostatic/benchmark/beam.Newton.TotLag.io0.run",
+    "beam_pcg": "3D.elastostatic/benchmark/beam.PCG.io0.run",
+}
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -42,11 +53,12 @@ def parse_run(text, ncols=3):
 
 
 def main():
-    src = sys.argv[1] if len(sys.argv) > 1 else REF
-    steps = parse_run(open(src).read())
-    np.savez(os.path.join(HERE, "q1p0_3d_b_run.npz"), time=np.array([s[0] for s in steps]),
-             u=np.array([s[1] for s in steps]))
-    print(f"{len(steps)} steps, {steps[0][1].shape[0]} nodes")
+    root = sys.argv[1] if len(sys.argv) > 1 else LEVEL0
+    for name, rel in RUNS.items():
+        steps = parse_run(open(os.path.join(root, rel)).read())
+        np.savez(os.path.join(HERE, name + "_run.npz"), time=np.array([s[0] for s in steps]),
+                 u=np.array([s[1] for s in steps]))
+        print(f"{name}: {len(steps)} steps, {steps[0][1].shape[0]} nodes")
 
 
 if __name__ == "__main__":

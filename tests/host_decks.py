@@ -51,3 +51,40 @@ def parse_run(path):
             i = j + n
         i += 1
     return out
+
+
+def read_geom(path):
+    """Nodes [nnd][3], hexahedra [nel][8] (0-based) and node sets {ID: 0-based
+    nodes} of a TahoeII text geometry file (the sections *nodesets,
+    *elements, *nodes; '#' starts a comment)."""
+    toks = []
+    for line in open(path):
+        toks += line.split("#")[0].split()
+    i = 0
+    sets, X, conn = [], None, []
+
+    def num():
+        nonlocal i
+        i += 1
+        return toks[i - 1]
+    while i < len(toks):
+        t = num()
+        if t == "*nodesets":
+            while i < len(toks) and toks[i] == "*set":
+                num()
+                n = int(num())
+                sets.append([int(num()) - 1 for _ in range(n)])
+        elif t == "*elements":
+            while i < len(toks) and toks[i] == "*set":
+                num()
+                nel, nen = int(num()), int(num())
+                for _ in range(nel):
+                    num()
+                    conn.append([int(num()) - 1 for _ in range(nen)])
+        elif t == "*nodes":
+            nnd, nsd = int(num()), int(num())
+            X = np.zeros((nnd, nsd))
+            for a in range(nnd):
+                num()
+                X[a] = [float(num()) for _ in range(nsd)]
+    return X, np.array(conn, dtype=np.int32), {k + 1: np.array(v, dtype=np.int32) for k, v in enumerate(sets)}

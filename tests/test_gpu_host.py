@@ -55,3 +55,19 @@ def test_fixture_deck(tmp_path, name, args):
     if not args:
         iters = [int(l.split(":")[1].split()[0]) for l in r.stdout.splitlines() if l.startswith(" step ")]
         assert iters == g["step_iters"].tolist()
+
+
+@pytest.mark.parametrize("deck,golden", [("beam.Newton.xml", "beam_newton_run.npz"),
+                                         ("beam.Newton.TotLag.xml", "beam_newton_totlag_run.npz")])
+def test_reference_beam_decks(tmp_path, deck, golden):
+    """The reference's own beam decks (updated_lagrangian / total_lagrangian,
+    large_strain_StVenant, force_BC, nonlinear_solver with profile_matrix)
+    against the reference's committed output (13 significant digits)."""
+    out = str(tmp_path / "beam.run")
+    r = run(["-f", os.path.join(DECKS, deck), "-o", out, "-linear_tolerance", "1e-13"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    ref = np.load(os.path.join(os.path.dirname(DECKS), golden))
+    got = parse_run(out)
+    assert [t for t, _ in got] == list(ref["time"])
+    for (t, u), ur in zip(got, ref["u"]):
+        assert np.abs(u - ur).max() <= 1e-10 * np.abs(ur).max(), t

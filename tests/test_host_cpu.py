@@ -19,8 +19,19 @@ def test_reference_q1p0_deck_parses():
     r = run(["-f", os.path.join(DECKS, "Q1P0.3D.b.xml"), "-check"])
     assert r.returncode == 0, r.stderr
     # 18 nodes, 4 hexes; 16 mapped nodes x 3 dofs + 2 nodes with dof 3 fixed
-    assert "updated_lagrangian_Q1P0, 18 nodes, 4 elements, 1 materials, 50 kinematic cards (48 scheduled), 2 steps" \
-        in r.stdout
+    assert "updated_lagrangian_Q1P0, 18 nodes, 4 elements, 1 materials, 50 kinematic cards (48 scheduled), " \
+        "0 force cards, Simo_isotropic, 2 steps" in r.stdout
+
+
+@pytest.mark.parametrize("deck,element", [("beam.Newton.xml", "updated_lagrangian"),
+                                          ("beam.Newton.TotLag.xml", "total_lagrangian")])
+def test_reference_beam_decks_parse(deck, element):
+    """The reference's beam decks: node set 1 (4 nodes) fixed in 3 dofs, one
+    force card on node set 2, large_strain_StVenant, profile_matrix."""
+    r = run(["-f", os.path.join(DECKS, deck), "-check"])
+    assert r.returncode == 0, r.stderr
+    assert f"{element}, 12 nodes, 2 elements, 1 materials, 12 kinematic cards (0 scheduled), 1 force cards, " \
+        "large_strain_StVenant, 1 steps, matrix profile_matrix" in r.stdout
 
 
 def test_fixture_deck_parses(tmp_path):
@@ -31,13 +42,14 @@ def test_fixture_deck_parses(tmp_path):
     nk = len(g["kbc_nodes"])
     nsched = int(np.count_nonzero(g["kbc_values"]))
     assert f"total_lagrangian, 4913 nodes, 4096 elements, 1 materials, {nk} kinematic cards ({nsched} scheduled), " \
-           "5 steps" in r.stdout
+           "0 force cards, Simo_isotropic, 5 steps" in r.stdout
 
 
 @pytest.mark.parametrize("edit,code,text", [
-    (lambda s: s.replace("total_lagrangian", "updated_lagrangian"), 6, "not on the path"),
-    (lambda s: s.replace("Simo_isotropic", "large_strain_StVenant"), 6, "Simo_isotropic"),
-    (lambda s: s.replace('<kinematic_BC dof="1"', '<force_BC dof="1"', 1), 6, "force_BC"),
+    (lambda s: s.replace("total_lagrangian", "small_strain"), 6, "not on the path"),
+    (lambda s: s.replace("Simo_isotropic", "Ogden_isotropic"), 6, "Simo_isotropic"),
+    (lambda s: s.replace('<kinematic_BC dof="1"', '<damping_BC dof="1"', 1), 6, "damping_BC"),
+    (lambda s: s.replace('<kinematic_BC dof="1"', '<force_BC dof="4"', 1), 6, "force_BC dof"),
     (lambda s: s.replace("</tahoe>", ""), 6, "missing </tahoe>"),
     (lambda s: s.replace('geometry_file="deck.geom"', 'geometry_file="none.geom"'), 9, "cannot open"),
 ])

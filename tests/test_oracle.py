@@ -8,6 +8,8 @@ reference's tests/materials/test_NeoHookean.cpp and test_IsotropicT.cpp
 assert), objectivity, finite differences of energy and residual, partition
 of unity, patch tests, and the MSR nonzero count formula.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -195,6 +197,61 @@ def test_simo_objectivity_and_energy_derivative():
     assert np.abs(sig - s).max() <= 1e-6 * np.abs(s).max()
 
 
+def test_stvenant_reference_state_and_uniaxial():
+    """large_strain_StVenant: at F = I, sigma = 0 and c = lambda 1x1 + 2 mu I
+    (IsotropicT::ComputeModuli); under F = diag(s, 1, 1), E11 = (s^2-1)/2,
+    S11 = (lambda + 2 mu) E11, S22 = S33 = lambda E11, and the Cauchy stress
+    is sigma11 = s S11, sigma22 = sigma33 = S22 / s (no shear)."""
+    E, nu = 100.0, 0.25
+    mu, ka = O.mu_kappa(E, nu)
+    lam = ka - 2 * mu / 3
+    assert np.abs(O.stvenant_cauchy(mu, ka, np.eye(3))).max() == 0.0
+    ref = np.zeros((6, 6))
+    ref[:3, :3] = lam
+    ref[np.arange(3), np.arange(3)] += 2 * mu
+    ref[np.arange(3, 6), np.arange(3, 6)] = mu
+    assert np.abs(O.stvenant_moduli(mu, ka, np.eye(3)) - ref).max() <= 1e-13 * mu
+    for st in (1.3, 0.8):
+        s = O.stvenant_cauchy(mu, ka, np.diag([st, 1.0, 1.0]))
+        E11 = 0.5 * (st * st - 1)
+        assert s[0] == pytest.approx(st * (lam + 2 * mu) * E11, rel=1e-13)
+        assert s[1] == pytest.approx(lam * E11 / st, rel=1e-13)
+        assert s[2] == pytest.approx(lam * E11 / st, rel=1e-13)
+        assert np.abs(s[3:]).max() == 0.0
+    assert O.stvenant_energy(mu, ka, np.eye(3)) == 0.0
+
+
+def test_stvenant_objectivity_and_derivatives():
+    """sigma(QF) = Q sigma(F) Q^T; sigma = (1/J) dW/dF F^T; and the spatial
+    moduli satisfy J c_ijkl = F_iI F_jJ dS_IJ/dE_KL F_kK F_lL, checked as the
+    directional derivative of J sigma along a symmetric rate D:
+    d(J sigma)/dt = L (J sigma) + (J sigma) L^T + J c:D for F' = L F, L = D."""
+    mu, ka = O.mu_kappa(100.0, 0.25)
+    F = np.array([[1.05, 0.02, -0.01], [0.03, 0.97, 0.015], [-0.02, 0.01, 1.08]])
+    Q = _rot(0.3, -0.7, 1.1)
+    s = _sym(O.stvenant_cauchy(mu, ka, F))
+    sq = _sym(O.stvenant_cauchy(mu, ka, Q @ F))
+    assert np.abs(sq - Q @ s @ Q.T).max() <= 1e-13 * np.abs(s).max()
+    h = 1e-6
+    P = np.zeros((3, 3))
+    for i in range(3):
+        for j in range(3):
+            Fp, Fm = F.copy(), F.copy()
+            Fp[i, j] += h
+            Fm[i, j] -= h
+            P[i, j] = (O.stvenant_energy(mu, ka, Fp) - O.stvenant_energy(mu, ka, Fm)) / (2 * h)
+    J = np.linalg.det(F)
+    assert np.abs(P @ F.T / J - s).max() <= 1e-6 * np.abs(s).max()
+    D = np.array([[0.3, 0.1, -0.2], [0.1, -0.4, 0.05], [-0.2, 0.05, 0.2]])
+    tau = lambda G: np.linalg.det(G) * _sym(O.stvenant_cauchy(mu, ka, G))
+    dtau = (tau(F + h * D @ F) - tau(F - h * D @ F)) / (2 * h)
+    c = O.stvenant_moduli(mu, ka, F)
+    Dv = np.array([D[0, 0], D[1, 1], D[2, 2], 2 * D[1, 2], 2 * D[0, 2], 2 * D[0, 1]])
+    cD = _sym(c @ Dv)
+    t = tau(F)
+    assert np.abs(dtau - (D @ t + t @ D.T + J * cD)).max() <= 1e-6 * np.abs(t).max()
+
+
 def _elem(seed=0):
     rng = np.random.default_rng(seed)
     verts = 0.5 * (1 + np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
@@ -203,7 +260,7 @@ def _elem(seed=0):
     return Xe
 
 
-@pytest.mark.parametrize("kind", [O.TL, O.Q1P0])
+@pytest.mark.parametrize("kind", [O.TL, O.Q1P0, O.TL_STVENANT])
 def test_element_stiffness_is_residual_derivative(kind):
     """K = d f_int / du = -dR/du, by central differences of the element residual."""
     mu, ka = O.mu_kappa(1000.0, 0.3)
@@ -220,13 +277,13 @@ def test_element_stiffness_is_residual_derivative(kind):
         Rp, _ = O.element(kind, Xe, up.reshape(8, 3), mu, ka, want_K=False)
         Rm, _ = O.element(kind, Xe, um.reshape(8, 3), mu, ka, want_K=False)
         Kfd[:, j] = -(Rp - Rm) / (2 * h)
-    tol = 1e-6 if kind == O.TL else 1e-5
+    tol = 1e-5 if kind == O.Q1P0 else 1e-6
     assert np.abs(K - Kfd).max() <= tol * np.abs(K).max()
-    if kind == O.TL:
+    if kind != O.Q1P0:
         assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
 
 
-@pytest.mark.parametrize("kind", [O.TL, O.Q1P0])
+@pytest.mark.parametrize("kind", [O.TL, O.Q1P0, O.TL_STVENANT])
 def test_element_rigid_motion_and_translation(kind):
     """A rigid rotation of the element produces no stress, so no force; a
     rigid translation leaves R and K unchanged."""
@@ -241,7 +298,7 @@ def test_element_rigid_motion_and_translation(kind):
     R1, K1 = O.element(kind, Xe, ue, mu, ka)
     R2, K2 = O.element(kind, Xe, ue + np.array([0.3, -0.1, 0.2]), mu, ka)
     assert np.abs(R1 - R2).max() <= 1e-12 * np.abs(R1).max()
-    tol = 1e-12 if kind == O.TL else 1e-7
+    tol = 1e-7 if kind == O.Q1P0 else 1e-12
     assert np.abs(K1 - K2).max() <= tol * np.abs(K1).max()
 
 
@@ -274,3 +331,40 @@ def test_pcg_and_multx_against_dense():
     assert rc == 0
     xd = np.linalg.solve(A, rhs)
     assert np.abs(xs - xd).max() <= 1e-9 * np.abs(xd).max()
+
+
+def test_beam_stvenant_force_matches_reference_run():
+    """The reference's beam.Newton.TotLag deck (2 hexahedra, node set 1
+    fixed, force_BC -0.1 in x on node set 2, large_strain_StVenant E = 100,
+    nu = 0.25, one step) solved by the oracle's Newton with the force card
+    in the residual (FieldT::FormRHS): the displacements of the reference's
+    committed .run (13 significant digits).  Both the total- and the
+    updated-Lagrangian runs of the reference print the same values."""
+    from conftest import GOLDEN
+    from host_decks import read_geom
+    X, conn, sets = read_geom(os.path.join(GOLDEN, "geometry", "beam.1.geom"))
+    kn = np.repeat(sets[1], 3)
+    kd = np.tile(np.arange(3), len(sets[1]))
+    eq, n = O.equation_numbers(X.shape[0], kn, kd)
+    b = O.msr_pattern(n, conn, eq)
+    mu, ka = O.mu_kappa(100.0, 0.25)
+    fext = np.zeros(n)
+    for node in sets[2]:
+        fext[eq[node, 0] - 1] += -0.1
+    u = np.zeros_like(X)
+    free = eq > 0
+    mat = np.zeros(len(conn), dtype=np.int32)
+    R = O.residual(O.TL_STVENANT, X, conn, mat, mu, ka, u, eq, n) + fext
+    r0 = np.linalg.norm(R)
+    for it in range(10):
+        val = O.stiffness(O.TL_STVENANT, X, conn, mat, mu, ka, u, eq, n, b)
+        du, its, rr, rc = O.pcg_jacobi(b, val, R, rtol=1e-14)
+        assert rc == 0
+        u[free] += du[eq[free] - 1]
+        R = O.residual(O.TL_STVENANT, X, conn, mat, mu, ka, u, eq, n) + fext
+        if np.linalg.norm(R) < 1e-12 * r0:
+            break
+    assert np.linalg.norm(R) < 1e-12 * r0
+    for name in ("beam_newton_totlag", "beam_newton"):
+        ref = np.load(os.path.join(GOLDEN, name + "_run.npz"))["u"][0]
+        assert np.abs(u - ref).max() <= 1e-10 * np.abs(ref).max(), name
