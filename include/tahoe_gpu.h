@@ -121,6 +121,16 @@ int tg_set_precision(tg_ctx* ctx, int precision);
  * structure is built on the host when first needed. */
 enum { TG_PC_JACOBI = 0, TG_PC_BLOCK_JACOBI = 1, TG_PC_AMG = 2 };
 int tg_set_preconditioner(tg_ctx* ctx, int pc);
+/* Linear solver inside tg_newton, tg_newton_correction and tg_solve:
+ * TG_LIN_PCG (default) is the segmented preconditioned CG; TG_LIN_GMRES is
+ * the newton_krylov_solver's restarted GMRES(restart) with left
+ * preconditioning (NLSolver_NK::GMRES, NLSolver_NK.cpp:121-238): a restart
+ * cycle stops when the preconditioned residual falls below lin_rtol times
+ * its value at the start of that cycle, and lin_maxit <= 0 means 3 n_eq
+ * Arnoldi steps (max_linear_iter = 0).  GMRES needs a single problem (no
+ * segments). */
+enum { TG_LIN_PCG = 0, TG_LIN_GMRES = 1 };
+int tg_set_linear_solver(tg_ctx* ctx, int solver, int restart);
 /* Kinematic BC cards: prescribed value = values[k] * t (KBC_CardT::Value,
  * tahoe/src/primitives/KBC_CardT.cpp:53-64, linear schedule). */
 int tg_set_kbc(tg_ctx* ctx, int nkbc, const int* nodes, const int* dofs, const double* values);

@@ -51,6 +51,8 @@ def lib():
         L.or_residual.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, I, C.c_int, D]
         L.or_stiffness.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, I, C.c_int, I, D]
         L.or_msr_multx.argtypes = [C.c_int, I, D, D, D]
+        L.or_gmres_dense.argtypes = [C.c_int, D, D, D, C.c_int, C.c_double, C.c_int, I]
+        L.or_gmres_msr.argtypes = [C.c_int, I, D, D, D, C.c_int, C.c_double, C.c_int, I]
         L.or_pcg_jacobi.argtypes = [C.c_int, I, D, D, D, C.c_double, C.c_int, I, D]
         L.or_static_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, C.c_int, I, I, D,
                                       C.c_int, C.POINTER(NewtonOpts), D, D, I, I]
@@ -220,3 +222,23 @@ def static_solve(kind, X, conn, mat, mu, kappa, kbc_nodes, kbc_dofs, kbc_values,
                                len(kn), _i(kn), _i(kd), _d(kv), num_steps, C.byref(o), _d(u), _d(step_u),
                                _i(iters), C.byref(lin))
     return dict(status=rc, u=u, step_u=step_u, step_iters=iters, lin_iters=lin.value)
+
+
+def gmres_dense(A, b, m, tol, maxit=0, x0=None):
+    """NLSolver_NK::GMRES on a dense matrix: returns (x, converged, Arnoldi steps)."""
+    A = f64(A)
+    n = A.shape[0]
+    b = f64(b)
+    x = f64(np.zeros(n) if x0 is None else x0).copy()
+    st = C.c_int()
+    rc = lib().or_gmres_dense(n, _d(A), _d(b), _d(x), m, tol, maxit, C.byref(st))
+    return x, bool(rc), st.value
+
+
+def gmres_msr(bindx, val, b, m, tol, maxit=0):
+    n = int(bindx[0]) - 1
+    bindx, val, b = i32(bindx), f64(val), f64(b)
+    x = np.zeros(n)
+    st = C.c_int()
+    rc = lib().or_gmres_msr(n, _i(bindx), _d(val), _d(b), _d(x), m, tol, maxit, C.byref(st))
+    return x, bool(rc), st.value

@@ -787,6 +787,125 @@ int or_pcg_jacobi(int n, const int* bindx, const double* val, const double* b, d
 /* A11: static Newton time loop                                              */
 /* ------------------------------------------------------------------------ */
 
+/* Restarted GMRES(m) with left Jacobi preconditioning, NLSolver_NK::GMRES
+ * (NLSolver_NK.cpp:121-238): M = diag(A) (1 where |A_ii| <= 1e-14); each
+ * restart cycle starts from r = M^-1 (b - A x), beta = |r|, and stops when the
+ * Givens-rotated residual |g_{j+1}| < tol * beta (tolerance relative to the
+ * cycle's own start); Arnoldi by modified Gram-Schmidt; Givens rotations
+ * [c s; -s c] (NLSolver_NK.cpp:241-246); back substitution skipping
+ * |H_ii| <= 1e-14; at most maxit Arnoldi steps in total (the reference's
+ * 3 n when max_linear_iter = 0).  matvec(ctx, x, y) computes y = A x.
+ * Returns 1 when converged; *steps gets the Arnoldi steps taken. */
+typedef void (*matvec_fn)(const void* ctx, const double* x, double* y);
+
+static int gmres(int n, matvec_fn matvec, const void* ctx, const double* diag, const double* b, double* x, int m,
+                 double tol, int maxit, int* steps)
+{
+    double* Q = (double*)calloc((size_t)(m + 1) * n, sizeof(double));
+    double* H = (double*)calloc((size_t)(m + 1) * m, sizeof(double));
+    double* g = (double*)calloc((size_t)m + 1, sizeof(double));
+    double* cs = (double*)calloc((size_t)m, sizeof(double));
+    double* sn = (double*)calloc((size_t)m, sizeof(double));
+    double* y = (double*)calloc((size_t)m, sizeof(double));
+    double* w = (double*)calloc((size_t)n, sizeof(double));
+    double* z = (double*)calloc((size_t)n, sizeof(double));
+    double* md = (double*)calloc((size_t)n, sizeof(double));
+#define HH(i, j) H[(size_t)(i) * m + (j)]
+    for (int i = 0; i < n; i++) md[i] = fabs(diag[i]) > 1.0e-14 ? 1.0 / diag[i] : 1.0;
+    int total = 0, done = 0;
+    while (!done && total < maxit) {
+        matvec(ctx, x, z);
+        for (int i = 0; i < n; i++) w[i] = (b[i] - z[i]) * md[i];
+        double beta = sqrt(dot(n, w, w));
+        if (beta == 0.0) { done = 1; break; }
+        const double tol_abs = tol * beta;
+        if (tol_abs == 0.0) { done = 1; break; }
+        for (int i = 0; i < n; i++) Q[i] = w[i] / beta;
+        for (int i = 0; i <= m; i++) g[i] = 0.0;
+        g[0] = beta;
+        int j_end = 0;
+        for (int j = 0; j < m && total < maxit; j++, total++) {
+            j_end = j;
+            matvec(ctx, Q + (size_t)j * n, w);
+            for (int i = 0; i < n; i++) w[i] *= md[i];
+            for (int i = 0; i <= j; i++) {
+                const double h = dot(n, w, Q + (size_t)i * n);
+                HH(i, j) = h;
+                for (int k = 0; k < n; k++) w[k] -= h * Q[(size_t)i * n + k];
+            }
+            const double hn = sqrt(dot(n, w, w));
+            HH(j + 1, j) = hn;
+            if (hn > 0.0)
+                for (int k = 0; k < n; k++) Q[(size_t)(j + 1) * n + k] = w[k] / hn;
+            for (int i = 0; i < j; i++) {
+                const double t = cs[i] * HH(i, j) + sn[i] * HH(i + 1, j);
+                HH(i + 1, j) = -sn[i] * HH(i, j) + cs[i] * HH(i + 1, j);
+                HH(i, j) = t;
+            }
+            const double den = sqrt(HH(j, j) * HH(j, j) + HH(j + 1, j) * HH(j + 1, j));
+            cs[j] = den > 0.0 ? HH(j, j) / den : 1.0;
+            sn[j] = den > 0.0 ? HH(j + 1, j) / den : 0.0;
+            double t = cs[j] * HH(j, j) + sn[j] * HH(j + 1, j);
+            HH(j + 1, j) = -sn[j] * HH(j, j) + cs[j] * HH(j + 1, j);
+            HH(j, j) = t;
+            t = cs[j] * g[j] + sn[j] * g[j + 1];
+            g[j + 1] = -sn[j] * g[j] + cs[j] * g[j + 1];
+            g[j] = t;
+            if (fabs(g[j + 1]) < tol_abs) {
+                j_end = j;
+                done = 1;
+                break;
+            }
+        }
+        for (int i = j_end; i >= 0; i--) {
+            double sum = g[i];
+            for (int l = i + 1; l <= j_end; l++) sum -= HH(i, l) * y[l];
+            y[i] = fabs(HH(i, i)) > 1.0e-14 ? sum / HH(i, i) : 0.0;
+        }
+        for (int i = 0; i <= j_end; i++)
+            for (int k = 0; k < n; k++) x[k] += y[i] * Q[(size_t)i * n + k];
+    }
+#undef HH
+    if (steps) *steps = total;
+    free(Q); free(H); free(g); free(cs); free(sn); free(y); free(w); free(z); free(md);
+    return done;
+}
+
+typedef struct { int n; const double* A; } dense_t;
+static void dense_mv(const void* ctx, const double* x, double* y)
+{
+    const dense_t* d = (const dense_t*)ctx;
+    for (int i = 0; i < d->n; i++) {
+        double s = 0.0;
+        for (int k = 0; k < d->n; k++) s += d->A[(size_t)i * d->n + k] * x[k];
+        y[i] = s;
+    }
+}
+
+int or_gmres_dense(int n, const double* A, const double* b, double* x, int m, double tol, int maxit, int* steps)
+{
+    dense_t d = {n, A};
+    double* diag = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) diag[i] = A[(size_t)i * n + i];
+    const int rc = gmres(n, dense_mv, &d, diag, b, x, m, tol, maxit > 0 ? maxit : 3 * n, steps);
+    free(diag);
+    return rc;
+}
+
+typedef struct { int n; const int* bindx; const double* val; } msr_t;
+static void msr_mv(const void* ctx, const double* x, double* y)
+{
+    const msr_t* d = (const msr_t*)ctx;
+    or_msr_multx(d->n, d->bindx, d->val, x, y);
+}
+
+int or_gmres_msr(int n, const int* bindx, const double* val, const double* b, double* x, int m, double tol,
+                 int maxit, int* steps)
+{
+    msr_t d = {n, bindx, val};
+    return gmres(n, msr_mv, &d, val, b, x, m, tol, maxit > 0 ? maxit : 3 * n, steps);
+}
+
 /* FEManagerT::Solve (FEManagerT.cpp:138-269) with the static integrator:
  * per step t = s/num_steps, nStaticIntegrator::ConsistentKBC
  * (nStaticIntegrator.cpp:18-46) sets u = value * t on each card (schedule

@@ -63,6 +63,10 @@ int or_stiffness(int type, int nnd, int nel, const double* X, const int* conn, c
 
 /* ---- linear algebra ---- */
 void or_msr_multx(int n, const int* bindx, const double* val, const double* x, double* b);
+/* restarted GMRES(m), left Jacobi (NLSolver_NK::GMRES); maxit <= 0: 3 n */
+int or_gmres_dense(int n, const double* A, const double* b, double* x, int m, double tol, int maxit, int* steps);
+int or_gmres_msr(int n, const int* bindx, const double* val, const double* b, double* x, int m, double tol,
+                 int maxit, int* steps);
 int or_pcg_jacobi(int n, const int* bindx, const double* val, const double* b, double* x,
                   double rtol, int maxit, int* iters, double* relres);
 

@@ -20,6 +20,7 @@ FP64, MIXED = 0, 1
 TUNE_STIFF_CHUNK, TUNE_STIFF_SHAPE, TUNE_NEWTON_ASSEMBLY, TUNE_PCG_GRAPH, TUNE_SPMV_XCD = 1, 2, 3, 4, 5
 TUNE_AMG_DEGREE, TUNE_AMG_WEIGHT = 6, 7
 PC_JACOBI, PC_BLOCK_JACOBI, PC_AMG = 0, 1, 2
+LIN_PCG, LIN_GMRES = 0, 1
 STIFF_SHAPE_EPB = (16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4, 12, 8, 4, 16)
 STIFF_SHAPE_BLOCK_MAJOR = (False,) * 4 + (True, True) + (False,) * 9 + (True,)
 STIFF_SHAPE_DB_FORM = (False,) * 6 + (True,) * 3 + (False,) * 7
@@ -31,7 +32,7 @@ I = C.POINTER(C.c_int)
 # exported symbols, in declaration order of include/tahoe_gpu.h
 SYMBOLS = [
     "tg_create", "tg_destroy", "tg_last_error", "tg_set_mesh", "tg_set_material", "tg_set_formulation",
-    "tg_set_material_model", "tg_set_precision", "tg_set_preconditioner", "tg_set_kbc", "tg_set_forces",
+    "tg_set_material_model", "tg_set_precision", "tg_set_preconditioner", "tg_set_linear_solver", "tg_set_kbc", "tg_set_forces",
     "tg_apply_forces", "tg_set_segments", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
     "tg_get_displacement", "tg_apply_kbc", "tg_close_step", "tg_reset_step", "tg_form_rhs", "tg_form_lhs",
     "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_batch", "tg_newton_correction",
@@ -102,6 +103,7 @@ def lib():
         L.tg_apply_forces.argtypes = [vp, C.c_double]
         L.tg_set_precision.argtypes = [vp, C.c_int]
         L.tg_set_preconditioner.argtypes = [vp, C.c_int]
+        L.tg_set_linear_solver.argtypes = [vp, C.c_int, C.c_int]
         L.tg_set_kbc.argtypes = [vp, C.c_int, I, I, D]
         L.tg_set_segments.argtypes = [vp, C.c_int, I]
         L.tg_initialize.argtypes = [vp, I, C.POINTER(C.c_long)]
@@ -192,6 +194,9 @@ class Solid:
 
     def set_material_model(self, model):
         self._ok(lib().tg_set_material_model(self._h, int(model)))
+
+    def set_linear_solver(self, solver, restart=30):
+        self._ok(lib().tg_set_linear_solver(self._h, int(solver), int(restart)))
 
     def set_forces(self, nodes, dofs, values):
         nodes = np.ascontiguousarray(nodes, dtype=np.int32)

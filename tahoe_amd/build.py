@@ -20,7 +20,7 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HOST_SOURCES = ["TotalLagrangianHIPT.cpp", "HIPMSRMatrixT.cpp", "NLSolverHIP.cpp", "InputDeck.cpp",
                 "FEManagerHIP.cpp"]
 
-SOURCES = ["tg_setup.cpp", "tg_amg.cpp", "tg_api.cpp", "k_element.hip", "k_solver.hip", "k_pcg.hip", "k_amg.hip"]
+SOURCES = ["tg_setup.cpp", "tg_amg.cpp", "tg_api.cpp", "k_element.hip", "k_solver.hip", "k_pcg.hip", "k_amg.hip", "k_gmres.hip"]
 HEADERS = ["tg_internal.h", "tg_kinematics.h", "../../include/tahoe_gpu.h"]
 
 ARCH = os.environ.get("TAHOE_GPU_ARCH", "gfx950")

@@ -340,6 +340,20 @@ int tg_set_preconditioner(tg_ctx* c, int pc)
     });
 }
 
+int tg_set_linear_solver(tg_ctx* c, int solver, int restart)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        if (solver != TG_LIN_PCG && solver != TG_LIN_GMRES) fail(TG_BAD_INPUT, "unknown linear solver");
+        if (solver == TG_LIN_GMRES && (restart < 1 || restart > 1000))
+            fail(TG_OUT_OF_RANGE, "GMRES restart must be 1..1000");
+        if (solver == TG_LIN_GMRES && c->seg.nseg != 1)
+            fail(TG_BAD_INPUT, "GMRES needs a single problem (no segments)");
+        c->linear_solver = solver;
+        if (solver == TG_LIN_GMRES) c->gmres_restart = restart;
+    });
+}
+
 int tg_set_kbc(tg_ctx* c, int nkbc, const int* nodes, const int* dofs, const double* values)
 {
     if (!c) return TG_BAD_INPUT;
@@ -645,6 +659,10 @@ static void newton_operator(tg_ctx* c)
 
 static void solve_linear(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
 {
+    if (c->linear_solver == TG_LIN_GMRES) {
+        gmres_solve(c, b, x, rtol, maxit, st);
+        return;
+    }
     if (mixed_values(c))
         pcg_solve_mixed(c, b, x, rtol, maxit, st);
     else

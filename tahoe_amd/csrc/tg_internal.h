@@ -115,6 +115,15 @@ struct Amg {
     ~Amg() { clear(); }
 };
 
+// GMRES work space (k_gmres.hip)
+struct Gmres {
+    int m = 0;
+    long n3 = 0;
+    DBuf<double> V;                          // [m+1][3 nnd] Krylov basis
+    DBuf<double> w, q;                       // [3 nnd]
+    DBuf<double> part, h;                    // reduction partials, projections
+};
+
 // device scalar slots of an iteration (per segment for the PCG)
 enum { S_RZ = 0, S_PQ, S_ALPHA, S_RZNEW, S_RR, S_BETA, S_BB, S_DONE, S_ITER, S_THRESH, S_MAXIT, S_DOT, S_NSCAL };
 
@@ -175,6 +184,8 @@ struct tg_ctx {
     int stiff_variant = 0;              // block shape of the stiffness kernel
     int newton_assembly = 0;            // 0: Newton's K into the MSR then expanded, 1: straight into the operator
     int precond = TG_PC_JACOBI;         // preconditioner of the inner PCG
+    int linear_solver = TG_LIN_PCG;     // inner linear solver of the Newton loop
+    int gmres_restart = 30;
     int pcg_graph = 1;                  // replay PCG chunks as a captured HIP graph
     int spmv_xcd = 0;                   // XCD-contiguous row ranges in the PCG SpMV
     struct {
@@ -217,6 +228,7 @@ struct tg_ctx {
     tg::Timing timing;
     tg::Segments seg;
     tg::Amg amg;                        // aggregation multigrid (precond == TG_PC_AMG)
+    tg::Gmres gm;                       // GMRES basis (linear_solver == TG_LIN_GMRES)
 };
 
 namespace tg {
@@ -251,6 +263,7 @@ template <class MT>
 void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, double* x, double* r, double* p,
              double* q, double rtol, int maxit, const int* d_active, int nactive, tg_lin_stats* st);
 void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out);
+void gmres_solve(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st);
 // aggregation multigrid (tg_amg.cpp: structure on the host; k_amg.hip: values and cycles)
 void amg_setup(tg_ctx* c);
 void amg_numeric(tg_ctx* c);

@@ -19,6 +19,8 @@ FEManagerHIP::FEManagerHIP(const InputDeck& deck, const Options& opt)
         fElements.SetMaterial((int)m, deck.materials[m].first, deck.materials[m].second);
     fElements.SetPrecision(opt.precision);
     fElements.SetMaterialModel(deck.material_model);
+    if (deck.solver == "newton_krylov_solver")
+        fCtx.Check(tg_set_linear_solver(fCtx.Handle(), TG_LIN_GMRES, deck.gmres_restart), caller);
     // one schedule drives every scheduled card: value_k * schedule(t)
     std::vector<int> nodes, dofs;
     std::vector<double> vals;

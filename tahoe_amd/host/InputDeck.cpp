@@ -349,7 +349,7 @@ InputDeck ReadInputDeck(const std::string& xml_path)
     buf << in.rdbuf();
     auto root = ParseXML(buf.str(), xml_path);
     if (root->name != "tahoe") ExceptionT::BadInputValue(caller, "root element must be <tahoe>");
-    reject_unknown(*root, {"time", "nodes", "element_list", "nonlinear_solver"});
+    reject_unknown(*root, {"time", "nodes", "element_list", "nonlinear_solver", "newton_krylov_solver"});
 
     InputDeck d;
     d.xml_path = xml_path;
@@ -485,9 +485,21 @@ InputDeck ReadInputDeck(const std::string& xml_path)
     if (d.conn.empty()) ExceptionT::BadInputValue(caller, "the element group has no elements");
     d.material_model = model < 0 ? TG_MAT_SIMO_ISOTROPIC : model;
 
-    // ---- nonlinear solver (NLSolver) ----
+    // ---- nonlinear solver (NLSolver, or NLSolver_NK: Newton with GMRES) ----
     const XMLNode* ns = root->Child("nonlinear_solver");
-    if (!ns) ExceptionT::BadInputValue(caller, "only <nonlinear_solver> is on the path");
+    const XMLNode* nk = root->Child("newton_krylov_solver");
+    if (ns && nk) ExceptionT::BadInputValue(caller, "one solver per deck");
+    if (nk) {
+        // NLSolver_NK parameters (NLSolver_NK.cpp:17-63): gmres_restart 30,
+        // linear_tolerance 1e-3, max_linear_iter 0 (= 3 n)
+        d.solver = "newton_krylov_solver";
+        d.gmres_restart = nk->Int("gmres_restart", 30);
+        if (d.gmres_restart < 1) ExceptionT::BadInputValue(caller, "gmres_restart must be >= 1");
+        d.linear_tolerance = nk->Real("linear_tolerance", 1.0e-3);
+        d.max_linear_iterations = nk->Int("max_linear_iter", 0);
+        ns = nk;
+    }
+    if (!ns) ExceptionT::BadInputValue(caller, "only <nonlinear_solver> and <newton_krylov_solver> are on the path");
     d.rel_tolerance = ns->Real("rel_tolerance", d.rel_tolerance);
     d.abs_tolerance = ns->Real("abs_tolerance", d.abs_tolerance);
     d.divergence_tolerance = ns->Real("divergence_tolerance", d.divergence_tolerance);
@@ -496,7 +508,7 @@ InputDeck ReadInputDeck(const std::string& xml_path)
     if (ns->children.size() == 1) {
         const XMLNode& m = *ns->children[0];
         d.matrix = m.name;
-        if (m.name == "HIP_MSR_matrix") {
+        if (m.name == "HIP_MSR_matrix" && !nk) {
             d.linear_tolerance = m.Real("linear_tolerance", d.linear_tolerance);
             d.max_linear_iterations = m.Int("max_linear_iterations", d.max_linear_iterations);
         }

@@ -20,6 +20,8 @@
 //                 <E_and_nu Young_modulus Poisson_ratio/>
 //     <nonlinear_solver rel_tolerance abs_tolerance divergence_tolerance
 //                       max_iterations [min_iterations]>
+//       or <newton_krylov_solver ... [gmres_restart] [linear_tolerance]
+//                                    [max_linear_iter]>       (NLSolver_NK)
 //         <any matrix: SPOOLES_matrix, profile_matrix, ...,
 //          HIP_MSR_matrix [linear_tolerance] [max_linear_iterations]>
 // Anything else on the path (other elements, materials, BC controllers,
@@ -102,6 +104,8 @@ struct InputDeck {
     double linear_tolerance = 1e-10;
     int max_linear_iterations = 100000;
     std::string matrix;                  // matrix element named in the deck
+    std::string solver = "nonlinear_solver";  // or "newton_krylov_solver" (GMRES inner solve)
+    int gmres_restart = 30;
 };
 
 // reads the XML and the geometry it names (relative to the XML's folder)

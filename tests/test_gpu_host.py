@@ -71,3 +71,22 @@ def test_reference_beam_decks(tmp_path, deck, golden):
     assert [t for t, _ in got] == list(ref["time"])
     for (t, u), ur in zip(got, ref["u"]):
         assert np.abs(u - ur).max() <= 1e-10 * np.abs(ur).max(), t
+
+
+@pytest.mark.parametrize("lin_tol", ["1.0e-3", "1.0e-10"])
+def test_newton_krylov_deck(tmp_path, lin_tol):
+    """A fixture deck with <newton_krylov_solver gmres_restart linear_tolerance
+    max_linear_iter="0"> (NLSolver_NK): Newton with the device GMRES reaches
+    the reference's converged displacements."""
+    g = load_golden("tl_n8")
+    xml = write_fixture_deck(g, str(tmp_path))
+    s = open(xml).read()
+    s = s.replace('<nonlinear_solver ', f'<newton_krylov_solver gmres_restart="30" linear_tolerance="{lin_tol}" '
+                  'max_linear_iter="0" ').replace("</nonlinear_solver>", "</newton_krylov_solver>")
+    open(xml, "w").write(s)
+    out = str(tmp_path / "nk.run")
+    r = run(["-f", xml, "-o", out, "-digits", "16"], timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = parse_run(out)
+    for k, (t, u) in enumerate(got):
+        assert rel_err(u, g["step_u"][k]) <= 1e-9, k

@@ -73,3 +73,17 @@ def test_geometry_errors(tmp_path):
     open(geom, "w").write(s.replace("*nodes\n27\n", "*nodes\n26\n"))
     r = run(["-f", xml, "-check"])
     assert r.returncode == 9 and "*nodes header" in r.stderr, r.stderr
+
+
+def test_newton_krylov_deck_parses(tmp_path):
+    g = load_golden("tl_n2")
+    xml = write_fixture_deck(g, str(tmp_path))
+    s = open(xml).read().replace("<nonlinear_solver ", '<newton_krylov_solver gmres_restart="20" ')
+    s = s.replace("</nonlinear_solver>", "</newton_krylov_solver>")
+    open(xml, "w").write(s)
+    r = run(["-f", xml, "-check"])
+    assert r.returncode == 0, r.stderr
+    s2 = s.replace('gmres_restart="20"', 'gmres_restart="0"')
+    open(xml, "w").write(s2)
+    r = run(["-f", xml, "-check"])
+    assert r.returncode == 6 and "gmres_restart" in r.stderr

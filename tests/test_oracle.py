@@ -368,3 +368,55 @@ def test_beam_stvenant_force_matches_reference_run():
     for name in ("beam_newton_totlag", "beam_newton"):
         ref = np.load(os.path.join(GOLDEN, name + "_run.npz"))["u"][0]
         assert np.abs(u - ref).max() <= 1e-10 * np.abs(ref).max(), name
+
+
+def test_gmres_reference_unit_cases():
+    """The oracle's NLSolver_NK::GMRES restatement on the reference's own
+    GMRES unit cases (tests/solvers/test_GMRES.cpp:164-240: SPD 2x2,
+    diagonal 3x3, non-symmetric 5x5, restart m = 2 on a 4x4, zero right-hand
+    side), with the tolerances those tests assert."""
+    x, ok, _ = O.gmres_dense([[4.0, 1.0], [1.0, 3.0]], [1.0, 2.0], 10, 1e-10)
+    assert ok and abs(x[0] - 1 / 11) <= 1e-10 and abs(x[1] - 7 / 11) <= 1e-10
+    x, ok, steps = O.gmres_dense(np.diag([2.0, 3.0, 5.0]), [4.0, 9.0, 5.0], 10, 1e-12)
+    assert ok and np.abs(x - [2.0, 3.0, 1.0]).max() <= 1e-12
+    assert steps == 0  # left Jacobi turns a diagonal system into the identity: converged in the first step
+    A5 = np.array([[10.0, -1.0, 2.0, 0.0, 0.0], [-1.0, 11.0, -1.0, 3.0, 0.0], [2.0, -1.0, 10.0, -1.0, 0.0],
+                   [0.0, 3.0, -1.0, 8.0, -2.0], [0.0, 0.0, 0.0, -2.0, 6.0]])
+    xe = np.arange(1.0, 6.0)
+    x, ok, _ = O.gmres_dense(A5, A5 @ xe, 10, 1e-10)
+    assert ok and np.abs(x - xe).max() <= 1e-8
+    A4 = np.array([[4.0, 1.0, 0.0, 0.0], [1.0, 4.0, 1.0, 0.0], [0.0, 1.0, 4.0, 1.0], [0.0, 0.0, 1.0, 4.0]])
+    xe = np.array([1.0, -1.0, 2.0, -2.0])
+    # the unit case's own loop allows 10 n Arnoldi steps (NLSolver_NK's
+    # max_linear_iter); with the default 3 n this system stops at 9e-8
+    x, ok, steps = O.gmres_dense(A4, A4 @ xe, 2, 1e-10, maxit=40)
+    assert ok and np.abs(x - xe).max() <= 1e-8 and steps > 2  # several restart cycles
+    x, ok, _ = O.gmres_dense([[3.0, 1.0], [1.0, 2.0]], [0.0, 0.0], 5, 1e-12)
+    assert ok and np.abs(x).max() == 0.0
+
+
+def test_gmres_full_space_and_fe_system():
+    """GMRES(m >= n) is exact after n Arnoldi steps (Krylov space = whole
+    space, up to round-off); on an assembled FE stiffness (tl_n4 golden, MSR)
+    its solution satisfies the relative residual it was asked for."""
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((7, 7)) + 7 * np.eye(7)
+    b = rng.standard_normal(7)
+    x, ok, steps = O.gmres_dense(A, b, 7, 1e-14)
+    assert np.abs(A @ x - b).max() <= 1e-12 * np.abs(b).max() and steps <= 7
+    g = load_golden("tl_n4")
+    n = int(g["eqnos"].max())
+    b_ = O.msr_pattern(n, g["conn"], g["eqnos"])
+    mu, ka = O.mu_kappa(1000.0, 0.3)
+    val = O.stiffness(O.TL, g["X"], g["conn"], g["mat"], mu, ka, g["step_u"][0], g["eqnos"], n, b_)
+    rhs = rng.standard_normal(n)
+    # each restart cycle must cut its own starting residual by tol, so once
+    # the residual is at round-off the cycles run to the 3 n cap and report
+    # no convergence (as NLSolver_NK does, which then goes on with Newton)
+    x, ok, steps = O.gmres_msr(b_, val, rhs, 30, 1e-12)
+    r = rhs - O.msr_multx(b_, val, x)
+    d = val[:n]
+    assert np.linalg.norm(r / d) <= 1e-13 * np.linalg.norm(rhs / d) and steps == 3 * n
+    x, ok, steps = O.gmres_msr(b_, val, rhs, 30, 1e-3)
+    r = rhs - O.msr_multx(b_, val, x)
+    assert ok and steps < 3 * n and np.linalg.norm(r / d) <= 1e-3 * np.linalg.norm(rhs / d)
