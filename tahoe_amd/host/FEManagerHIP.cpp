@@ -57,6 +57,13 @@ const std::vector<StepRecord>& FEManagerHIP::Solve(std::ostream* log)
     p.divergence_tolerance = fDeck.divergence_tolerance;
     p.linear_tolerance = fOpt.linear_tolerance > 0.0 ? fOpt.linear_tolerance : fDeck.linear_tolerance;
     p.max_linear_iterations = fDeck.max_linear_iterations;
+    if (fDeck.solver == "PCG_solver") {
+        p.method = NLSolverHIP::kNonlinearCG;
+        p.restart = fDeck.cg_restart;
+        p.search_iterations = fDeck.cg_search_iterations;
+        p.orthog_tolerance = fDeck.cg_orthog_tolerance;
+        p.max_step = fDeck.cg_max_step;
+    }
     NLSolverHIP solver(fElements, fLHS, p, fOpt.driver);
     fSteps.clear();
     for (int n = 1; n <= fDeck.num_steps; n++) {

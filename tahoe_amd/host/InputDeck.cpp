@@ -349,7 +349,7 @@ InputDeck ReadInputDeck(const std::string& xml_path)
     buf << in.rdbuf();
     auto root = ParseXML(buf.str(), xml_path);
     if (root->name != "tahoe") ExceptionT::BadInputValue(caller, "root element must be <tahoe>");
-    reject_unknown(*root, {"time", "nodes", "element_list", "nonlinear_solver", "newton_krylov_solver"});
+    reject_unknown(*root, {"time", "nodes", "element_list", "nonlinear_solver", "newton_krylov_solver", "PCG_solver"});
 
     InputDeck d;
     d.xml_path = xml_path;
@@ -488,7 +488,19 @@ InputDeck ReadInputDeck(const std::string& xml_path)
     // ---- nonlinear solver (NLSolver, or NLSolver_NK: Newton with GMRES) ----
     const XMLNode* ns = root->Child("nonlinear_solver");
     const XMLNode* nk = root->Child("newton_krylov_solver");
-    if (ns && nk) ExceptionT::BadInputValue(caller, "one solver per deck");
+    const XMLNode* cg = root->Child("PCG_solver");
+    if ((ns != nullptr) + (nk != nullptr) + (cg != nullptr) > 1) ExceptionT::BadInputValue(caller, "one solver per deck");
+    if (cg) {
+        // PCGSolver_LS parameters (PCGSolver_LS.cpp:48-98)
+        d.solver = "PCG_solver";
+        d.cg_restart = cg->Int("restart", 0);
+        d.cg_search_iterations = cg->Int("line_search_iterations", 3);
+        d.cg_orthog_tolerance = cg->Real("line_search_tolerance", 0.25);
+        d.cg_max_step = cg->Real("max_step", 2.5);
+        if (d.cg_restart < 0 || d.cg_search_iterations < 0)
+            ExceptionT::BadInputValue(caller, "PCG_solver restart and line_search_iterations must be >= 0");
+        ns = cg;
+    }
     if (nk) {
         // NLSolver_NK parameters (NLSolver_NK.cpp:17-63): gmres_restart 30,
         // linear_tolerance 1e-3, max_linear_iter 0 (= 3 n)
@@ -499,7 +511,9 @@ InputDeck ReadInputDeck(const std::string& xml_path)
         d.max_linear_iterations = nk->Int("max_linear_iter", 0);
         ns = nk;
     }
-    if (!ns) ExceptionT::BadInputValue(caller, "only <nonlinear_solver> and <newton_krylov_solver> are on the path");
+    if (!ns)
+        ExceptionT::BadInputValue(caller,
+                                  "only <nonlinear_solver>, <newton_krylov_solver> and <PCG_solver> are on the path");
     d.rel_tolerance = ns->Real("rel_tolerance", d.rel_tolerance);
     d.abs_tolerance = ns->Real("abs_tolerance", d.abs_tolerance);
     d.divergence_tolerance = ns->Real("divergence_tolerance", d.divergence_tolerance);

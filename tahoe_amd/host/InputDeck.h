@@ -22,6 +22,8 @@
 //                       max_iterations [min_iterations]>
 //       or <newton_krylov_solver ... [gmres_restart] [linear_tolerance]
 //                                    [max_linear_iter]>       (NLSolver_NK)
+//       or <PCG_solver ... restart [line_search_iterations]
+//                          [line_search_tolerance] [max_step]> (PCGSolver_LS)
 //         <any matrix: SPOOLES_matrix, profile_matrix, ...,
 //          HIP_MSR_matrix [linear_tolerance] [max_linear_iterations]>
 // Anything else on the path (other elements, materials, BC controllers,
@@ -104,8 +106,11 @@ struct InputDeck {
     double linear_tolerance = 1e-10;
     int max_linear_iterations = 100000;
     std::string matrix;                  // matrix element named in the deck
-    std::string solver = "nonlinear_solver";  // or "newton_krylov_solver" (GMRES inner solve)
+    std::string solver = "nonlinear_solver";  // or "newton_krylov_solver" (GMRES inner solve), "PCG_solver"
     int gmres_restart = 30;
+    // PCG_solver (nonlinear CG with line search)
+    int cg_restart = 0, cg_search_iterations = 3;
+    double cg_orthog_tolerance = 0.25, cg_max_step = 2.5;
 };
 
 // reads the XML and the geometry it names (relative to the XML's folder)

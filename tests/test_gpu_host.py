@@ -58,10 +58,12 @@ def test_fixture_deck(tmp_path, name, args):
 
 
 @pytest.mark.parametrize("deck,golden", [("beam.Newton.xml", "beam_newton_run.npz"),
-                                         ("beam.Newton.TotLag.xml", "beam_newton_totlag_run.npz")])
+                                         ("beam.Newton.TotLag.xml", "beam_newton_totlag_run.npz"),
+                                         ("beam.PCG.xml", "beam_pcg_run.npz")])
 def test_reference_beam_decks(tmp_path, deck, golden):
     """The reference's own beam decks (updated_lagrangian / total_lagrangian,
-    large_strain_StVenant, force_BC, nonlinear_solver with profile_matrix)
+    large_strain_StVenant, force_BC; nonlinear_solver with profile_matrix, or
+    PCG_solver -- nonlinear CG with line search -- with diagonal_matrix)
     against the reference's committed output (13 significant digits)."""
     out = str(tmp_path / "beam.run")
     r = run(["-f", os.path.join(DECKS, deck), "-o", out, "-linear_tolerance", "1e-13"])

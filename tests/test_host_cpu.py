@@ -87,3 +87,12 @@ def test_newton_krylov_deck_parses(tmp_path):
     open(xml, "w").write(s2)
     r = run(["-f", xml, "-check"])
     assert r.returncode == 6 and "gmres_restart" in r.stderr
+
+
+def test_reference_beam_pcg_deck_parses():
+    """beam.PCG: the reference's PCG_solver (nonlinear CG, line search) with a
+    diagonal_matrix."""
+    r = run(["-f", os.path.join(DECKS, "beam.PCG.xml"), "-check"])
+    assert r.returncode == 0, r.stderr
+    assert "updated_lagrangian, 12 nodes, 2 elements, 1 materials, 12 kinematic cards (0 scheduled), 1 force cards, " \
+        "large_strain_StVenant, 1 steps, matrix diagonal_matrix" in r.stdout
