@@ -9,7 +9,9 @@ namespace tahoe_hip {
 HIPMSRMatrixT::HIPMSRMatrixT(HIPContext& ctx, double rtol, int maxit, int check_code)
     : fCtx(ctx), fRTol(rtol), fMaxIt(maxit), fCheckCode(check_code)
 {
-    if (!(rtol > 0.0) || maxit <= 0) ExceptionT::BadInputValue("HIPMSRMatrixT", "linear tolerance and iteration limit");
+    // maxit 0: the linear solver's own limit (3 n_eq Arnoldi steps for GMRES,
+    // NLSolver_NK's max_linear_iter = 0)
+    if (!(rtol > 0.0) || maxit < 0) ExceptionT::BadInputValue("HIPMSRMatrixT", "linear tolerance and iteration limit");
 }
 
 void HIPMSRMatrixT::Initialize(int tot_num_eq, int loc_num_eq, int start_eq)
