@@ -125,6 +125,7 @@ static void setup_segments(tg_ctx* c)
     S.d_part.alloc(2 * (size_t)S.nslices);
     S.d_out.alloc((size_t)S.nseg);
     S.d_ndone.alloc(1);
+    S.d_cnt.alloc((size_t)S.nseg);
     std::vector<int> ones((size_t)S.nseg, 1);
     auto up = [&](int* dst, const std::vector<int>& v) {
         TG_HIP(hipMemcpyAsync(dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, c->stream));
@@ -134,6 +135,7 @@ static void setup_segments(tg_ctx* c)
     up(S.d_active.p, ones);
     up(S.d_elem_seg.p, S.elem_seg);
     TG_HIP(hipMemsetAsync(S.d_scal.p, 0, sizeof(double) * S_NSCAL * S.nseg, c->stream));
+    TG_HIP(hipMemsetAsync(S.d_cnt.p, 0, sizeof(int) * S.nseg, c->stream));
     // every copy and clear of the context goes through its own stream (a
     // null-stream operation is not ordered with a non-blocking stream), and
     // the host vectors copied from must outlive the copies
@@ -925,6 +927,9 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
         } else if (knob == TG_TUNE_SPMV_XCD) {
             if (value < 0 || value > 3) fail(TG_OUT_OF_RANGE, "SpMV variant must be 0..3");
             c->spmv_xcd = (int)value;
+        } else if (knob == TG_TUNE_PCG_FUSED) {
+            if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "PCG fused must be 0 or 1");
+            c->pcg_fused = (int)value;
         } else if (knob == TG_TUNE_AMG_DEGREE) {
             if (value < 2 || value > 4) fail(TG_OUT_OF_RANGE, "multigrid smoother degree must be 2..4");
             c->amg.degree = (int)value;
@@ -952,6 +957,8 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
             *value = c->pcg_graph;
         else if (knob == TG_TUNE_SPMV_XCD)
             *value = c->spmv_xcd;
+        else if (knob == TG_TUNE_PCG_FUSED)
+            *value = c->pcg_fused;
         else if (knob == TG_TUNE_AMG_DEGREE)
             *value = c->amg.degree;
         else if (knob == TG_TUNE_AMG_WEIGHT)

@@ -143,6 +143,7 @@ struct Segments {
     DBuf<double> d_part;                  // [2][nslices]
     DBuf<double> d_out;                   // [nseg]
     DBuf<int> d_ndone;
+    DBuf<int> d_cnt;                      // [nseg] slices that finished a fused pass (reset by the last one)
     std::vector<double> h_scal;
 };
 
@@ -188,6 +189,8 @@ struct tg_ctx {
     int gmres_restart = 30;
     int pcg_graph = 1;                  // replay PCG chunks as a captured HIP graph
     int spmv_xcd = 0;                   // XCD-contiguous row ranges in the PCG SpMV
+    int pcg_fused = 1;                  // two kernels per PCG iteration (direction and reductions fused)
+    tg::DBuf<double> vp2;               // second direction buffer of the fused iteration
     struct {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
