@@ -63,26 +63,39 @@ struct Op0 {
     }
 };
 
-// Levels >= 1: block CSR of 6x6 FP32 blocks, one thread per dof.
+// Levels >= 1: sliced 6x6 FP32 blocks (slot e of a slice: its 36 values at
+// val[(e / 64) * 2304 + comp * 64 + e % 64], coalesced over the slice's 64
+// rows), one thread per node and 6 outputs.
 struct OpC {
-    static constexpr int NC = 1;
+    static constexpr int NC = 6;
     int n;
-    const int* rowptr;
+    const int* sbase;
+    const int* swidth;
     const int* col;
     const float* val;
-    __host__ __device__ long units() const { return 6L * n; }
-    __device__ __forceinline__ void apply(long t, const double* __restrict__ x, double (&y)[1], long (&k)[1]) const
+    __host__ __device__ long units() const { return n; }
+    __device__ __forceinline__ void apply(long t, const double* __restrict__ x, double (&y)[6], long (&k)[6]) const
     {
-        const int a = (int)(t / 6), r = (int)(t - 6L * a);
-        double s = 0.0;
-        for (int q = rowptr[a]; q < rowptr[a + 1]; q++) {
-            const float* v = val + (long)q * 36 + 6 * r;
-            const double* xc = x + 6L * col[q];
-            s += (double)v[0] * xc[0] + (double)v[1] * xc[1] + (double)v[2] * xc[2] + (double)v[3] * xc[3] +
-                 (double)v[4] * xc[4] + (double)v[5] * xc[5];
+        const int s = (int)(t >> 6), l = (int)(t & 63);
+        const long b0 = sbase[s];
+        const int w = swidth[s];
+#pragma unroll
+        for (int r = 0; r < 6; r++) y[r] = 0.0;
+        for (int q = 0; q < w; q++) {
+            const long slot = (b0 + q) * 64 + l;
+            const int c = col[slot];
+            if (c < 0) continue;
+            double xc[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++) xc[j] = x[6L * c + j];
+            const float* v = val + (b0 + q) * 2304 + l;
+#pragma unroll
+            for (int r = 0; r < 6; r++)
+#pragma unroll
+                for (int j = 0; j < 6; j++) y[r] += (double)v[(6 * r + j) * 64] * xc[j];
         }
-        y[0] = s;
-        k[0] = t;
+#pragma unroll
+        for (int r = 0; r < 6; r++) k[r] = 6 * t + r;
     }
 };
 
@@ -258,11 +271,12 @@ __global__ void __launch_bounds__(AB) k_amg_rap0(long nup, const int* __restrict
         }
         s += t;
     }
-    cval[(long)up[2 * u] * 36 + 6 * r + c] = (float)s;
-    if (up[2 * u + 1] >= 0) cval[(long)up[2 * u + 1] * 36 + 6 * c + r] = (float)s;
+    const int ab = up[2 * u], ba = up[2 * u + 1];
+    cval[(long)(ab >> 6) * 2304 + (ab & 63) + (6 * r + c) * 64] = (float)s;
+    if (ba >= 0) cval[(long)(ba >> 6) * 2304 + (ba & 63) + (6 * c + r) * 64] = (float)s;
 }
 
-// Galerkin product, fine level = a 6x6 block CSR level
+// Galerkin product, fine level = a sliced 6x6 level
 __global__ void __launch_bounds__(AB) k_amg_rapc(long nup, const int* __restrict__ up, const long* __restrict__ gptr,
                                                  const int* __restrict__ gl, const int* __restrict__ fcol,
                                                  const float* __restrict__ fval, const float* __restrict__ P,
@@ -276,7 +290,7 @@ __global__ void __launch_bounds__(AB) k_amg_rapc(long nup, const int* __restrict
     for (long e = gptr[u]; e < gptr[u + 1]; e++) {
         const int fb = gl[2 * e], i = gl[2 * e + 1];
         const int j = fcol[fb];
-        const float* v = fval + (long)fb * 36;
+        const float* v = fval + (long)(fb >> 6) * 2304 + (fb & 63);
         const float* pi = P + (long)i * 36 + r;
         const float* pj = P + (long)j * 36 + c;
         double t = 0.0;
@@ -284,23 +298,25 @@ __global__ void __launch_bounds__(AB) k_amg_rapc(long nup, const int* __restrict
         for (int p = 0; p < 6; p++) {
             double ap = 0.0;
 #pragma unroll
-            for (int q = 0; q < 6; q++) ap += (double)v[6 * p + q] * pj[6 * q];
+            for (int q = 0; q < 6; q++) ap += (double)v[(6 * p + q) * 64] * pj[6 * q];
             t += (double)pi[6 * p] * ap;
         }
         s += t;
     }
-    cval[(long)up[2 * u] * 36 + 6 * r + c] = (float)s;
-    if (up[2 * u + 1] >= 0) cval[(long)up[2 * u + 1] * 36 + 6 * c + r] = (float)s;
+    const int ab = up[2 * u], ba = up[2 * u + 1];
+    cval[(long)(ab >> 6) * 2304 + (ab & 63) + (6 * r + c) * 64] = (float)s;
+    if (ba >= 0) cval[(long)(ba >> 6) * 2304 + (ba & 63) + (6 * c + r) * 64] = (float)s;
 }
 
-// inverse diagonal of a 6x6 block CSR level (0 for inactive dofs)
+// inverse diagonal of a sliced 6x6 level (0 for inactive dofs)
 __global__ void __launch_bounds__(AB) k_amg_dinv(int n, const int* __restrict__ diagblk, const float* __restrict__ val,
                                                  double* __restrict__ dinv)
 {
     const long t = (long)blockIdx.x * AB + threadIdx.x;
     if (t >= 6L * n) return;
     const int a = (int)(t / 6), r = (int)(t - 6L * a);
-    const double d = (double)val[(long)diagblk[a] * 36 + 7 * r];
+    const int e = diagblk[a];
+    const double d = (double)val[(long)(e >> 6) * 2304 + (e & 63) + 7 * r * 64];
     dinv[t] = d > 0.0 ? 1.0 / d : 0.0;
 }
 
@@ -391,7 +407,7 @@ static Op0<MT> op0(tg_ctx* c, const MT* val)
     return Op0<MT>{c->nnd, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val};
 }
 
-static OpC opc(const AmgLevel& L) { return OpC{L.n, L.d_rowptr.p, L.d_col.p, L.d_val.p}; }
+static OpC opc(const AmgLevel& L) { return OpC{L.n, L.d_sbase.p, L.d_swidth.p, L.d_col.p, L.d_val.p}; }
 
 static double* level_dinv(tg_ctx* c, int l) { return l == 0 ? c->vdinv.p : c->amg.lv[l]->dinv; }
 
@@ -432,9 +448,10 @@ static void coarse_inverse(const AmgLevel& L, const std::vector<float>& val, std
     std::vector<double> A((size_t)n * n, 0.0);
     for (int a = 0; a < L.n; a++)
         for (int q = L.rowptr[a]; q < L.rowptr[a + 1]; q++) {
-            const int b = L.col[q];
+            const int b = L.col[q], e = L.slot[q];
+            const size_t v0 = (size_t)(e >> 6) * 2304 + (e & 63);
             for (int r = 0; r < 6; r++)
-                for (int s = 0; s < 6; s++) A[(size_t)(6 * a + r) * n + 6 * b + s] = val[(size_t)q * 36 + 6 * r + s];
+                for (int s = 0; s < 6; s++) A[(size_t)(6 * a + r) * n + 6 * b + s] = val[v0 + (6 * r + s) * 64];
         }
     double dmax = 0.0;
     for (int i = 0; i < n; i++) dmax = std::max(dmax, A[(size_t)i * n + i]);
@@ -521,7 +538,7 @@ static void amg_numeric_t(tg_ctx* c, const MT* val0)
     std::vector<double> red(16 * (size_t)L);
     TG_HIP(hipMemcpyAsync(red.data(), G.d_red.p, sizeof(double) * red.size(), hipMemcpyDeviceToHost, c->stream));
     const AmgLevel& Cst = *G.lv[L - 1];
-    std::vector<float> cval(Cst.col.size() * 36);
+    std::vector<float> cval((size_t)Cst.nslots * 36);
     TG_HIP(hipMemcpyAsync(cval.data(), Cst.d_val.p, sizeof(float) * cval.size(), hipMemcpyDeviceToHost, c->stream));
     TG_HIP(hipStreamSynchronize(c->stream));
     std::vector<double> coef(8 * (size_t)L, 0.0);

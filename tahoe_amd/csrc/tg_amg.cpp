@@ -259,10 +259,31 @@ void amg_setup(tg_ctx* c)
         if (gc.adj.size() > 0x7fffffffL / 36) fail(TG_OUT_OF_RANGE, "multigrid level exceeds 32-bit indexing");
         nx->rowptr.assign(gc.ptr.begin(), gc.ptr.end());
         nx->col = gc.adj;
+        // sliced layout of the next level
+        {
+            const int ns = (nagg + 63) / 64;
+            nx->sbase.assign((size_t)ns, 0);
+            nx->swidth.assign((size_t)ns, 0);
+            long base = 0;
+            for (int sl = 0; sl < ns; sl++) {
+                int w = 0;
+                for (int a = 64 * sl; a < std::min(nagg, 64 * sl + 64); a++)
+                    w = std::max(w, (int)(gc.ptr[a + 1] - gc.ptr[a]));
+                nx->sbase[sl] = (int)base;
+                nx->swidth[sl] = w;
+                base += w;
+            }
+            nx->nslots = base * 64;
+            if (nx->nslots > 0x7fffffffL / 36) fail(TG_OUT_OF_RANGE, "multigrid level exceeds 32-bit indexing");
+            nx->slot.assign(gc.adj.size(), 0);
+            for (int a = 0; a < nagg; a++)
+                for (long k = gc.ptr[a]; k < gc.ptr[a + 1]; k++)
+                    nx->slot[k] = (int)(((long)nx->sbase[a >> 6] + (k - gc.ptr[a])) * 64 + (a & 63));
+        }
         std::vector<int> diag((size_t)nagg);
         for (int a = 0; a < nagg; a++)
-            diag[a] = (int)(std::lower_bound(gc.adj.begin() + gc.ptr[a], gc.adj.begin() + gc.ptr[a + 1], a) -
-                            gc.adj.begin());
+            diag[a] = nx->slot[std::lower_bound(gc.adj.begin() + gc.ptr[a], gc.adj.begin() + gc.ptr[a + 1], a) -
+                               gc.adj.begin()];
         // Galerkin gather lists: upper coarse block (A, B), B >= A, gets
         // P_i^T A_ij P_j over the fine blocks (i, j) with i in A (ascending)
         // and j in B (in the fine row's block order)
@@ -285,8 +306,9 @@ void amg_setup(tg_ctx* c)
                         j = g.adj[g.ptr[i] + k];
                         fb = (int)(((long)c->op.slice_base[i >> 6] + k) * 64 + (i & 63));
                     } else {
-                        fb = (int)(cur->rowptr[i] + k);
-                        j = cur->col[fb];
+                        const int q = (int)(cur->rowptr[i] + k);
+                        j = cur->col[q];
+                        fb = cur->slot[q];
                     }
                     const int b = agg[j];
                     if (b < a) continue;
@@ -302,16 +324,16 @@ void amg_setup(tg_ctx* c)
                 if (b != a)
                     ba = (int)(std::lower_bound(gc.adj.begin() + gc.ptr[b], gc.adj.begin() + gc.ptr[b + 1], a) -
                                gc.adj.begin());
-                up.push_back(ab);
-                up.push_back(ba);
+                up.push_back(nx->slot[ab]);
+                up.push_back(ba < 0 ? -1 : nx->slot[ba]);
                 gl.insert(gl.end(), bucket[k].begin(), bucket[k].end());
                 gp.push_back((long)gl.size() / 2);
             }
         }
         // every index the device kernels will follow, checked here
         {
-            const long fine_blocks = level == 0 ? c->op.nblocks_padded : (long)cur->col.size();
-            const long coarse_blocks = (long)gc.adj.size();
+            const long fine_blocks = level == 0 ? c->op.nblocks_padded : cur->nslots;
+            const long coarse_blocks = nx->nslots;
             for (size_t k = 0; k < gl.size(); k += 2)
                 if (gl[k] < 0 || gl[k] >= fine_blocks || gl[k + 1] < 0 || gl[k + 1] >= n)
                     fail(TG_OUT_OF_RANGE, "multigrid gather list out of range");
@@ -334,10 +356,16 @@ void amg_setup(tg_ctx* c)
         upload(c, cur->d_up, up);
         upload(c, cur->d_gptr, gp);
         upload(c, cur->d_glist, gl);
-        upload(c, nx->d_rowptr, nx->rowptr);
-        upload(c, nx->d_col, nx->col);
+        {
+            std::vector<int> col_slot((size_t)nx->nslots, -1);
+            for (size_t q = 0; q < nx->col.size(); q++) col_slot[(size_t)nx->slot[q]] = nx->col[q];
+            upload(c, nx->d_col, col_slot);
+            upload(c, nx->d_sbase, nx->sbase);
+            upload(c, nx->d_swidth, nx->swidth);
+            TG_HIP(hipStreamSynchronize(c->stream));  // col_slot outlives its copy
+        }
         upload(c, nx->d_diagblk, diag);
-        nx->d_val.alloc(nx->col.size() * 36);
+        nx->d_val.alloc((size_t)nx->nslots * 36);
         nx->d_dinv_own.alloc(6 * (size_t)nagg);
         nx->dinv = nx->d_dinv_own.p;
         for (auto* v : {&nx->d_f, &nx->d_x, &nx->d_r, &nx->d_d0, &nx->d_d1}) v->alloc(6 * (size_t)nagg);

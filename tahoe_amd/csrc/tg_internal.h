@@ -79,8 +79,14 @@ struct Timing {
 struct AmgLevel {
     int n = 0, bs = 3;                       // nodes, dofs per node
     std::vector<int> rowptr, col;            // pattern (levels >= 1; sorted, diagonal included)
-    DBuf<int> d_rowptr, d_col, d_diagblk;
-    DBuf<float> d_val;                       // [nnz blocks][36]
+    // levels >= 1 are stored sliced like the fine operator: slice s holds
+    // block rows [64 s, 64 s + 64); entry k of row a is slot
+    // (sbase[a / 64] + k) * 64 + a % 64; the 36 values of slot e (row-major
+    // inside the block) are val[(e / 64) * 2304 + comp * 64 + e % 64]
+    std::vector<int> sbase, swidth, slot;    // slot: CSR entry -> slot
+    long nslots = 0;
+    DBuf<int> d_sbase, d_swidth, d_col, d_diagblk;  // d_col: column node per slot, -1 padding; d_diagblk: slot
+    DBuf<float> d_val;                       // [nslots][36], component-major per slice
     DBuf<double> d_dinv_own;                 // [n*bs] inverse diagonal (levels >= 1)
     double* dinv = nullptr;                  // level 0: the PCG's Jacobi vector
     DBuf<double> d_f, d_x, d_r, d_d0, d_d1;  // V-cycle vectors [n*bs]
@@ -90,9 +96,9 @@ struct AmgLevel {
     DBuf<int> d_mptr, d_mem;                 // members of each aggregate (CSR)
     DBuf<float> d_P;                         // [n][bs][6] tentative prolongator rows
     long nup = 0;                            // upper blocks (A <= B) of level + 1
-    DBuf<int> d_up;                          // [nup][2] block index of (A,B) and of (B,A) (-1 when A == B)
+    DBuf<int> d_up;                          // [nup][2] slot of (A,B) and of (B,A) (-1 when A == B) in level + 1
     DBuf<long> d_gptr;                       // [nup+1]
-    DBuf<int> d_glist;                       // [entries][2] (block index in this level, block row)
+    DBuf<int> d_glist;                       // [entries][2] (slot in this level, block row)
     double lmax = 0.0;
 };
 
