@@ -257,11 +257,11 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      percent, 100 .. 190 (default 150: unsmoothed
  *                      aggregation under-corrects; any weight in (0, 2)
  *                      keeps the V-cycle symmetric positive definite). */
-/* TG_TUNE_PCG_FUSED    1 (default): two kernels per PCG iteration -- the
- *                      direction formed inside the SpMV and the per-segment
- *                      reductions finished by each segment's last wave --
- *                      for the Jacobi and multigrid preconditioners with SpMV
- *                      variant 0; 0: the five-kernel iteration. */
+/* TG_TUNE_PCG_FUSED    1 (default): the direction update is formed inside
+ *                      the SpMV (four kernels per PCG iteration instead of
+ *                      five, no separate direction pass) for the Jacobi and
+ *                      multigrid preconditioners with SpMV variant 0;
+ *                      0: the five-kernel iteration. */
 enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3, TG_TUNE_PCG_GRAPH = 4,
        TG_TUNE_SPMV_XCD = 5, TG_TUNE_AMG_DEGREE = 6, TG_TUNE_AMG_WEIGHT = 7, TG_TUNE_PCG_FUSED = 8 };
 int tg_set_tuning(tg_ctx* ctx, int knob, long value);

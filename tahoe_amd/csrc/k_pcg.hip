@@ -305,16 +305,16 @@ __global__ void __launch_bounds__(PB) k_seg_direction(int nnd, const int* __rest
     }
 }
 
-// ---- fused iteration: two kernels per PCG iteration -------------------------
+// ---- fused iteration: four kernels per PCG iteration -----------------------
 //
-// Kernel A forms the direction p = z + beta p_old on the fly (for its own
-// rows, which it stores, and for every neighbour it gathers), q = A p and the
-// per-slice partials of p.q; kernel B updates x and r and the partials of
-// r.z and r.r.  The last wave of a segment to finish a pass (a per-segment
-// arrival counter, reset by that wave) sums the segment's partials in slice
-// order and does the scalar work of the separate finalize kernels (alpha; or
-// beta, the stopping test and the done flag), so the order of every sum --
-// and the result -- still depends only on the segment's own slices.
+// The direction update p = z + beta p_old is formed inside the SpMV -- for
+// the SpMV's own rows, which it stores, and for every neighbour it gathers --
+// so the separate direction pass (and its reads of r, z and p) disappears;
+// p is double-buffered (the SpMV reads p_old of its neighbours while it
+// writes p of its own rows).  The per-segment reductions stay in their own
+// finalize kernels: a kernel boundary publishes every workgroup's partials
+// at no cost, where a device-scope fence per wave would write back and
+// invalidate the XCD's L2 each time.
 
 // z at dof k: Jacobi (m = 1/diag) or multigrid (m = z)
 template <int PC>
@@ -323,46 +323,21 @@ __device__ __forceinline__ double zval(const double* __restrict__ m, const doubl
     return PC == 2 ? m[k] : m[k] * r[k];
 }
 
-// Arrival of one slice of segment g; true in every lane of the segment's last wave.
-__device__ __forceinline__ bool last_slice(int* __restrict__ cnt, const int* __restrict__ seg_slice, int g)
-{
-    int last = 0;
-    if ((threadIdx.x & 63) == 0) {
-        __threadfence();
-        last = atomicAdd(&cnt[g], 1) == seg_slice[g + 1] - seg_slice[g] - 1;
-    }
-    last = __shfl(last, 0);
-    if (last) __threadfence();
-    return last != 0;
-}
-
-// sum of part[sb..se) by one wave in a fixed order (lane l: slices sb+l,
-// sb+l+64, ...; then a butterfly); the partials come from other workgroups,
-// so they are read past the (non-coherent) first-level cache
-__device__ __forceinline__ double wave_total(const double* part, int sb, int se)
-{
-    const volatile double* v = part;
-    double t = 0.0;
-    for (int k = sb + (int)(threadIdx.x & 63); k < se; k += 64) t += v[k];
-    return wsum(t);
-}
-
 template <class MT, int PC>
 __global__ void __launch_bounds__(PB) k_fused_spmv(int nnd, const int* __restrict__ slice_seg,
-                                                   const int* __restrict__ seg_slice, const int* __restrict__ sbase,
-                                                   const int* __restrict__ swidth, const int* __restrict__ bcol,
-                                                   const MT* __restrict__ bval, const double* __restrict__ m,
-                                                   const double* __restrict__ r, const double* __restrict__ pold,
-                                                   double* __restrict__ pnew, double* __restrict__ y,
-                                                   double* __restrict__ part, long nslices, int nseg,
-                                                   double* __restrict__ sc, int* __restrict__ cnt)
+                                                   const int* __restrict__ sbase, const int* __restrict__ swidth,
+                                                   const int* __restrict__ bcol, const MT* __restrict__ bval,
+                                                   const double* __restrict__ m, const double* __restrict__ r,
+                                                   const double* __restrict__ pold, double* __restrict__ pnew,
+                                                   double* __restrict__ y, double* __restrict__ part, long nslices,
+                                                   int nseg, const double* __restrict__ sc)
 {
     const int a = blockIdx.x * PB + threadIdx.x;
     const int s = a >> 6, l = a & 63;
     if (a >= 64L * nslices) return;
     const int g = slice_seg[s];  // wave-uniform
-    if (*sf(sc, S_DONE, nseg, g) != 0.0) return;
-    const double beta = *sf(sc, S_BETA, nseg, g);
+    if (*sf((double*)sc, S_DONE, nseg, g) != 0.0) return;
+    const double beta = *sf((double*)sc, S_BETA, nseg, g);
     double dv = 0.0;
     if (a < nnd) {
         const long b0 = sbase[s];
@@ -393,104 +368,6 @@ __global__ void __launch_bounds__(PB) k_fused_spmv(int nnd, const int* __restric
     }
     dv = wsum(dv);
     if (l == 0) part[s] = dv;
-    if (last_slice(cnt, seg_slice, g)) {
-        const double pq = wave_total(part, seg_slice[g], seg_slice[g + 1]);
-        if (l == 0) {
-            cnt[g] = 0;
-            *sf(sc, S_PQ, nseg, g) = pq;
-            *sf(sc, S_ALPHA, nseg, g) = *sf(sc, S_RZ, nseg, g) / pq;
-        }
-    }
-}
-
-// beta = rz'/rz and the stopping test, from the segment's totals (the work of
-// k_seg_fin_beta)
-__device__ __forceinline__ void finish_beta(double* __restrict__ sc, int nseg, int g, double rz, double rr,
-                                            int* __restrict__ ndone)
-{
-    const double it = *sf(sc, S_ITER, nseg, g) + 1.0;
-    *sf(sc, S_ITER, nseg, g) = it;
-    *sf(sc, S_BETA, nseg, g) = rz / *sf(sc, S_RZ, nseg, g);
-    *sf(sc, S_RZ, nseg, g) = rz;
-    *sf(sc, S_RR, nseg, g) = rr;
-    if (rr <= *sf(sc, S_THRESH, nseg, g) || it >= *sf(sc, S_MAXIT, nseg, g)) {
-        *sf(sc, S_DONE, nseg, g) = 1.0;
-        atomicAdd(ndone, 1);
-    }
-}
-
-// x += alpha p, r -= alpha q; partials of r.z (Jacobi) and r.r; with FIN the
-// segment's last wave finishes beta (Jacobi); without it (multigrid) the r.z
-// partials follow the V-cycle in k_fused_rz
-template <int PC, bool FIN>
-__global__ void __launch_bounds__(PB) k_fused_update(int nnd, const int* __restrict__ slice_seg,
-                                                     const int* __restrict__ seg_slice, const double* __restrict__ p,
-                                                     const double* __restrict__ q, const double* __restrict__ m,
-                                                     double* __restrict__ x, double* __restrict__ r,
-                                                     double* __restrict__ part, long nslices, int nseg,
-                                                     double* __restrict__ sc, int* __restrict__ cnt,
-                                                     int* __restrict__ ndone)
-{
-    const int a = blockIdx.x * PB + threadIdx.x;
-    const int s = a >> 6, l = a & 63;
-    if (a >= 64L * nslices) return;
-    const int g = slice_seg[s];
-    if (*sf(sc, S_DONE, nseg, g) != 0.0) return;
-    const double alpha = *sf(sc, S_ALPHA, nseg, g);
-    double v0 = 0.0, v1 = 0.0;
-    if (a < nnd) {
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const long k = 3L * a + i;
-            x[k] += alpha * p[k];
-            const double ri = r[k] - alpha * q[k];
-            r[k] = ri;
-            if (PC != 2) v0 += ri * (m[k] * ri);
-            v1 += ri * ri;
-        }
-    }
-    v0 = wsum(v0);
-    v1 = wsum(v1);
-    if (l == 0) {
-        if (PC != 2) part[s] = v0;
-        part[nslices + s] = v1;
-    }
-    if (FIN && last_slice(cnt, seg_slice, g)) {
-        const double rz = wave_total(part, seg_slice[g], seg_slice[g + 1]);
-        const double rr = wave_total(part + nslices, seg_slice[g], seg_slice[g + 1]);
-        if (l == 0) {
-            cnt[g] = 0;
-            finish_beta(sc, nseg, g, rz, rr, ndone);
-        }
-    }
-}
-
-// multigrid: partials of r.z after the V-cycle, and beta in the last wave
-__global__ void __launch_bounds__(PB) k_fused_rz(int nnd, const int* __restrict__ slice_seg,
-                                                 const int* __restrict__ seg_slice, const double* __restrict__ r,
-                                                 const double* __restrict__ z, double* __restrict__ part,
-                                                 long nslices, int nseg, double* __restrict__ sc,
-                                                 int* __restrict__ cnt, int* __restrict__ ndone)
-{
-    const int a = blockIdx.x * PB + threadIdx.x;
-    const int s = a >> 6, l = a & 63;
-    if (a >= 64L * nslices) return;
-    const int g = slice_seg[s];
-    if (*sf(sc, S_DONE, nseg, g) != 0.0) return;
-    double v = 0.0;
-    if (a < nnd)
-#pragma unroll
-        for (int i = 0; i < 3; i++) v += r[3L * a + i] * z[3L * a + i];
-    v = wsum(v);
-    if (l == 0) part[s] = v;
-    if (last_slice(cnt, seg_slice, g)) {
-        const double rz = wave_total(part, seg_slice[g], seg_slice[g + 1]);
-        const double rr = wave_total(part + nslices, seg_slice[g], seg_slice[g + 1]);
-        if (l == 0) {
-            cnt[g] = 0;
-            finish_beta(sc, nseg, g, rz, rr, ndone);
-        }
-    }
 }
 
 // per-segment dot product a.b into out[seg]
@@ -587,24 +464,25 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
             {
                 TimedRegion tr(c, T_SPMV);
                 hipLaunchKernelGGL((k_fused_spmv<MT, PC>), dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p,
-                                   S.d_seg_slice.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val,
-                                   dinv, r, pold, pnew, q, S.d_part.p, ns, nseg, S.d_scal.p, S.d_cnt.p);
+                                   c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, dinv, r, pold,
+                                   pnew, q, S.d_part.p, ns, nseg, S.d_scal.p);
             }
             {
                 TimedRegion tr(c, T_VECTOR);
-                hipLaunchKernelGGL((k_fused_update<PC, PC != 2>), dim3(nb), dim3(PB), 0, c->stream, c->nnd,
-                                   S.d_slice_seg.p, S.d_seg_slice.p, pnew, q, dinv, x, r, S.d_part.p, ns, nseg,
-                                   S.d_scal.p, S.d_cnt.p, S.d_ndone.p);
+                hipLaunchKernelGGL(k_seg_fin_alpha, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
+                                   S.d_seg_slice.p, nseg, S.d_scal.p);
+                hipLaunchKernelGGL(k_seg_update<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, pnew,
+                                   q, dinv, x, r, S.d_part.p, ns, nseg, S.d_scal.p);
             }
             if (PC == 2) {
-                {
-                    TimedRegion tr(c, T_PRECOND);
-                    amg_vcycle(c, r, (double*)dinv, done);
-                }
+                TimedRegion tr(c, T_PRECOND);
+                amg_vcycle(c, r, (double*)dinv, done);
+                hipLaunchKernelGGL(k_seg_dot, dim3(nb), dim3(PB), 0, c->stream, c->nnd, r, dinv, S.d_part.p, ns);
+            }
+            {
                 TimedRegion tr(c, T_VECTOR);
-                hipLaunchKernelGGL(k_fused_rz, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p,
-                                   S.d_seg_slice.p, r, dinv, S.d_part.p, ns, nseg, S.d_scal.p, S.d_cnt.p,
-                                   S.d_ndone.p);
+                hipLaunchKernelGGL(k_seg_fin_beta, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
+                                   S.d_seg_slice.p, nseg, S.d_scal.p, S.d_ndone.p);
             }
         }
     };

@@ -125,7 +125,6 @@ static void setup_segments(tg_ctx* c)
     S.d_part.alloc(2 * (size_t)S.nslices);
     S.d_out.alloc((size_t)S.nseg);
     S.d_ndone.alloc(1);
-    S.d_cnt.alloc((size_t)S.nseg);
     std::vector<int> ones((size_t)S.nseg, 1);
     auto up = [&](int* dst, const std::vector<int>& v) {
         TG_HIP(hipMemcpyAsync(dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, c->stream));
@@ -135,7 +134,6 @@ static void setup_segments(tg_ctx* c)
     up(S.d_active.p, ones);
     up(S.d_elem_seg.p, S.elem_seg);
     TG_HIP(hipMemsetAsync(S.d_scal.p, 0, sizeof(double) * S_NSCAL * S.nseg, c->stream));
-    TG_HIP(hipMemsetAsync(S.d_cnt.p, 0, sizeof(int) * S.nseg, c->stream));
     // every copy and clear of the context goes through its own stream (a
     // null-stream operation is not ordered with a non-blocking stream), and
     // the host vectors copied from must outlive the copies

@@ -149,7 +149,6 @@ struct Segments {
     DBuf<double> d_part;                  // [2][nslices]
     DBuf<double> d_out;                   // [nseg]
     DBuf<int> d_ndone;
-    DBuf<int> d_cnt;                      // [nseg] slices that finished a fused pass (reset by the last one)
     std::vector<double> h_scal;
 };
 
