@@ -218,7 +218,11 @@ def main():
     spmv_avg_ms = tm["spmv_ms"] / max(1, tm["spmv_launches"])
     nblocks = (n + 1 + 2 * n) ** 3  # (3n+1)^3 node-neighbour blocks of the structured cube
     vbytes = 8 if a.precision == "fp64" else 4
-    spmv_bytes = nblocks * (9 * vbytes + 4) + 2 * 3 * 8 * m.nnd  # operator values + columns, x and y (FP64)
+    fused = s.tuning(api.TUNE_PCG_FUSED) == 1 and s.tuning(api.TUNE_SPMV_XCD) == 0 and a.precond != "block"
+    # operator values + columns once; node vectors once: x and y (FP64), and for
+    # the fused iteration r, z's source (1/diag or z) and p_old read, p written
+    node_vectors = 5 if fused else 2
+    spmv_bytes = nblocks * (9 * vbytes + 4) + node_vectors * 3 * 8 * m.nnd
     spmv_gbs = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9
     res_gflops = RES_FLOP_PER_EL * m.nel / (r_ms * 1e-3) / 1e9
     stf_gflops = STIFF_FLOP_PER_EL * m.nel / (k_ms * 1e-3) / 1e9
@@ -246,8 +250,11 @@ def main():
                    "stiffness_chunk": tuned["chunk"], "spmv_xcd_rows": tuned_spmv["xcd"]},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": spmv_gbs / (HBM_PEAK / 1e9), "traffic": traffic_from_profiles(),
-                     "kernel": f"k_seg_spmv<{'double' if a.precision == 'fp64' else 'float'}> "
-                               "(3x3-block sliced-ELL SpMV + per-slice p.q partials)",
+                     "kernel": (f"k_fused_spmv<{'double' if a.precision == 'fp64' else 'float'}> "
+                                "(3x3-block sliced-ELL SpMV with the direction update p = z + beta p_old "
+                                "formed inside, + per-slice p.q partials)") if fused else
+                               (f"k_seg_spmv<{'double' if a.precision == 'fp64' else 'float'}> "
+                                "(3x3-block sliced-ELL SpMV + per-slice p.q partials)"),
                      "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_avg_ms},
         "cpu_baseline": cpu_baseline(a.cpu_n) if a.cpu_n > 0 else None,
         "newton_solves_per_s": 1e3 / ms_per_step,
@@ -470,7 +477,7 @@ def bench_rve_serve(a):
         if i % 32 == 0:
             print(f"# plain FP64 request {i}/{n_plain}", file=sys.stderr, flush=True)
         t_r = time.perf_counter()
-        u, _, _ = serve.solve_request(r, api.FP64, fp64, log=log)
+        u, _, _ = serve.solve_request(r, api.FP64, fp64, log=log, multigrid=False)
         wall = time.perf_counter() - t_r
         clock = max(clock, r.arrival) + wall
         plain_lat.append(clock - r.arrival)

@@ -84,7 +84,10 @@ def pack(problems: list) -> Packed:
                   np.concatenate(u0s) if any_u0 else None)
 
 
-def make_solid(pk: Packed, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64, device=0) -> api.Solid:
+def make_solid(pk: Packed, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64, device=0,
+               precond=api.PC_JACOBI) -> api.Solid:
+    """A context for the packed problems (the multigrid preconditioner needs
+    a single problem)."""
     s = api.Solid(device)
     s.set_mesh(pk.X, pk.conn, pk.mat)
     for m, (E, nu) in enumerate(pk.materials):
@@ -93,6 +96,7 @@ def make_solid(pk: Packed, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64,
     s.set_precision(precision)
     s.set_kbc(pk.kbc.nodes, pk.kbc.dofs, pk.kbc.values)
     s.set_segments(pk.node_begin)
+    s.set_preconditioner(precond)
     s.initialize()
     return s
 
