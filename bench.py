@@ -477,7 +477,7 @@ def bench_rve_serve(a):
         if i % 32 == 0:
             print(f"# plain FP64 request {i}/{n_plain}", file=sys.stderr, flush=True)
         t_r = time.perf_counter()
-        u, _, _ = serve.solve_request(r, api.FP64, fp64, log=log, multigrid=False)
+        u, _, _ = serve.solve_request(r, api.FP64, fp64, log=log)
         wall = time.perf_counter() - t_r
         clock = max(clock, r.arrival) + wall
         plain_lat.append(clock - r.arrival)

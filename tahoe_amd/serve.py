@@ -6,8 +6,8 @@ budget; admitted requests of one formulation and size class are grouped
 into a mini-batch (at most `max_batch` requests, or whatever has arrived
 when a worker is free) that is packed into one device context and solved to
 Newton convergence by a worker thread on that context's own HIP stream.
-Large near-incompressible requests are solved one per context with the
-multigrid preconditioner instead.  Up to
+A batch's PCG runs until its slowest member converges, so batches are
+kept moderate (max_batch) and of one size class, and several run at once.  Up to
 `workers` mini-batches are in flight at once, so requests that arrive while
 others run are admitted as memory and workers free up and their work
 overlaps on the device.  Each request's latency is completion time minus
@@ -42,9 +42,6 @@ def footprint_bytes(req, precision=api.MIXED) -> int:
     per_el = 4 * (300 + 72 + 8 + 1 + 2)
     val_b = 8 + (4 if precision == api.MIXED else 0)
     per_node = 123 * (4 + val_b) + 27 * (4 + 9 * val_b + 9 * 4) + 12 * 3 * 8
-    if uses_multigrid(req):
-        # FP32 operator copy, gather lists, the coarser levels and their vectors
-        per_node += 27 * (9 * 4 + 8) + 27 * 36 * 4 // 4 + 10 * 3 * 8
     return nel * per_el + nnd * per_node
 
 
@@ -73,16 +70,6 @@ class ServerStats:
     wall: float = 0.0
 
 
-# Near-incompressible requests of at least this edge are solved on their own
-# context with the multigrid preconditioner: their Jacobi-PCG needs thousands
-# of iterations per Newton step, and a batch waits for its slowest member.
-MULTIGRID_MIN_EDGE = 16
-
-
-def uses_multigrid(req) -> bool:
-    return req.formulation == api.Q1P0 and req.L >= MULTIGRID_MIN_EDGE
-
-
 def size_class(req) -> int:
     """Requests are batched with others of the same formulation and size
     class (edges 4-8, 9-15, 16+), so a batch's members converge in similar
@@ -90,16 +77,13 @@ def size_class(req) -> int:
     return 0 if req.L <= 8 else (1 if req.L < 16 else 2)
 
 
-def solve_request(req, precision, opts, device=0, log=None, multigrid=None):
+def solve_request(req, precision, opts, device=0, log=None):
     """One request on its own context: one load step from the affine
     predictor, and when that fails the adaptive load path (batch.solve_adaptive).
     Returns (u, Newton corrections, load steps).  A request's result does not
-    depend on which batch, if any, it was served in.  multigrid None: the
-    server's rule (uses_multigrid); False: point Jacobi (the plain path)."""
-    mg = uses_multigrid(req) if multigrid is None else multigrid
-    pc = api.PC_AMG if mg else api.PC_JACOBI
+    depend on which batch, if any, it was served in."""
     pk = batch.pack([req.problem()])
-    s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device, precond=pc)
+    s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device)
     try:
         try:
             us, st = batch.static_solve_batch(s, pk, 1, opts)
@@ -108,7 +92,7 @@ def solve_request(req, precision, opts, device=0, log=None, multigrid=None):
             pass
     finally:
         s.close()
-    s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device, precond=pc)
+    s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device)
     try:
         us, steps, newton = batch.solve_adaptive(
             s, pk, opts, log=(lambda m: log(f"request {req.rid} (L={req.L}): {m}")) if log else None)
@@ -118,7 +102,7 @@ def solve_request(req, precision, opts, device=0, log=None, multigrid=None):
 
 
 class RVEServer:
-    def __init__(self, precision=api.MIXED, workers=4, max_batch=256, budget_bytes=64 << 30, lin_rtol=None,
+    def __init__(self, precision=api.MIXED, workers=4, max_batch=32, budget_bytes=64 << 30, lin_rtol=None,
                  device=0, keep_u=True, on_batch=None, log=None):
         self.precision = precision
         self.workers = workers
@@ -148,10 +132,6 @@ class RVEServer:
         form = head.formulation
         picked, size = [], 0
         for r in list(self._queue):
-            if uses_multigrid(head) and r is not head:
-                break  # a multigrid request is solved on its own
-            if uses_multigrid(r) and r is not head:
-                continue
             if r.formulation != form or size_class(r) != size_class(head):
                 continue
             fb = footprint_bytes(r, self.precision)
@@ -177,8 +157,7 @@ class RVEServer:
         """Solve reqs as one batch context in one load step (from the affine
         predictor); raises TahoeGpuError when any sub-problem fails."""
         pk = batch.pack([r.problem() for r in reqs])
-        s = batch.make_solid(pk, formulation=reqs[0].formulation, precision=self.precision, device=self.device,
-                             precond=api.PC_AMG if uses_multigrid(reqs[0]) else api.PC_JACOBI)
+        s = batch.make_solid(pk, formulation=reqs[0].formulation, precision=self.precision, device=self.device)
         try:
             us, st = batch.static_solve_batch(s, pk, 1, self.opts)
         finally:
