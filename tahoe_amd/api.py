@@ -151,6 +151,14 @@ def calib_copy(nbytes, reps=10, device=0):
     return ms.value
 
 
+def init_device(device=0):
+    """Make the process's first HIP calls (runtime start-up, device
+    selection, a stream and a few allocations) on the calling thread, so that
+    worker threads created afterwards each start on an initialised runtime
+    instead of racing to start it."""
+    Solid(device).close()
+
+
 class Solid:
     """One finite-element problem resident on a GPU (a tg_ctx handle)."""
 

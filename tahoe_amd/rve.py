@@ -43,6 +43,17 @@ class Request:
         return self.L ** 3
 
     def problem(self, E1=1000.0, ratio=10.0, p2=0.3, hmax=0.02) -> batch.Problem:
+        """The request's mesh, phases, boundary values and predictor (built
+        once per request and kept: a request carries its problem)."""
+        key = (E1, ratio, p2, hmax)
+        cached = self.__dict__.get("_problem")
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        p = self._build(E1, ratio, p2, hmax)
+        self.__dict__["_problem"] = (key, p)
+        return p
+
+    def _build(self, E1, ratio, p2, hmax) -> batch.Problem:
         m = M.cube(self.L, seed=self.seed, perturb=0.0)
         ph = M.uniform01(self.seed, np.arange(m.nel, dtype=np.uint64)) < p2
         m.mat = ph.astype(np.int32)
