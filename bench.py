@@ -69,6 +69,8 @@ def parse_args():
     ap.add_argument("--stiff-chunk", type=int, default=None,
                     help="elements per range of the colour-ordered stiffness launches (default: autotuned)")
     ap.add_argument("--spmv-xcd", type=int, default=None, help="PCG SpMV variant 0..3 (default: autotuned)")
+    ap.add_argument("--pcg-fused", type=int, choices=[0, 1], default=None,
+                    help="1: PCG direction formed inside the SpMV, 0: its own pass (default: autotuned)")
     ap.add_argument("--assembly", type=int, default=None,
                     help="Newton tangent: 0 into the MSR and expanded, 1 straight into the operator (default: autotuned)")
     ap.add_argument("--stiff-shape", type=int, default=None,
@@ -82,13 +84,22 @@ PRECOND = {"jacobi": 0, "block": 1, "amg": 2}   # api.PC_JACOBI, PC_BLOCK_JACOBI
 PC_NAME = {"jacobi": "Jacobi", "block": "block-Jacobi", "amg": "multigrid"}
 
 
-def traffic_from_profiles():
-    """HBM bytes per SpMV launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+def spmv_kernel_name(mt, fused, variant, pc=0):
+    """Template instance of the PCG SpMV that runs (pc: 0 Jacobi, 2 multigrid)."""
+    x, u = ("true" if variant & 1 else "false"), ("true" if variant & 2 else "false")
+    return f"k_fused_spmv<{mt}, {pc}, {x}, {u}>" if fused else f"k_seg_spmv<{mt}, {x}, {u}>"
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` (a template instance of the 128^3
+    FP64 SpMV) from the committed rocprofv3 PMC summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration); None when the
+    summary does not cover it."""
     p = os.path.join(ROOT, "profiles", "spmv_bsell_128cube_pmc.md")
     if not os.path.exists(p):
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read(), flags=re.M)
+    m = re.search(r"^traffic_bytes_per_launch\[" + re.escape(kernel) + r"\]:\s*([0-9.eE+]+)", open(p).read(),
+                  flags=re.M)
     return float(m.group(1)) if m else None
 
 
@@ -170,12 +181,16 @@ def main():
         tuned = autotune.tune_stiffness(s, reps=1, run_correction=short_pcg)
     print(f"# Newton assembly {tuned['assembly']} stiffness shape {tuned['shape']} chunk {tuned['chunk']} "
           f"(autotune ms: {tuned['ms']})", file=sys.stderr, flush=True)
+    # a profiling run times every variant too, so its trace and counters cover all of them
+    explicit = a.spmv_xcd is not None or a.pcg_fused is not None
+    tuned_spmv = autotune.tune_spmv_xcd(s, short_pcg) if a.profile or not explicit else {"ms_per_iteration": {}}
     if a.spmv_xcd is not None:
         s.set_tuning(api.TUNE_SPMV_XCD, a.spmv_xcd)
-        tuned_spmv = {"xcd": a.spmv_xcd, "ms": {}}
-    else:
-        tuned_spmv = autotune.tune_spmv_xcd(s, short_pcg)
-    print(f"# SpMV XCD mapping {tuned_spmv['xcd']} (autotune ms: {tuned_spmv['ms']})", file=sys.stderr, flush=True)
+    if a.pcg_fused is not None:
+        s.set_tuning(api.TUNE_PCG_FUSED, a.pcg_fused)
+    tuned_spmv.update(xcd=s.tuning(api.TUNE_SPMV_XCD), fused=s.tuning(api.TUNE_PCG_FUSED))
+    print(f"# PCG: SpMV variant {tuned_spmv['xcd']}, fused direction {tuned_spmv['fused']} "
+          f"(autotune ms per iteration: {tuned_spmv['ms_per_iteration']})", file=sys.stderr, flush=True)
 
     def step():
         s.reset_step()
@@ -220,10 +235,14 @@ def main():
     spmv_avg_ms = tm["spmv_ms"] / max(1, tm["spmv_launches"])
     nblocks = (n + 1 + 2 * n) ** 3  # (3n+1)^3 node-neighbour blocks of the structured cube
     vbytes = 8 if a.precision == "fp64" else 4
-    fused = s.tuning(api.TUNE_PCG_FUSED) == 1 and s.tuning(api.TUNE_SPMV_XCD) == 0 and a.precond != "block"
-    # operator values + columns once; node vectors once: x and y (FP64), and for
-    # the fused iteration r, z's source (1/diag or z) and p_old read, p written
-    node_vectors = 5 if fused else 2
+    fused = s.tuning(api.TUNE_PCG_FUSED) == 1 and a.precond != "block"
+    # operator values + columns once; node vectors once: x and y (FP64); the
+    # fused iteration reads p_old and z's sources (r and 1/diag for Jacobi, z
+    # for the multigrid) instead of x, and writes the new p besides y
+    node_vectors = (5 if a.precond == "jacobi" else 4) if fused else 2
+    kname = spmv_kernel_name("double" if a.precision == "fp64" else "float", fused, s.tuning(api.TUNE_SPMV_XCD),
+                             2 if a.precond == "amg" else 0)
+    profiled = n == 128 and a.precision == "fp64" and a.precond == "jacobi"
     spmv_bytes = nblocks * (9 * vbytes + 4) + node_vectors * 3 * 8 * m.nnd
     spmv_gbs = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9
     res_gflops = RES_FLOP_PER_EL * m.nel / (r_ms * 1e-3) / 1e9
@@ -249,14 +268,13 @@ def main():
                    "elements": m.nel, "nodes": m.nnd, "equations": n_eq, "msr_upper_nnz": msr_len - 1,
                    "newton_assembly": "operator" if tuned["assembly"] == 1 else "msr+expansion",
                    "stiffness_shape_elements_per_workgroup": api.STIFF_SHAPE_EPB[tuned["shape"]],
-                   "stiffness_chunk": tuned["chunk"], "spmv_xcd_rows": tuned_spmv["xcd"]},
+                   "stiffness_chunk": tuned["chunk"], "spmv_variant": tuned_spmv["xcd"], "pcg_fused": bool(fused)},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": spmv_gbs / (HBM_PEAK / 1e9), "traffic": traffic_from_profiles(),
-                     "kernel": (f"k_fused_spmv<{'double' if a.precision == 'fp64' else 'float'}> "
-                                "(3x3-block sliced-ELL SpMV with the direction update p = z + beta p_old "
-                                "formed inside, + per-slice p.q partials)") if fused else
-                               (f"k_seg_spmv<{'double' if a.precision == 'fp64' else 'float'}> "
-                                "(3x3-block sliced-ELL SpMV + per-slice p.q partials)"),
+                     "frac": spmv_gbs / (HBM_PEAK / 1e9),
+                     "traffic": traffic_from_profiles(kname) if profiled else None,
+                     "kernel": kname + (" (3x3-block sliced-ELL SpMV with the direction update p = z + beta p_old "
+                                        "formed inside, + per-slice p.q partials)" if fused else
+                                        " (3x3-block sliced-ELL SpMV + per-slice p.q partials)"),
                      "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_avg_ms},
         "cpu_baseline": cpu_baseline(a.cpu_n) if a.cpu_n > 0 else None,
         "newton_solves_per_s": 1e3 / ms_per_step,
@@ -271,7 +289,7 @@ def main():
                 "preconditioner": a.precond},
         "timing_ms": tm,
         "autotune_tangent_ms": {k: {str(x): y for x, y in v.items()} for k, v in tuned["ms"].items()},
-        "autotune_spmv_ms": {str(k): v for k, v in tuned_spmv["ms"].items()},
+        "autotune_pcg_ms_per_iteration": tuned_spmv["ms_per_iteration"],
     }
     print(json.dumps(line), flush=True)
 

@@ -260,7 +260,7 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
 /* TG_TUNE_PCG_FUSED    1 (default): the direction update is formed inside
  *                      the SpMV (four kernels per PCG iteration instead of
  *                      five, no separate direction pass) for the Jacobi and
- *                      multigrid preconditioners with SpMV variant 0;
+ *                      multigrid preconditioners, in any SpMV variant;
  *                      0: the five-kernel iteration. */
 enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3, TG_TUNE_PCG_GRAPH = 4,
        TG_TUNE_SPMV_XCD = 5, TG_TUNE_AMG_DEGREE = 6, TG_TUNE_AMG_WEIGHT = 7, TG_TUNE_PCG_FUSED = 8 };

@@ -85,25 +85,36 @@ def tune_stiffness(solid: api.Solid, shapes=None, chunks=None, reps: int = 2, ru
             "ms": {"assembly_shape": {f"{m}/{v}": t for (m, v), t in t_shape.items()}, "chunk": t_chunk}}
 
 
-def tune_spmv_xcd(solid: api.Solid, run_pcg) -> dict:
-    """Time the four PCG SpMV variants (hardware or XCD-contiguous row order,
-    plain or branch-free unrolled inner loop); run_pcg() must run a short PCG
-    (e.g. one Newton correction with a small iteration cap) from a fixed
-    state.  Sets the fastest."""
+def tune_spmv_xcd(solid: api.Solid, run_pcg, fused=(1, 0), reps: int = 2) -> dict:
+    """Time the PCG iteration over the four SpMV variants (hardware or
+    XCD-contiguous row order, plain or branch-free unrolled inner loop), with
+    the direction formed inside the SpMV (fused) and as its own pass; run_pcg()
+    must run a short PCG (e.g. one Newton correction with a small iteration
+    cap) from a fixed state.  The figure compared is device time per
+    iteration (SpMV + vector kernels + preconditioner), the best of `reps`
+    timed runs: the fused SpMV reads more per launch and saves the direction
+    pass.  Sets the fastest."""
     times = {}
     solid.timing_enable(True)
     try:
-        for v in (0, 1, 2, 3):
-            solid.set_tuning(api.TUNE_SPMV_XCD, v)
-            run_pcg()
-            solid.synchronize()
-            solid.timing_reset()
-            run_pcg()
-            solid.synchronize()
-            t = solid.timing()
-            times[v] = t["spmv_ms"] / max(1, t["spmv_launches"])
+        for f in fused:
+            solid.set_tuning(api.TUNE_PCG_FUSED, f)
+            for v in (0, 1, 2, 3):
+                solid.set_tuning(api.TUNE_SPMV_XCD, v)
+                run_pcg()
+                solid.synchronize()
+                best = float("inf")
+                for _ in range(reps):
+                    solid.timing_reset()
+                    run_pcg()
+                    solid.synchronize()
+                    t = solid.timing()
+                    its = max(1, t["spmv_launches"])
+                    best = min(best, (t["spmv_ms"] + t["vector_ms"] + t["precond_ms"]) / its)
+                times[(f, v)] = best
     finally:
         solid.timing_enable(False)
-    best = min(times, key=times.get)
-    solid.set_tuning(api.TUNE_SPMV_XCD, best)
-    return {"xcd": best, "ms": times}
+    bf, bv = min(times, key=times.get)
+    solid.set_tuning(api.TUNE_PCG_FUSED, bf)
+    solid.set_tuning(api.TUNE_SPMV_XCD, bv)
+    return {"xcd": bv, "fused": bf, "ms_per_iteration": {f"{f}/{v}": t for (f, v), t in times.items()}}

@@ -323,7 +323,7 @@ __device__ __forceinline__ double zval(const double* __restrict__ m, const doubl
     return PC == 2 ? m[k] : m[k] * r[k];
 }
 
-template <class MT, int PC>
+template <class MT, int PC, bool XCD, bool UNR>
 __global__ void __launch_bounds__(PB) k_fused_spmv(int nnd, const int* __restrict__ slice_seg,
                                                    const int* __restrict__ sbase, const int* __restrict__ swidth,
                                                    const int* __restrict__ bcol, const MT* __restrict__ bval,
@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(PB) k_fused_spmv(int nnd, const int* __restric
                                                    double* __restrict__ y, double* __restrict__ part, long nslices,
                                                    int nseg, const double* __restrict__ sc)
 {
-    const int a = blockIdx.x * PB + threadIdx.x;
+    const int a = row_block<XCD>() * PB + threadIdx.x;
     const int s = a >> 6, l = a & 63;
     if (a >= 64L * nslices) return;
     const int g = slice_seg[s];  // wave-uniform
@@ -343,10 +343,14 @@ __global__ void __launch_bounds__(PB) k_fused_spmv(int nnd, const int* __restric
         const long b0 = sbase[s];
         const int w = swidth[s];
         double y0 = 0, y1 = 0, y2 = 0;
+        // UNR: padding slots (column -1, zero values) multiply the row's own
+        // direction, as in k_seg_spmv
+#pragma unroll(UNR ? 3 : 1)
         for (int k = 0; k < w; k++) {
             const long blk = b0 + k;
-            const int c = bcol[blk * 64 + l];
-            if (c < 0) continue;
+            const int cr = bcol[blk * 64 + l];
+            if (!UNR && cr < 0) continue;
+            const int c = cr < 0 ? a : cr;
             const double x0 = zval<PC>(m, r, 3L * c) + beta * pold[3L * c];
             const double x1 = zval<PC>(m, r, 3L * c + 1) + beta * pold[3L * c + 1];
             const double x2 = zval<PC>(m, r, 3L * c + 2) + beta * pold[3L * c + 2];
@@ -451,7 +455,7 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
                            d_active, nseg, S.d_scal.p, rtol, maxit, S.d_ndone.p);
     }
     const int CHUNK = PC == 2 ? 8 : 16;  // even: the fused iteration alternates its two direction buffers
-    const bool fused = c->pcg_fused && PC != 1 && c->spmv_xcd == 0;
+    const bool fused = c->pcg_fused && PC != 1;
     double* pbuf[2] = {p, nullptr};
     if (fused) {
         if (c->vp2.n != 3 * (size_t)c->nnd) c->vp2.alloc(3 * (size_t)c->nnd);
@@ -463,9 +467,17 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
             double* pnew = pbuf[(k + 1) & 1];
             {
                 TimedRegion tr(c, T_SPMV);
-                hipLaunchKernelGGL((k_fused_spmv<MT, PC>), dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p,
-                                   c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, dinv, r, pold,
-                                   pnew, q, S.d_part.p, ns, nseg, S.d_scal.p);
+#define TG_FSPMV(X, U)                                                                                        \
+    hipLaunchKernelGGL((k_fused_spmv<MT, PC, X, U>), dim3(X ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream,  \
+                       c->nnd, S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, \
+                       dinv, r, pold, pnew, q, S.d_part.p, ns, nseg, S.d_scal.p)
+                switch (c->spmv_xcd) {
+                case 0: TG_FSPMV(false, false); break;
+                case 1: TG_FSPMV(true, false); break;
+                case 2: TG_FSPMV(false, true); break;
+                default: TG_FSPMV(true, true); break;
+                }
+#undef TG_FSPMV
             }
             {
                 TimedRegion tr(c, T_VECTOR);

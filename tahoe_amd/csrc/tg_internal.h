@@ -194,7 +194,7 @@ struct tg_ctx {
     int gmres_restart = 30;
     int pcg_graph = 1;                  // replay PCG chunks as a captured HIP graph
     int spmv_xcd = 0;                   // XCD-contiguous row ranges in the PCG SpMV
-    int pcg_fused = 1;                  // two kernels per PCG iteration (direction and reductions fused)
+    int pcg_fused = 1;                  // PCG direction formed inside the SpMV (four kernels per iteration)
     tg::DBuf<double> vp2;               // second direction buffer of the fused iteration
     struct {
         hipGraph_t graph = nullptr;

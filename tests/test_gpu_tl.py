@@ -234,16 +234,18 @@ def test_block_jacobi_newton(name):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_spmv_xcd_mapping_bitwise(precision):
     """The SpMV variants change only which workgroup computes which rows and
-    how the inner loop is issued (padding slots add exact zeros): the Newton
-    correction is bitwise equal for all four."""
+    how the inner loop is issued (padding slots add exact zeros), with the
+    direction formed inside the SpMV or in its own pass: the Newton
+    correction is bitwise equal for all eight."""
     from tahoe_amd import api
     g = load_golden("rve_n4")
     out = []
-    for xcd in (0, 1, 2, 3):
+    for fused, xcd in [(f, v) for f in (1, 0) for v in (0, 1, 2, 3)]:
         s = make_solid(g)
         if precision:
             s.set_precision(api.MIXED)
             s.initialize()
+        s.set_tuning(api.TUNE_PCG_FUSED, fused)
         s.set_tuning(api.TUNE_SPMV_XCD, xcd)
         s.apply_kbc(1.0)
         st = s.newton_correction(s.opts(lin_rtol=1e-10))
