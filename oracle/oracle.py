@@ -50,6 +50,9 @@ def lib():
         L.or_msr_pattern.restype = C.c_long
         L.or_residual.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, I, C.c_int, D]
         L.or_stiffness.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, D, I, C.c_int, I, D]
+        L.or_color_elements.argtypes = [C.c_int, C.c_int, I, I, I]
+        L.or_element_passes_parallel.argtypes = [C.c_int, C.c_int, D, I, I, D, D, D, I, C.c_int, I, D, D, C.c_int,
+                                                 I, I, C.c_int]
         L.or_msr_multx.argtypes = [C.c_int, I, D, D, D]
         L.or_gmres_dense.argtypes = [C.c_int, D, D, D, C.c_int, C.c_double, C.c_int, I]
         L.or_gmres_msr.argtypes = [C.c_int, I, D, D, D, C.c_int, C.c_double, C.c_int, I]
@@ -187,6 +190,34 @@ def stiffness(kind, X, conn, mat, mu, kappa, u, eqnos, n_eq, bindx):
     if rc:
         raise ArithmeticError(rc)
     return val
+
+
+def color_elements(nnd, conn):
+    """Greedy element colouring: (colour pointer, elements by colour)."""
+    conn = i32(conn)
+    ptr = np.zeros(65, dtype=np.int32)
+    elems = np.zeros(conn.shape[0], dtype=np.int32)
+    nc = lib().or_color_elements(nnd, conn.shape[0], _i(conn), _i(ptr), _i(elems))
+    if nc < 0:
+        raise RuntimeError("colouring failed")
+    return ptr[:nc + 1].copy(), elems
+
+
+def element_passes_parallel(kind, X, conn, mat, mu, kappa, u, eqnos, n_eq, bindx, colors, nthreads,
+                            want_R=True, want_K=True):
+    """Residual and stiffness passes over the colouring on nthreads OpenMP
+    threads (the multi-threaded CPU baseline); returns (R, val)."""
+    X, conn, mat, u, eqnos, bindx = f64(X), i32(conn), i32(mat), f64(u), i32(eqnos), i32(bindx)
+    mu, kappa = _mats(mu, kappa)
+    ptr, elems = colors
+    R = np.zeros(n_eq) if want_R else None
+    val = np.zeros(len(bindx)) if want_K else None
+    rc = lib().or_element_passes_parallel(kind, conn.shape[0], _d(X), _i(conn), _i(mat), _d(mu), _d(kappa), _d(u),
+                                          _i(eqnos), n_eq, _i(bindx), _d(R) if want_R else None,
+                                          _d(val) if want_K else None, len(ptr) - 1, _i(ptr), _i(elems), nthreads)
+    if rc:
+        raise ArithmeticError(rc)
+    return R, val
 
 
 def msr_multx(bindx, val, x):

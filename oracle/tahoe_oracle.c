@@ -719,6 +719,89 @@ int or_stiffness(int type, int nnd, int nel, const double* X, const int* conn, c
 }
 
 /* ------------------------------------------------------------------------ */
+/* Multi-threaded element passes (CPU baseline only)                         */
+/* ------------------------------------------------------------------------ */
+
+/* Greedy element colouring: an element takes the lowest colour that none of
+ * the elements sharing one of its nodes has taken.  Elements of one colour
+ * share no node, so they write disjoint R entries and disjoint MSR rows.
+ * Returns the colour count (at most 64), or -1. */
+int or_color_elements(int nnd, int nel, const int* conn, int* color_ptr, int* color_elems)
+{
+    unsigned long long* used = (unsigned long long*)calloc((size_t)nnd, sizeof(unsigned long long));
+    int* col = (int*)malloc(sizeof(int) * (size_t)(nel > 0 ? nel : 1));
+    if (!used || !col) { free(used); free(col); return -1; }
+    int ncol = 0;
+    for (int e = 0; e < nel; e++) {
+        unsigned long long m = 0;
+        for (int a = 0; a < 8; a++) m |= used[conn[8 * e + a]];
+        int c = 0;
+        while (c < 64 && (m >> c) & 1ULL) c++;
+        if (c == 64) { free(used); free(col); return -1; }
+        col[e] = c;
+        for (int a = 0; a < 8; a++) used[conn[8 * e + a]] |= 1ULL << c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    for (int c = 0; c <= ncol; c++) color_ptr[c] = 0;
+    for (int e = 0; e < nel; e++) color_ptr[col[e] + 1]++;
+    for (int c = 0; c < ncol; c++) color_ptr[c + 1] += color_ptr[c];
+    int* fill = (int*)malloc(sizeof(int) * (size_t)(ncol + 1));
+    if (!fill) { free(used); free(col); return -1; }
+    memcpy(fill, color_ptr, sizeof(int) * (size_t)ncol);
+    for (int e = 0; e < nel; e++) color_elems[fill[col[e]]++] = e;
+    free(fill);
+    free(used);
+    free(col);
+    return ncol;
+}
+
+/* The residual and/or stiffness pass of or_residual / or_stiffness with the
+ * elements of each colour spread over nthreads OpenMP threads (R and val may
+ * be NULL to skip that pass).  Same element arithmetic; the summation order
+ * into shared entries follows the colour order instead of the element
+ * order. */
+int or_element_passes_parallel(int type, int nel, const double* X, const int* conn, const int* mat,
+                               const double* mu, const double* kappa, const double* u, const int* eqnos,
+                               int n_eq, const int* bindx, double* R, double* val, int ncolors,
+                               const int* color_ptr, const int* color_elems, int nthreads)
+{
+    (void)nel;
+    element_fn fn = element_for(type);
+    (void)parent();  /* its one-time set-up happens here, before the threads start */
+    int status = 0;
+    if (R)
+        for (int k = 0; k < n_eq; k++) R[k] = 0.0;
+    if (val) {
+        long len = bindx[n_eq];
+        for (long k = 0; k < len; k++) val[k] = 0.0;
+    }
+    for (int c = 0; c < ncolors; c++) {
+        const int b = color_ptr[c], en = color_ptr[c + 1];
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+        for (int i = b; i < en; i++) {
+            const int e = color_elems[i];
+            double Xe[24], ue[24], Re[24], Ke[576];
+            int eq[24];
+            gather(X, conn, e, Xe);
+            gather(u, conn, e, ue);
+            for (int a = 0; a < 24; a++) eq[a] = eqnos[3 * conn[8 * e + a / 3] + a % 3];
+            const int m = mat ? mat[e] : 0;
+            int rc = fn(Xe, ue, mu[m], kappa[m], R ? Re : NULL, val ? Ke : NULL) ? 7 : 0;
+            if (!rc && R)
+                for (int a = 0; a < 24; a++)
+                    if (eq[a] > 0) R[eq[a] - 1] += Re[a];
+            if (!rc && val) rc = msr_assemble_element(Ke, eq, bindx, val);
+            if (rc) {
+#pragma omp atomic write
+                status = rc;
+            }
+        }
+        if (status) break;
+    }
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
 /* A13: SpMV, Jacobi-PCG                                                     */
 /* ------------------------------------------------------------------------ */
 

@@ -61,6 +61,15 @@ int or_stiffness(int type, int nnd, int nel, const double* X, const int* conn, c
                  const double* mu, const double* kappa, const double* u, const int* eqnos,
                  int n_eq, const int* bindx, double* val);
 
+/* Element colouring (no two elements of a colour share a node) and the
+ * residual/stiffness passes over it on nthreads OpenMP threads: the
+ * multi-threaded CPU baseline.  color_ptr has room for 65 entries. */
+int or_color_elements(int nnd, int nel, const int* conn, int* color_ptr, int* color_elems);
+int or_element_passes_parallel(int type, int nel, const double* X, const int* conn, const int* mat,
+                               const double* mu, const double* kappa, const double* u, const int* eqnos,
+                               int n_eq, const int* bindx, double* R, double* val, int ncolors,
+                               const int* color_ptr, const int* color_elems, int nthreads);
+
 /* ---- linear algebra ---- */
 void or_msr_multx(int n, const int* bindx, const double* val, const double* x, double* b);
 /* restarted GMRES(m), left Jacobi (NLSolver_NK::GMRES); maxit <= 0: 3 n */

@@ -420,3 +420,29 @@ def test_gmres_full_space_and_fe_system():
     x, ok, steps = O.gmres_msr(b_, val, rhs, 30, 1e-3)
     r = rhs - O.msr_multx(b_, val, x)
     assert ok and steps < 3 * n and np.linalg.norm(r / d) <= 1e-3 * np.linalg.norm(rhs / d)
+
+
+@pytest.mark.parametrize("kind", [O.TL, O.Q1P0])
+def test_parallel_element_passes_equal_serial(kind):
+    """The multi-threaded CPU baseline: a valid colouring (every element
+    once, no two elements of a colour sharing a node), and R and K equal the
+    element-order passes up to the summation order."""
+    m = M.cube(6, seed=5, perturb=0.05)
+    kbc = M.kbc_uniaxial(m, 0.05)
+    eq, n = O.equation_numbers(m.nnd, kbc.nodes, kbc.dofs)
+    b = O.msr_pattern(n, m.conn, eq)
+    mu, ka = O.mu_kappa(1000.0, 0.3)
+    u = np.zeros((m.nnd, 3))
+    u[kbc.nodes, kbc.dofs] = kbc.values * 0.5
+    ptr, elems = O.color_elements(m.nnd, m.conn)
+    assert 8 <= len(ptr) - 1 <= 27
+    assert np.array_equal(np.sort(elems), np.arange(m.nel))
+    for c in range(len(ptr) - 1):
+        nodes = m.conn[elems[ptr[c]:ptr[c + 1]]].ravel()
+        assert len(np.unique(nodes)) == len(nodes)
+    R0 = O.residual(kind, m.X, m.conn, m.mat, mu, ka, u, eq, n)
+    K0 = O.stiffness(kind, m.X, m.conn, m.mat, mu, ka, u, eq, n, b)
+    for threads in (1, 4):
+        R, K = O.element_passes_parallel(kind, m.X, m.conn, m.mat, mu, ka, u, eq, n, b, (ptr, elems), threads)
+        assert np.abs(R - R0).max() <= 1e-12 * np.abs(R0).max()
+        assert np.abs(K - K0).max() <= 1e-12 * np.abs(K0).max()
