@@ -25,9 +25,15 @@ def mixed_solid(g):
     return s
 
 
-def test_mixed_stiffness_is_fp32_rounding_of_fp64():
+@pytest.mark.parametrize("shape", [0, 6, 9, 15])
+def test_mixed_stiffness_is_fp32_rounding_of_fp64(shape):
+    """In several stiffness shapes: element-major lanes (0), the D*B form
+    (6), loads issued before the block products (9), block-major lanes with
+    atomic additions (15)."""
+    from tahoe_amd import api
     g = load_golden("tl_n8")
     s64, s32 = make_solid(g), mixed_solid(g)
+    s32.set_tuning(api.TUNE_STIFF_SHAPE, shape)
     u = g["step_u"][2]
     for s in (s64, s32):
         s.set_displacement(u)

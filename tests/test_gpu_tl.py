@@ -174,6 +174,9 @@ def test_stiffness_every_launch_shape(shape, chunk):
     from tahoe_amd import api
     g = load_golden("tl_n8")
     s = make_solid(g)
+    assert len(api.STIFF_SHAPE_EPB) == 16
+    with pytest.raises(api.TahoeGpuError):
+        s.set_tuning(api.TUNE_STIFF_SHAPE, len(api.STIFF_SHAPE_EPB))
     s.set_tuning(api.TUNE_STIFF_SHAPE, shape)
     s.set_tuning(api.TUNE_STIFF_CHUNK, chunk)
     assert s.tuning(api.TUNE_STIFF_SHAPE) == shape and s.tuning(api.TUNE_STIFF_CHUNK) == chunk
