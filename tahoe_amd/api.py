@@ -316,6 +316,37 @@ class Solid:
         self._ok(rc)
         return st
 
+    def newton_stepwise(self, opts=None, log=None):
+        """tg_newton's loop (NLSolver::Solve with ExitIteration,
+        NLSolver.cpp:57-264,675-757) driven from the host one correction at
+        a time (tg_newton_correction, then tg_form_rhs for ||R||), so that
+        log(correction, ||R||/||R0||, PCG iterations) can report each one on
+        a long solve.  Same tests and the same limits as tg_newton; a
+        Jacobian J <= 0 raises TahoeGpuError (code 7)."""
+        o = opts or self.opts()
+        _, err0 = self.form_rhs(want=False)
+        out = NewtonStats(0, 0, err0, err0, 0)
+        if err0 < o.abs_tolerance:
+            return out
+        it, lin = -1, 0
+        while True:
+            st = self.newton_correction(o)
+            it += 1
+            lin += st.lin_iterations
+            _, err = self.form_rhs(want=False)
+            out.error, out.iterations, out.lin_iterations = err, it + 1, lin
+            rel = err / err0
+            if log:
+                log(it + 1, rel, st.lin_iterations)
+            if rel > o.divergence_tolerance:
+                out.status = 1
+                return out
+            if rel < o.rel_tolerance or err < o.abs_tolerance:
+                return out
+            if it >= o.max_iterations:
+                out.status = 1
+                return out
+
     def newton_batch(self, opts=None):
         o = opts or self.opts()
         st = (NewtonStats * getattr(self, "nseg", 1))()
@@ -369,24 +400,33 @@ def static_solve(solid: Solid, num_steps: int, opts=None, on_step=None):
     return np.array(out_u), iters
 
 
-def static_solve_adaptive(solid: Solid, num_steps: int, opts=None, on_step=None, min_dt=1.0 / 4096, log=None):
+def static_solve_adaptive(solid: Solid, num_steps: int, opts=None, on_step=None, min_dt=1.0 / 4096, log=None,
+                          last=None, log_corrections=False):
     """The static time loop to the checkpoints t = s/num_steps with step
     cutting inside each interval: a sub-step whose Newton iteration fails
     (J <= 0, divergence, iteration limit) is reset and retried at half the
     increment, and a converged sub-step doubles the next one, never past the
     next checkpoint (FEManagerT's time-step cutting, FEManagerT.cpp:243-266,
-    with ResetStep).  Returns the converged displacement at every checkpoint,
-    the Newton corrections summed per checkpoint and the sub-step counts."""
+    with ResetStep).  last: stop after that many checkpoints (default all).
+    log_corrections: Newton runs one correction per call (Solid.newton_stepwise)
+    and log reports each.
+    Returns the converged displacement at every checkpoint, the Newton
+    corrections summed per checkpoint and the sub-step counts."""
     out_u, iters, subs = [], [], []
     t, dt = 0.0, 1.0 / num_steps
-    for k in range(1, num_steps + 1):
+    for k in range(1, (num_steps if last is None else min(last, num_steps)) + 1):
         tk = k / num_steps
         n_newton, n_sub, lin = 0, 0, 0
         while t < tk - 1e-12:
             dt = min(dt, tk - t)
             solid.apply_kbc(t + dt)
             try:
-                st = solid.newton(opts)
+                if log_corrections and log:
+                    st = solid.newton_stepwise(
+                        opts, log=lambda i, rel, pcg: log(f"t={t:.5g}+{dt:.5g} correction {i}: "
+                                                          f"|R|/|R0| {rel:.3e}, PCG {pcg}"))
+                else:
+                    st = solid.newton(opts)
                 ok = st.status == 0
             except TahoeGpuError as e:
                 if e.code != 7:

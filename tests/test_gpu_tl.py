@@ -360,3 +360,26 @@ def test_fused_pcg_iteration(name, precision, pc):
     for k in range(u1.shape[0]):
         ref = g["step_u"][k]
         assert np.abs(u1[k] - ref).max() <= TOL_U * np.abs(ref).max()
+
+
+def test_newton_stepwise_equals_newton():
+    """The host-driven Newton loop (one tg_newton_correction per call, the
+    norm from tg_form_rhs) makes the same corrections as tg_newton: bitwise
+    equal displacements, the same correction counts per step."""
+    from tahoe_amd import api
+    g = load_golden("tl_n8")
+    out = []
+    for stepwise in (False, True):
+        s = make_solid(g)
+        seen = []
+        us, iters, subs = api.static_solve_adaptive(s, int(g["num_steps"]), s.opts(lin_rtol=1e-12),
+                                                    log=seen.append, log_corrections=stepwise)
+        out.append((us, iters, subs))
+        assert any("correction" in m for m in seen) == stepwise
+        s.close()
+    (u0, n0, s0), (u1, n1, s1) = out
+    assert n0 == n1 and s0 == s1
+    assert np.array_equal(u0, u1)
+    for k in range(u1.shape[0]):
+        ref = g["step_u"][k]
+        assert np.abs(u1[k] - ref).max() <= TOL_U * np.abs(ref).max()
