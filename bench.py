@@ -196,7 +196,8 @@ def main():
     from tahoe_amd import autotune
     if a.stiff_chunk is not None or a.stiff_shape is not None or a.assembly is not None:
         s.set_tuning(api.TUNE_STIFF_CHUNK, -1 if a.stiff_chunk is None else a.stiff_chunk)
-        s.set_tuning(api.TUNE_STIFF_SHAPE, 0 if a.stiff_shape is None else a.stiff_shape)
+        if a.stiff_shape is not None:
+            s.set_tuning(api.TUNE_STIFF_SHAPE, a.stiff_shape)
         s.set_tuning(api.TUNE_NEWTON_ASSEMBLY, 0 if a.assembly is None else a.assembly)
         tuned = {"assembly": s.tuning(api.TUNE_NEWTON_ASSEMBLY), "shape": s.tuning(api.TUNE_STIFF_SHAPE),
                  "chunk": s.tuning(api.TUNE_STIFF_CHUNK), "ms": {}}

@@ -233,7 +233,7 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      before the block products, 12..14 for 12, 8 or 4
  *                      and 15 for 16 block-major with the additions issued
  *                      as memory-side atomics (order per entry unchanged:
- *                      colours);
+ *                      colours); default 9;
  * TG_TUNE_NEWTON_ASSEMBLY  where tg_newton / tg_newton_batch /
  *                      tg_newton_correction put the tangent: 0 (default)
  *                      into the MSR and then expanded into the inner solve's

@@ -187,7 +187,7 @@ struct tg_ctx {
     // stiff_chunk (by rank inside the segment), all colours of range j
     // before range j+1 (0: whole colours)
     long stiff_chunk = -1;              // -1: chosen from the matrix footprint
-    int stiff_variant = 0;              // block shape of the stiffness kernel
+    int stiff_variant = 9;              // block shape of the stiffness kernel (9: 12 elements, loads first)
     int newton_assembly = 0;            // 0: Newton's K into the MSR then expanded, 1: straight into the operator
     int precond = TG_PC_JACOBI;         // preconditioner of the inner PCG
     int linear_solver = TG_LIN_PCG;     // inner linear solver of the Newton loop
