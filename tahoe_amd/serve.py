@@ -112,7 +112,7 @@ def solve_request(req, precision, opts, device=0, log=None, precond=None):
 
 
 class RVEServer:
-    def __init__(self, precision=api.MIXED, workers=4, max_batch=32, budget_bytes=64 << 30, lin_rtol=None,
+    def __init__(self, precision=api.MIXED, workers=6, max_batch=32, budget_bytes=64 << 30, lin_rtol=None,
                  device=0, keep_u=True, on_batch=None, log=None):
         self.precision = precision
         self.workers = workers
