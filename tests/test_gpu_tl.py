@@ -383,3 +383,44 @@ def test_newton_stepwise_equals_newton():
     for k in range(u1.shape[0]):
         ref = g["step_u"][k]
         assert np.abs(u1[k] - ref).max() <= TOL_U * np.abs(ref).max()
+
+
+def test_every_dof_prescribed_and_isolated_node():
+    """Edge cases of the equation system: with every dof prescribed (no
+    equations) Newton is converged at once and u is the prescribed field;
+    a node that no element uses keeps free dofs with an empty row (zero
+    diagonal, skipped by the Jacobi preconditioner as DiagonalMatrixT skips
+    small pivots) and does not disturb the solve of the others."""
+    from tahoe_amd import api, mesh as M
+    m = M.cube(2, seed=3)
+    H = np.array([[0.01, 0.002, 0.0], [0.0, -0.005, 0.001], [0.003, 0.0, 0.008]])
+    nodes = np.repeat(np.arange(m.nnd, dtype=np.int32), 3)
+    dofs = np.tile(np.arange(3, dtype=np.int32), m.nnd)
+    vals = (m.X @ H.T).ravel()
+    s = api.Solid(0)
+    s.set_mesh(m.X, m.conn, m.mat)
+    s.set_material(0, *api.mu_kappa(1000.0, 0.3))
+    s.set_kbc(nodes, dofs, vals)
+    n_eq, _ = s.initialize()
+    assert n_eq == 0
+    s.apply_kbc(1.0)
+    st = s.newton(s.opts())
+    assert st.status == 0 and st.iterations == 0
+    assert np.allclose(s.displacement(), m.X @ H.T, rtol=0, atol=1e-15)
+    s.close()
+    # the golden 4^3 problem plus one node that no element uses
+    g = load_golden("tl_n4")
+    X = np.vstack([g["X"], [[5.0, 5.0, 5.0]]])
+    s = api.Solid(0)
+    s.set_mesh(X, g["conn"], g["mat"])
+    for k, (E, nu) in enumerate(zip(g["E"], g["nu"])):
+        s.set_material(k, *api.mu_kappa(E, nu))
+    s.set_kbc(g["kbc_nodes"], g["kbc_dofs"], g["kbc_values"])
+    n_eq, _ = s.initialize()
+    assert n_eq == len(g["eqnos"][g["eqnos"] > 0]) + 3
+    us, iters = api.static_solve(s, int(g["num_steps"]), s.opts(lin_rtol=1e-12))
+    for k in range(us.shape[0]):
+        ref = g["step_u"][k]
+        assert np.abs(us[k][:-1] - ref).max() <= TOL_U * np.abs(ref).max()
+        assert np.all(us[k][-1] == 0.0)
+    s.close()
