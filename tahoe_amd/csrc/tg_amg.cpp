@@ -154,15 +154,15 @@ void amg_setup(tg_ctx* c)
     Graph g;
     {
         std::vector<long> ptr;
-        std::vector<int> lst;
-        if (c->nptr.empty())
+        ivec lst;
+        if (c->nptr.empty()) {
             node_graph(nnd, c->nel, c->conn.data(), ptr, lst);
-        else {
+            g.adj.assign(lst.begin(), lst.end());
+        } else {
             ptr = c->nptr;
-            lst = c->nlist;
+            g.adj.assign(c->nlist.begin(), c->nlist.end());
         }
         g.ptr.swap(ptr);
-        g.adj.swap(lst);
     }
     std::vector<double> xyz(c->X);
     std::vector<char> active((size_t)nnd, 0);

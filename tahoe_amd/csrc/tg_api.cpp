@@ -477,7 +477,7 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
             c->amg.clear();
         }
         std::vector<long>().swap(c->nptr);
-        std::vector<int>().swap(c->nlist);
+        ivec().swap(c->nlist);
         setup_segments(c);
         TG_HIP(hipStreamSynchronize(c->stream));
         phase("segments");

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <memory>
 #include <utility>
 #include <string>
 #include <vector>
@@ -13,6 +14,32 @@
 #include "../../include/tahoe_gpu.h"
 
 namespace tg {
+
+// Allocator whose value-less construct leaves the element uninitialized:
+// resize() of a vector<int, NoInit<int>> does not zero-fill, so the large
+// index arrays of the setup (the MSR pattern, the node adjacency) are
+// written once, by the parallel passes that fill them.
+template <class T>
+struct NoInit : std::allocator<T> {
+    NoInit() = default;
+    template <class U>
+    NoInit(const NoInit<U>&) noexcept {}
+    template <class U>
+    struct rebind {
+        using other = NoInit<U>;
+    };
+    template <class U>
+    void construct(U* p) noexcept
+    {
+        ::new ((void*)p) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a)
+    {
+        ::new ((void*)p) U(std::forward<A>(a)...);
+    }
+};
+using ivec = std::vector<int, NoInit<int>>;
 
 // Status raised inside the library; converted to the C status at the ABI.
 struct Error {
@@ -177,12 +204,12 @@ struct tg_ctx {
     bool initialized = false;
     int n_eq = 0;
     std::vector<int> eqnos;   // [nnd*3] 1-based, -1 prescribed
-    std::vector<int> bindx;   // upper MSR pattern
+    tg::ivec bindx;           // upper MSR pattern
     long msr_len = 0;
     std::vector<int> color_ptr, color_elems;
     int ncolors = 0;
     std::vector<long> nptr;             // node adjacency (setup only, released after tg_initialize)
-    std::vector<int> nlist;
+    tg::ivec nlist;
     // stiffness launch order: every segment's elements in ranges of
     // stiff_chunk (by rank inside the segment), all colours of range j
     // before range j+1 (0: whole colours)
@@ -243,9 +270,8 @@ namespace tg {
 
 // host setup (tg_setup.cpp)
 int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
-void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, std::vector<int>& nlist);
-void msr_pattern(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const std::vector<int>& nlist,
-                 std::vector<int>& bindx);
+void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist);
+void msr_pattern(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const ivec& nlist, ivec& bindx);
 int color_elements(int nnd, int nel, const int* conn, std::vector<int>& ptr, std::vector<int>& elems);
 void build_bsell(tg_ctx* c);
 

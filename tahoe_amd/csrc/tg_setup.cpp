@@ -78,13 +78,14 @@ static int node_neighbours(int a, const std::vector<int>& ptr, const std::vector
 // built once and shared by the MSR pattern and the block operator.  Each
 // worker de-duplicates with a per-node stamp array, so only the unique
 // neighbours are sorted.
-void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, std::vector<int>& nlist)
+void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist)
 {
     std::vector<int> ptr, list;
     node_elements(nnd, nel, conn, ptr, list);
     const int MAXNB = 8 * 64;
     for (int a = 0; a < nnd; a++)
         if (ptr[a + 1] - ptr[a] > MAXNB / 8) fail(TG_OUT_OF_RANGE, "a node is shared by more than 64 elements");
+    // out == nullptr: count only
     auto unique_neighbours = [&](int a, std::vector<int>& stamp, int* out) {
         int n = 0;
         for (int p = ptr[a]; p < ptr[a + 1]; p++) {
@@ -92,16 +93,17 @@ void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, std:
             for (int b = 0; b < 8; b++)
                 if (stamp[c[b]] != a) {
                     stamp[c[b]] = a;
-                    out[n++] = c[b];
+                    if (out) out[n] = c[b];
+                    n++;
                 }
         }
-        std::sort(out, out + n);
+        if (out) std::sort(out, out + n);
         return n;
     };
     std::vector<int> deg((size_t)nnd);
     parallel_for(nnd, [&](long b, long e) {
-        std::vector<int> nb(MAXNB), stamp((size_t)nnd, -1);
-        for (long a = b; a < e; a++) deg[a] = unique_neighbours((int)a, stamp, nb.data());
+        std::vector<int> stamp((size_t)nnd, -1);
+        for (long a = b; a < e; a++) deg[a] = unique_neighbours((int)a, stamp, nullptr);
     });
     nptr.assign((size_t)nnd + 1, 0);
     for (int a = 0; a < nnd; a++) nptr[a + 1] = nptr[a] + deg[a];
@@ -116,8 +118,7 @@ void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, std:
 // (toolbox/src/graph/GraphT.cpp:376-456) packed by MSRBuilderT::GenerateMSR
 // (toolbox/src/graph/MSRBuilderT.cpp:134-181): row i lists every active
 // equation j > i sharing an element with i, ascending; bindx[0] = n_eq + 1.
-void msr_pattern(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const std::vector<int>& nlist,
-                 std::vector<int>& bindx)
+void msr_pattern(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const ivec& nlist, ivec& bindx)
 {
     std::vector<int> rowlen((size_t)n_eq, 0);
     auto columns = [&](long a, int row, int* cols) {
@@ -134,17 +135,20 @@ void msr_pattern(int n_eq, int nnd, const int* eqnos, const std::vector<long>& n
     };
     const int MAXNB = 8 * 64;
     parallel_for(nnd, [&](long b, long e) {
-        std::vector<int> cols(3 * MAXNB);
         for (long a = b; a < e; a++)
             for (int i = 0; i < 3; i++) {
-                int row = eqnos[3 * a + i];
-                if (row > 0) rowlen[row - 1] = columns(a, row, cols.data());
+                const int row = eqnos[3 * a + i];
+                if (row <= 0) continue;
+                int m = 0;  // the count of columns() without building the list
+                for (long k = nptr[a]; k < nptr[a + 1]; k++)
+                    for (int j = 0; j < 3; j++) m += eqnos[3L * nlist[k] + j] > row;
+                rowlen[row - 1] = m;
             }
     });
     long total = 0;
     for (int r = 0; r < n_eq; r++) total += rowlen[r];
     if ((long)n_eq + 1 + total > 0x7fffffffL) fail(TG_OUT_OF_RANGE, "MSR pattern exceeds 32-bit indexing");
-    bindx.assign((size_t)(n_eq + 1 + total), 0);
+    bindx.resize((size_t)(n_eq + 1 + total));  // every entry is written below
     bindx[0] = n_eq + 1;
     for (int r = 0; r < n_eq; r++) bindx[r + 1] = bindx[r] + rowlen[r];
     parallel_for(nnd, [&](long b, long e) {
@@ -195,7 +199,7 @@ void build_bsell(tg_ctx* c)
     Bsell& B = c->op;
     const int nnd = c->nnd;
     const std::vector<long>& nptr = c->nptr;
-    const std::vector<int>& nlist = c->nlist;
+    const ivec& nlist = c->nlist;
     B.nslices = (nnd + 63) / 64;
     B.slice_base.assign((size_t)B.nslices, 0);
     B.slice_width.assign((size_t)B.nslices, 0);
@@ -212,15 +216,18 @@ void build_bsell(tg_ctx* c)
     }
     B.nblocks_padded = base * 64;
     if (B.nblocks_padded * 9 > 0x7fffffffL) fail(TG_OUT_OF_RANGE, "block operator exceeds 32-bit indexing");
-    std::vector<int> col((size_t)B.nblocks_padded, -1);
+    ivec col((size_t)B.nblocks_padded);  // each slice fills its own range, padding -1
     parallel_for(B.nslices, [&](long sb, long se) {
-        for (long s = sb; s < se; s++)
+        for (long s = sb; s < se; s++) {
+            std::fill(col.begin() + (long)B.slice_base[s] * 64, col.begin() + ((long)B.slice_base[s] + B.slice_width[s]) * 64,
+                      -1);
             for (int l = 0; l < 64; l++) {
                 const long a = 64 * s + l;
                 if (a >= nnd) break;
                 for (long k = nptr[a]; k < nptr[a + 1]; k++)
                     col[(size_t)(((long)B.slice_base[s] + (k - nptr[a])) * 64 + l)] = nlist[k];
             }
+        }
     });
     B.d_slice_base.alloc(B.nslices);
     B.d_slice_width.alloc(B.nslices);
