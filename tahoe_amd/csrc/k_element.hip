@@ -259,6 +259,12 @@ __device__ __forceinline__ bool stiffness_ip(int q, bool valid, double2 mk, XU x
     return !bad;
 }
 
+// The element slot map holds, per element upper entry, the MSR value index
+// s, or -s-2 where that entry is the first contribution to s in the
+// stiffness launch order (the values are cleared before the passes, so the
+// read of a first contribution would return 0), or -1 for no entry.
+__device__ __forceinline__ int slot_target(int s) { return s < -1 ? -s - 2 : s; }
+
 // Scatter of one upper 3x3 node block (a <= b, local upper index p) of
 // element e: into the MSR values through the slot map (TO_OP = false,
 // MSRMatrixT::Assemble) or the block and its transpose into the solver
@@ -313,7 +319,7 @@ __device__ __forceinline__ void scatter_block(int e, int a, int b, int p, const 
             for (int j = 0; j < 3; j++) {
                 const int m = 3 * a + i, n = 3 * b + j;
                 if (n < m) continue;  // lower half of a diagonal block
-                const int s = es[upper_index(m, n)];
+                const int s = slot_target(es[upper_index(m, n)]);
                 if (s >= 0) add_to(&val[s], K[i][j], ATOM);
             }
     }
@@ -413,8 +419,11 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
                     const int m = 3 * a + i, n = 3 * b + j;
                     sidx[PRE ? 3 * i + j : 0] = n < m ? -1 : es[upper_index(m, n)];
                 }
+            // a first contribution skips the read (the entry is still 0)
 #pragma unroll
             for (int k = 0; k < (PRE ? 9 : 1); k++) old[k] = sidx[k] >= 0 ? val[sidx[k]] : VT(0);
+#pragma unroll
+            for (int k = 0; k < (PRE ? 9 : 1); k++) sidx[k] = slot_target(sidx[k]);
         }
         VT K[3][3];
 #pragma unroll
@@ -470,6 +479,31 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
     if (bad) atomicOr(status, 7);
 }
 
+// First contributions in launch order: first[s] = the smallest launch-order
+// position of an element with a contribution to MSR value s (within one
+// launch no two elements share an entry, so that element's entry is the
+// first to reach s); then those slots are marked -s-2.
+__global__ void k_first_touch(int nel, const int* __restrict__ order, const int* __restrict__ slot,
+                              int* __restrict__ first)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 300L * nel) return;
+    const int pos = (int)(idx / 300);
+    const int s = slot_target(slot[300L * order[pos] + idx % 300]);
+    if (s >= 0) atomicMin(&first[s], pos);
+}
+
+__global__ void k_mark_first(int nel, const int* __restrict__ order, int* __restrict__ slot,
+                             const int* __restrict__ first)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 300L * nel) return;
+    const int pos = (int)(idx / 300);
+    int* p = slot + 300L * order[pos] + idx % 300;
+    const int s = slot_target(*p);
+    if (s >= 0) *p = first[s] == pos ? -s - 2 : s;
+}
+
 // Launch list of the colour passes.  Within one colour no two elements share
 // a node, so any split of a colour into launches is race-free.  Each
 // segment's elements are split by their rank inside the segment into ranges
@@ -517,7 +551,20 @@ static const std::vector<std::pair<int, int>>& stiffness_launches(tg_ctx* c)
         if (cnt[k + 1] > cnt[k]) c->stiff_launches.emplace_back(cnt[k], cnt[k + 1] - cnt[k]);
     if (c->dstiff_elems.n != (size_t)c->nel) c->dstiff_elems.alloc((size_t)c->nel);
     TG_HIP(hipMemcpyAsync(c->dstiff_elems.p, order.data(), sizeof(int) * c->nel, hipMemcpyHostToDevice, c->stream));
-    TG_HIP(hipStreamSynchronize(c->stream));
+    {
+        // mark the first contributions of this order in the slot map
+        tg::DBuf<int> first;
+        first.alloc((size_t)c->msr_len);
+        TG_HIP(hipMemsetAsync(first.p, 0x7f, sizeof(int) * (size_t)c->msr_len, c->stream));  // > any position
+        const long n = 300L * c->nel;
+        const unsigned blocks = (unsigned)((n + 255) / 256);
+        hipLaunchKernelGGL(k_first_touch, dim3(blocks), dim3(256), 0, c->stream, c->nel, c->dstiff_elems.p,
+                           c->dslot.p, first.p);
+        hipLaunchKernelGGL(k_mark_first, dim3(blocks), dim3(256), 0, c->stream, c->nel, c->dstiff_elems.p,
+                           c->dslot.p, first.p);
+        TG_HIP(hipGetLastError());
+        TG_HIP(hipStreamSynchronize(c->stream));  // order (host) and first outlive their use
+    }
     c->stiff_chunk_built = chunk;
     return c->stiff_launches;
 }
@@ -1071,7 +1118,7 @@ __global__ void __launch_bounds__(256) k_q1p0(const int* __restrict__ elems, int
                 for (int i = 0; i < 3; i++) {
                     const int m = 3 * q + i;
                     if (m <= col) {
-                        const int s = es[upper_index(m, col)];
+                        const int s = slot_target(es[upper_index(m, col)]);
                         if (s >= 0) val[s] += (vp[i] - v0[i]) / eps;
                     }
                 }
