@@ -124,8 +124,9 @@ class RVEServer:
         self.on_batch = on_batch     # on_batch(batch_id, size, formulation, seconds) after each batch
                                      # (seconds = -1 when the batch failed and is re-solved request by request)
         # inner tolerance: Newton is driven by the FP64 residual, so a looser
-        # inner solve costs at most a Newton step, never accuracy
-        rt = lin_rtol if lin_rtol is not None else (1e-4 if precision == api.MIXED else 1e-12)
+        # inner solve costs at most a Newton step, never accuracy (1e-3 is
+        # also NLSolver_NK's default linear tolerance)
+        rt = lin_rtol if lin_rtol is not None else (1e-3 if precision == api.MIXED else 1e-12)
         self.opts = api.Solid.opts(lin_rtol=rt)
         self._lock = threading.Condition()
         self._queue = deque()        # arrived, not yet admitted
