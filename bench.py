@@ -58,7 +58,11 @@ def parse_args():
                     help="mixed-steps: solve the first K of the 10 load-step checkpoints (default all)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: 2 Newton corrections with 48 PCG iterations each")
-    ap.add_argument("--workload", choices=["cube", "mixed-steps", "rve-batch", "rve-serve"], default="cube",
+    ap.add_argument("--step-n", type=int, default=128, help="newton-step: cube edge")
+    ap.add_argument("--step-stretch", type=float, default=0.005,
+                    help="newton-step: top-face displacement of the one load step, fraction of the height")
+    ap.add_argument("--workload", choices=["cube", "mixed-steps", "rve-batch", "rve-serve", "newton-step"],
+                    default="cube",
                     help="cube: one Newton correction on the n^3 cube (default); mixed-steps: the 20%%/10-step "
                          "load path in mixed precision vs FP64, error per step; rve-batch: 4096 RVEs batched; "
                          "rve-serve: Poisson stream of RVE requests")
@@ -171,6 +175,8 @@ def main():
         return bench_rve_batch(a)
     if a.workload == "rve-serve":
         return bench_rve_serve(a)
+    if a.workload == "newton-step":
+        return bench_newton_step(a)
     from tahoe_amd import api, mesh as M
 
     n = a.n
@@ -397,6 +403,64 @@ def bench_mixed_steps(a):
                  "wall_s": {"mixed": res["mixed"]["wall_s"], "fp64": res["fp64"]["wall_s"]},
                  "speedup_wall": res["fp64"]["wall_s"] / res["mixed"]["wall_s"],
                  "timing_ms": {k: res[k]["timing"] for k in res}})
+    print(json.dumps(line), flush=True)
+
+
+def bench_newton_step(a):
+    """One full Newton load step on the n^3 cube (Newton to 1e-10 relative
+    residual from u = 0, top face raised by --step-stretch of the height) on
+    the plain FP64 path (FP64 operator, Jacobi-PCG to 1e-10, the bench line's
+    solver) and on the mixed path (FP32 stiffness and operator, multigrid-PCG
+    to 1e-3 inside Newton on the FP64 residual); wall time of the whole step,
+    elements/s = elements / that time, relative displacement error of mixed
+    vs FP64."""
+    from tahoe_amd import api, mesh as M
+    n = a.step_n
+    m = M.cube(n, seed=2024)
+    kbc = M.kbc_uniaxial(m, a.step_stretch)
+    res = {}
+    # inner solves capped (100,000 Jacobi iterations is about 100 s at 128^3)
+    for leg, prec, pc, rt, cap in (("plain_fp64", api.FP64, api.PC_JACOBI, 1e-10, 100000),
+                                   ("mixed", api.MIXED, api.PC_AMG, 1e-3, 20000)):
+        s = api.Solid(0)
+        s.set_mesh(m.X, m.conn, m.mat)
+        s.set_material(0, *api.mu_kappa(1000.0, 0.3))
+        s.set_precision(prec)
+        s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
+        s.set_preconditioner(pc)
+        n_eq, _ = s.initialize()
+        print(f"# {leg}: context ready ({n_eq} equations)", file=sys.stderr, flush=True)
+        s.apply_kbc(1.0)
+        s.synchronize()
+        t0 = time.perf_counter()
+        st = s.newton_stepwise(
+            s.opts(lin_rtol=rt, lin_maxit=cap),
+            log=lambda i, rel, pcg, leg=leg, t0=t0: print(
+                f"# {leg} correction {i}: |R|/|R0| {rel:.3e}, PCG {pcg}, {time.perf_counter() - t0:.1f} s",
+                file=sys.stderr, flush=True))
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        if st.status != 0:
+            raise RuntimeError(f"{leg}: Newton did not converge ({st.iterations} corrections)")
+        res[leg] = {"u": s.displacement(), "wall_s": wall, "newton": st.iterations, "pcg": st.lin_iterations}
+        s.close()
+    ud, um = res["plain_fp64"]["u"], res["mixed"]["u"]
+    err = float(np.abs(um - ud).max() / np.abs(ud).max())
+    value = m.nel / res["mixed"]["wall_s"]
+    line = base_line(value, "elements/s", 1, 0, res["mixed"]["wall_s"] * 1e3, "f32/f64",
+                     {"workload": f"{n}^3 Q1 hex TL neo-Hookean (Simo_isotropic E=1000 nu=0.3), +-5% perturbed, one "
+                                  f"load step: top face raised by {a.step_stretch:g} of the height from u = 0, Newton "
+                                  "to 1e-10 relative residual. Mixed: FP32 K and operator, multigrid-PCG to 1e-3 "
+                                  "inside Newton on the FP64 residual; plain FP64: Jacobi-PCG to 1e-10",
+                      "elements": m.nel})
+    line.update({"value_is": "elements / wall time of the whole Newton step, mixed path",
+                 "plain_fp64": {"elements_per_s": m.nel / res["plain_fp64"]["wall_s"],
+                                "wall_s": res["plain_fp64"]["wall_s"], "newton": res["plain_fp64"]["newton"],
+                                "pcg": res["plain_fp64"]["pcg"]},
+                 "mixed": {"wall_s": res["mixed"]["wall_s"], "newton": res["mixed"]["newton"],
+                           "pcg": res["mixed"]["pcg"]},
+                 "speedup_vs_plain_fp64": res["plain_fp64"]["wall_s"] / res["mixed"]["wall_s"],
+                 "rel_err_vs_fp64": err})
     print(json.dumps(line), flush=True)
 
 
