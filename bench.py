@@ -54,6 +54,9 @@ def parse_args():
                     help="inner PCG preconditioner: point Jacobi (the reference's DiagonalMatrixT), 3x3 nodal "
                          "block Jacobi, or one V-cycle of aggregation multigrid")
     ap.add_argument("--mixed-n", type=int, default=24, help="cube edge of the mixed-steps workload")
+    ap.add_argument("--mixed-lin-rtol", type=float, default=1e-3,
+                    help="mixed-steps: relative tolerance of the mixed leg's inner PCG (Newton runs to 1e-10 on the "
+                         "FP64 residual)")
     ap.add_argument("--mixed-checkpoints", type=int, default=10,
                     help="mixed-steps: solve the first K of the 10 load-step checkpoints (default all)")
     ap.add_argument("--profile", action="store_true",
@@ -355,7 +358,7 @@ def bench_mixed_steps(a):
         print(f"# {prec}: context ready ({n_eq} equations)", file=sys.stderr, flush=True)
         # large meshes: an inner solve is capped at 15,000 iterations (about a
         # minute with the multigrid at 128^3), so no correction runs unbounded
-        opts = s.opts(lin_rtol=1e-10 if prec == "fp64" else 1e-6, lin_maxit=15000 if n >= 96 else 200000)
+        opts = s.opts(lin_rtol=1e-10 if prec == "fp64" else a.mixed_lin_rtol, lin_maxit=15000 if n >= 96 else 200000)
         # per-kernel event timing on small meshes only: at 96^3 and above the
         # solve launches millions of kernels, and the line reports wall time
         s.timing_enable(n < 96)
@@ -391,7 +394,7 @@ def bench_mixed_steps(a):
                      {"workload": f"{n}^3 Q1 hex TL neo-Hookean, +-5% perturbed, uniaxial 20% in 10 load steps "
                                   f"({'all' if kk == 10 else f'the first {kk}'} solved), "
                                   "(a load step that fails is cut in half and retried), "
-                                  "Newton to 1e-10; mixed: FP32 K and operator, PCG to 1e-6 inside FP64 "
+                                  f"Newton to 1e-10; mixed: FP32 K and operator, PCG to {a.mixed_lin_rtol:g} inside FP64 "
                                   f"residual/Newton; FP64: PCG to 1e-10; preconditioner {a.precond}",
                       "elements": m.nel, "equations": n_eq, "preconditioner": a.precond})
     line.update({"value_is": ("elements x Newton corrections / (R + K device time), mixed" if timed else
