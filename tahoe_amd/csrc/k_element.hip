@@ -1012,7 +1012,7 @@ __device__ __forceinline__ bool q1p0_ip_force(int q, const double d[3][8], const
     deformation_gradient(u, g0, F);
     const double J = det3(F);
     ok &= J > 0.0;
-    const double sc = pow(v / (H * J), 1.0 / 3.0);
+    const double sc = cbrt(v / (H * J));
 #pragma unroll
     for (int k = 0; k < 9; k++) F[k] *= sc;
     double Jb, bb[6], sig[6];

@@ -119,7 +119,8 @@ __device__ __forceinline__ bool simo_kinematics(const double* f, double& J, doub
                         b[4] * (b[5] * b[3] - b[1] * b[4]);
     if (!(detb > 0.0)) return false;
     J = sqrt(detb);
-    const double s = pow(J, -2.0 / 3.0);
+    const double r = rcbrt(J);  // J^(-1/3)
+    const double s = r * r;
 #pragma unroll
     for (int k = 0; k < 6; k++) bb[k] = s * b[k];
     return true;
