@@ -61,19 +61,6 @@ static void node_elements(int nnd, int nel, const int* conn, std::vector<int>& p
         for (int a = 0; a < 8; a++) list[(size_t)fill[conn[8 * e + a]]++] = e;
 }
 
-// sorted unique neighbour nodes (including the node itself)
-static int node_neighbours(int a, const std::vector<int>& ptr, const std::vector<int>& list, const int* conn,
-                           int* out)
-{
-    int n = 0;
-    for (int p = ptr[a]; p < ptr[a + 1]; p++) {
-        const int* c = conn + 8L * list[p];
-        for (int b = 0; b < 8; b++) out[n++] = c[b];
-    }
-    std::sort(out, out + n);
-    return (int)(std::unique(out, out + n) - out);
-}
-
 // Node adjacency as CSR (sorted unique neighbours including the node itself),
 // built once and shared by the MSR pattern and the block operator.  Each
 // worker de-duplicates with a per-node stamp array, so only the unique
