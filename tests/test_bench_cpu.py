@@ -1,0 +1,39 @@
+"""Host-side pieces of bench.py: the committed SpMV traffic profile covers
+every kernel variant the bench line can run (FP64, Jacobi), so its roofline
+`traffic` is never left empty; kernel names, thread counts and the
+multi-threaded CPU baseline are well formed."""
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.join(os.path.dirname(__file__), "..")
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_profile_covers_every_spmv_variant(fused, variant):
+    name = bench.spmv_kernel_name("double", fused, variant)
+    t = bench.traffic_from_profiles(name)
+    assert t is not None, name
+    # at least the algorithmic bytes of one launch (the per-node vectors of
+    # the fused form included), and not absurdly more
+    alg = 4440104572 + (3 * 3 * 8 * 2146689 if fused else 0)
+    assert alg * 0.95 <= t <= alg * 2.0, (name, t)
+
+
+def test_kernel_names_and_threads():
+    assert bench.spmv_kernel_name("float", False, 3) == "k_seg_spmv<float, true, true>"
+    assert bench.spmv_kernel_name("double", True, 1, pc=2) == "k_fused_spmv<double, 2, true, false>"
+    assert bench.traffic_from_profiles("k_seg_spmv<float, false, false>") is None
+    assert 1 <= bench.cpu_threads() <= (os.cpu_count() or 1)
+
+
+def test_cpu_baselines_small_sample():
+    one, par = bench.cpu_baseline(8)
+    assert one["cores"] == 1 and one["kind"] == "port" and one["value"] > 0
+    assert par["cores"] == bench.cpu_threads() and par["value"] > 0
+    assert "8^3" in one["sample"] and "colours" in par["sample"]
