@@ -18,7 +18,7 @@ _lib = None
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
 
-TL, Q1P0, TL_STVENANT = 0, 1, 2
+TL, Q1P0, TL_STVENANT, Q1P0_STVENANT = 0, 1, 2, 3
 
 
 class NewtonOpts(C.Structure):
@@ -45,6 +45,7 @@ def lib():
         L.or_hex8_parent.argtypes = [D, D, D]
         L.or_tl_element.argtypes = [D, D, C.c_double, C.c_double, D, D]
         L.or_q1p0_element.argtypes = [D, D, C.c_double, C.c_double, D, D]
+        L.or_q1p0_stvenant_element.argtypes = [D, D, C.c_double, C.c_double, D, D]
         L.or_equation_numbers.argtypes = [C.c_int, C.c_int, I, I, I]
         L.or_msr_pattern.argtypes = [C.c_int, C.c_int, C.c_int, I, I, I, C.c_long]
         L.or_msr_pattern.restype = C.c_long
@@ -143,7 +144,8 @@ def element(kind, Xe, ue, mu, kappa, want_R=True, want_K=True):
     Xe = f64(Xe).ravel(); ue = f64(ue).ravel()
     R = np.zeros(24) if want_R else None
     K = np.zeros((24, 24)) if want_K else None
-    fn = {Q1P0: lib().or_q1p0_element, TL_STVENANT: lib().or_tl_stvenant_element}.get(kind, lib().or_tl_element)
+    fn = {Q1P0: lib().or_q1p0_element, TL_STVENANT: lib().or_tl_stvenant_element,
+          Q1P0_STVENANT: lib().or_q1p0_stvenant_element}.get(kind, lib().or_tl_element)
     rc = fn(_d(Xe), _d(ue), mu, kappa, _d(R) if want_R else None, _d(K) if want_K else None)
     if rc:
         raise ArithmeticError(rc)

@@ -456,13 +456,13 @@ static int q1p0_state(const double* Xe, const double* ue, q1p0_state_t* st, int 
 /* ComputeInternalForce / FormKd (SimoQ1P0.cpp:390-439, 528-582) with E = 0 and
  * no phase field: f = constK sum_q w det_q (J_bar/J_q) B_bar^T sigma(F_bar).
  * B_bar from SolidElementT::Set_B_bar (SolidElementT.cpp:1011-1041). */
-static int q1p0_force(const q1p0_state_t* st, double mu, double kappa, double constK, double* f)
+static int q1p0_force(stress_fn cauchy, const q1p0_state_t* st, double mu, double kappa, double constK, double* f)
 {
     const parent_t* P = parent();
     memset(f, 0, 24 * sizeof(double));
     for (int q = 0; q < 8; q++) {
         double sig[6];
-        if (or_simo_cauchy(mu, kappa, st->F[q], sig)) return 7;
+        if (cauchy(mu, kappa, st->F[q], sig)) return 7;
         double nee[24];
         for (int a = 0; a < 8; a++) {
             double nx = st->dnx[q][0][a], ny = st->dnx[q][1][a], nz = st->dnx[q][2][a];
@@ -486,17 +486,20 @@ static int q1p0_force(const q1p0_state_t* st, double mu, double kappa, double co
 
 /* Re = FormKd(-1); Ke = FormStiffness(1): forward finite differences of the
  * internal force, eps = 1e-7, column by column (SimoQ1P0.cpp:446-525).  Ke is
- * the full (unsymmetric) FD matrix; assembly reads its upper triangle. */
-int or_q1p0_element(const double Xe[24], const double ue[24], double mu, double kappa,
-                    double Re[24], double Ke[576])
+ * the full (unsymmetric) FD matrix; assembly reads its upper triangle.  The
+ * material sees F_bar through its Cauchy stress (SimoQ1P0 with any large-
+ * strain material: Simo_isotropic, or large_strain_StVenant as in the
+ * reference deck Q1P0.3D.a). */
+static int q1p0_element(stress_fn cauchy, const double Xe[24], const double ue[24], double mu, double kappa,
+                        double Re[24], double Ke[576])
 {
     q1p0_state_t st;
     if (q1p0_state(Xe, ue, &st, 1)) return 7;
-    if (Re && q1p0_force(&st, mu, kappa, -1.0, Re)) return 7;
+    if (Re && q1p0_force(cauchy, &st, mu, kappa, -1.0, Re)) return 7;
     if (Ke) {
         const double eps = 1.0e-7;
         double f0[24], fp[24], up[24];
-        if (q1p0_force(&st, mu, kappa, 1.0, f0)) return 7;
+        if (q1p0_force(cauchy, &st, mu, kappa, 1.0, f0)) return 7;
         memset(Ke, 0, 576 * sizeof(double));
         for (int col = 0; col < 24; col++) {
             memcpy(up, ue, sizeof up);
@@ -506,11 +509,23 @@ int or_q1p0_element(const double Xe[24], const double ue[24], double mu, double 
             memcpy(sp.dNX, st.dNX, sizeof st.dNX);
             /* current coordinates are X + u: perturbing u perturbs x by the same eps */
             if (q1p0_state(Xe, up, &sp, 0)) return 7;
-            if (q1p0_force(&sp, mu, kappa, 1.0, fp)) return 7;
+            if (q1p0_force(cauchy, &sp, mu, kappa, 1.0, fp)) return 7;
             for (int j = 0; j < 24; j++) Ke[24 * j + col] += (fp[j] - f0[j]) / eps;
         }
     }
     return 0;
+}
+
+int or_q1p0_element(const double Xe[24], const double ue[24], double mu, double kappa, double Re[24],
+                    double Ke[576])
+{
+    return q1p0_element(or_simo_cauchy, Xe, ue, mu, kappa, Re, Ke);
+}
+
+int or_q1p0_stvenant_element(const double Xe[24], const double ue[24], double mu, double kappa, double Re[24],
+                             double Ke[576])
+{
+    return q1p0_element(or_stvenant_cauchy, Xe, ue, mu, kappa, Re, Ke);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -620,7 +635,12 @@ typedef int (*element_fn)(const double*, const double*, double, double, double*,
 
 static element_fn element_for(int type)
 {
-    return type == OR_Q1P0 ? or_q1p0_element : (type == OR_TL_STVENANT ? or_tl_stvenant_element : or_tl_element);
+    switch (type) {
+    case OR_Q1P0: return or_q1p0_element;
+    case OR_TL_STVENANT: return or_tl_stvenant_element;
+    case OR_Q1P0_STVENANT: return or_q1p0_stvenant_element;
+    default: return or_tl_element;
+    }
 }
 
 static void gather(const double* A, const int* conn, int e, double* ae)

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-enum { OR_TL = 0, OR_Q1P0 = 1, OR_TL_STVENANT = 2 };
+enum { OR_TL = 0, OR_Q1P0 = 1, OR_TL_STVENANT = 2, OR_Q1P0_STVENANT = 3 };
 
 /* ---- material (SimoIso3D) ---- */
 void or_mu_kappa(double E, double nu, double* mu, double* kappa);
@@ -45,6 +45,9 @@ int or_tl_element(const double Xe[24], const double ue[24], double mu, double ka
                   double Re[24], double Ke[576]);
 int or_q1p0_element(const double Xe[24], const double ue[24], double mu, double kappa,
                     double Re[24], double Ke[576]);
+/* the same element with the large_strain_StVenant material (reference deck Q1P0.3D.a) */
+int or_q1p0_stvenant_element(const double Xe[24], const double ue[24], double mu, double kappa, double Re[24],
+                             double Ke[576]);
 int or_tl_stvenant_element(const double Xe[24], const double ue[24], double mu, double kappa,
                            double Re[24], double Ke[576]);
 

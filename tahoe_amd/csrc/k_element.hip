@@ -989,7 +989,7 @@ __device__ __forceinline__ double allreduce8(double v)
 //   F_bar = (v/(H J))^(1/3) F, F from reference derivatives (:286-296)
 //   B_bar from SolidElementT::Set_B_bar (SolidElementT.cpp:1011-1041).
 // All 8 lanes of the element must call it together (wave-wide shuffles).
-template <class XF, class UF>
+template <int MAT, class XF, class UF>
 __device__ __forceinline__ bool q1p0_ip_force(int q, const double d[3][8], const double det0, const double g0[3][8],
                                               XF xcur, UF u, double mu, double kappa, double f[24])
 {
@@ -1015,10 +1015,10 @@ __device__ __forceinline__ bool q1p0_ip_force(int q, const double d[3][8], const
     const double sc = cbrt(v / (H * J));
 #pragma unroll
     for (int k = 0; k < 9; k++) F[k] *= sc;
-    double Jb, bb[6], sig[6];
-    ok &= simo_kinematics(F, Jb, bb);
-    simo_cauchy(mu, kappa, Jb, bb, sig);
+    // the material sees F_bar (SimoQ1P0 passes it on as the mechanical F)
     const double Jbar = det3(F);
+    double Jk, kin[6], sig[6];
+    ok &= material_stress<MAT>(mu, kappa, F, Jbar, Jk, kin, sig);
     const double s = det * (Jbar / J);  // w det_q J_bar/J_q, weight 1
 #pragma unroll
     for (int a = 0; a < 8; a++) {
@@ -1042,7 +1042,7 @@ __device__ __forceinline__ bool q1p0_ip_force(int q, const double d[3][8], const
 // for each element dof col, u_col += eps and x_col += eps, K(:,col) =
 // (f(u+eps) - f(u))/eps, then both are restored by subtracting eps, eps = 1e-7.
 // Upper entries (row <= col) are scattered through the slot map.
-template <bool STIFF>
+template <bool STIFF, int MAT>
 __global__ void __launch_bounds__(256) k_q1p0(const int* __restrict__ elems, int count, const int* __restrict__ conn,
                                               const int* __restrict__ emat, const double2* __restrict__ props,
                                               const double* __restrict__ X, const double* __restrict__ U,
@@ -1088,7 +1088,7 @@ __global__ void __launch_bounds__(256) k_q1p0(const int* __restrict__ elems, int
     auto xa = [&](int a, int c) { return sx[le][a][c]; };
     auto ua = [&](int a, int c) { return su[le][a][c]; };
     double f[24], v0[3];
-    ok &= q1p0_ip_force(q, d, det0, g0, xa, ua, mk.x, mk.y, f);
+    ok &= q1p0_ip_force<MAT>(q, d, det0, g0, xa, ua, mk.x, mk.y, f);
     reduce_scatter8(f, q, v0);
     if (!STIFF) {
         if (valid)
@@ -1106,7 +1106,7 @@ __global__ void __launch_bounds__(256) k_q1p0(const int* __restrict__ elems, int
             }
             __syncthreads();
             double fp[24], vp[3];
-            ok &= q1p0_ip_force(q, d, det0, g0, xa, ua, mk.x, mk.y, fp);
+            ok &= q1p0_ip_force<MAT>(q, d, det0, g0, xa, ua, mk.x, mk.y, fp);
             reduce_scatter8(fp, q, vp);
             __syncthreads();
             if (q == pa) {
@@ -1135,14 +1135,20 @@ void launch_q1p0(tg_ctx* c, const double* u, double* R_node, bool stiff)
         const int first = c->color_ptr[col], count = c->color_ptr[col + 1] - first;
         if (count == 0) continue;
         const int blocks = (count + 31) / 32;
-        if (stiff)
-            hipLaunchKernelGGL(k_q1p0<true>, dim3(blocks), dim3(256), 0, c->stream, c->dcolor_elems.p + first, count,
-                               c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->deqn.p, R_node, c->dslot.p,
-                               c->dval.p, c->dstatus.p);
+#define TG_Q1P0(S, M)                                                                                          \
+    hipLaunchKernelGGL((k_q1p0<S, M>), dim3(blocks), dim3(256), 0, c->stream, c->dcolor_elems.p + first, count, \
+                       c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->deqn.p, R_node, c->dslot.p, c->dval.p, \
+                       c->dstatus.p)
+        const bool stv = c->material_model == TG_MAT_STVENANT;
+        if (stiff && stv)
+            TG_Q1P0(true, MAT_STVENANT);
+        else if (stiff)
+            TG_Q1P0(true, MAT_SIMO);
+        else if (stv)
+            TG_Q1P0(false, MAT_STVENANT);
         else
-            hipLaunchKernelGGL(k_q1p0<false>, dim3(blocks), dim3(256), 0, c->stream, c->dcolor_elems.p + first, count,
-                               c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->deqn.p, R_node, c->dslot.p,
-                               c->dval.p, c->dstatus.p);
+            TG_Q1P0(false, MAT_SIMO);
+#undef TG_Q1P0
     }
     TG_HIP(hipGetLastError());
 }

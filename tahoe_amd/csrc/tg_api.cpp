@@ -380,8 +380,6 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         if ((int)c->mu.size() < nmat) fail(TG_BAD_INPUT, "element refers to an undefined material");
         for (int m = 0; m < nmat; m++)
             if (c->mu[m] < 0.0) fail(TG_BAD_INPUT, "element refers to an undefined material");
-        if (c->material_model == TG_MAT_STVENANT && c->formulation != TG_TOTAL_LAGRANGIAN)
-            fail(TG_BAD_INPUT, "the St.Venant material is on the total-Lagrangian element only");
         for (int k : c->fbc_idx)
             if (k >= 3 * c->nnd) fail(TG_OUT_OF_RANGE, "force card out of range");
         c->initialized = false;

@@ -467,8 +467,6 @@ InputDeck ReadInputDeck(const std::string& xml_path)
         if (model >= 0 && this_model != model)
             ExceptionT::BadInputValue(caller, "all blocks of the group must use one material model");
         model = this_model;
-        if (model == TG_MAT_STVENANT && d.formulation != TG_TOTAL_LAGRANGIAN)
-            ExceptionT::BadInputValue(caller, "large_strain_StVenant is on the path with the Lagrangian elements only");
         const XMLNode* en = mat->Child("E_and_nu");
         if (!en) ExceptionT::BadInputValue(caller, mat->name + " needs <E_and_nu>");
         const int mid = (int)d.materials.size();

@@ -21,6 +21,7 @@ import numpy as np
 
 LEVEL0 = "/root/reference/benchmark_XML/level.0"
 RUNS = {
+    "q1p0_3d_a": "enhanced.strain/benchmark/Q1P0.3D.a.io0.run",
     "q1p0_3d_b": "enhanced.strain/benchmark/Q1P0.3D.b.io0.run",
     "beam_newton": "3D.elastostatic/benchmark/beam.Newton.io0.run",
     "beam_newton_totlag": "3D.elastostatic/benchmark/beam.Newton.TotLag.io0.run",

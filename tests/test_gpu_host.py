@@ -59,12 +59,15 @@ def test_fixture_deck(tmp_path, name, args):
 
 @pytest.mark.parametrize("deck,golden", [("beam.Newton.xml", "beam_newton_run.npz"),
                                          ("beam.Newton.TotLag.xml", "beam_newton_totlag_run.npz"),
-                                         ("beam.PCG.xml", "beam_pcg_run.npz")])
+                                         ("beam.PCG.xml", "beam_pcg_run.npz"),
+                                         ("Q1P0.3D.a.xml", "q1p0_3d_a_run.npz")])
 def test_reference_beam_decks(tmp_path, deck, golden):
-    """The reference's own beam decks (updated_lagrangian / total_lagrangian,
-    large_strain_StVenant, force_BC; nonlinear_solver with profile_matrix, or
-    PCG_solver -- nonlinear CG with line search -- with diagonal_matrix)
-    against the reference's committed output (13 significant digits)."""
+    """The reference's own beam decks (updated_lagrangian / total_lagrangian /
+    updated_lagrangian_Q1P0, large_strain_StVenant, force_BC; nonlinear_solver
+    with profile_matrix, or PCG_solver -- nonlinear CG with line search --
+    with diagonal_matrix) against the reference's committed output (13
+    significant digits; Q1P0.3D.a ramps the force over 5 steps, and its
+    forward-difference tangent leaves the last Newton iterate within 1e-10)."""
     out = str(tmp_path / "beam.run")
     r = run(["-f", os.path.join(DECKS, deck), "-o", out, "-linear_tolerance", "1e-13"])
     assert r.returncode == 0, r.stdout + r.stderr

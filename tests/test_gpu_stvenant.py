@@ -14,11 +14,13 @@ pytestmark = pytest.mark.gpu
 TOL_RK = 1e-12
 
 
-def cube_solid(n=6, precision=0, seed=11):
+def cube_solid(n=6, precision=0, seed=11, formulation=None):
     from tahoe_amd import api, mesh as M
     m = M.cube(n, seed=seed)
     kbc = M.kbc_uniaxial(m, 0.05)
     s = api.Solid(0)
+    if formulation is not None:
+        s.set_formulation(formulation)
     s.set_mesh(m.X, m.conn, m.mat)
     s.set_material(0, *api.mu_kappa(100.0, 0.25))
     s.set_material_model(api.MAT_STVENANT)
@@ -115,10 +117,26 @@ def test_beam_with_force_card(precision):
     s.close()
 
 
-def test_stvenant_rejected_with_q1p0():
+def test_q1p0_residual_and_stiffness_match_oracle():
+    """updated_lagrangian_Q1P0 with large_strain_StVenant (the reference's
+    Q1P0.3D.a): the material sees F_bar.  R to 1e-12; the forward-difference
+    tangent (eps = 1e-7 on both sides) to 1e-7."""
+    from oracle import oracle as O
     from tahoe_amd import api
-    m, kbc, s = cube_solid(n=2)
-    s.set_formulation(api.Q1P0)
-    with pytest.raises(api.TahoeGpuError):
-        s.initialize()
+    m, kbc, s = cube_solid(n=4, formulation=api.Q1P0)
+    eq, n = O.equation_numbers(m.nnd, kbc.nodes, kbc.dofs)
+    rng = np.random.default_rng(5)
+    u = 0.02 * rng.standard_normal((m.nnd, 3))
+    s.set_displacement(u)
+    R, _ = s.form_rhs()
+    mu, ka = O.mu_kappa(100.0, 0.25)
+    Ro = O.residual(O.Q1P0_STVENANT, m.X, m.conn, m.mat, mu, ka, u, eq, n)
+    assert np.abs(R - Ro).max() <= TOL_RK * np.abs(Ro).max()
+    s.form_lhs()
+    b = s.msr_bindx()
+    vo = O.stiffness(O.Q1P0_STVENANT, m.X, m.conn, m.mat, mu, ka, u, eq, n, b)
+    assert np.abs(s.msr_values() - vo).max() <= 1e-7 * np.abs(vo).max()
+    # the Simo Q1P0 element on the same state differs (the switch is live)
+    Rs = O.residual(O.Q1P0, m.X, m.conn, m.mat, mu, ka, u, eq, n)
+    assert np.abs(Rs - Ro).max() > 1e-6 * np.abs(Ro).max()
     s.close()

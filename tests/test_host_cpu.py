@@ -34,6 +34,15 @@ def test_reference_beam_decks_parse(deck, element):
         "large_strain_StVenant, 1 steps, matrix profile_matrix" in r.stdout
 
 
+def test_reference_q1p0_stvenant_deck_parses():
+    """Q1P0.3D.a: the beam geometry under updated_lagrangian_Q1P0 with
+    large_strain_StVenant, one scheduled force card, 5 steps."""
+    r = run(["-f", os.path.join(DECKS, "Q1P0.3D.a.xml"), "-check"])
+    assert r.returncode == 0, r.stderr
+    assert "updated_lagrangian_Q1P0, 12 nodes, 2 elements, 1 materials, 12 kinematic cards (0 scheduled), " \
+        "1 force cards, large_strain_StVenant, 5 steps, matrix profile_matrix" in r.stdout
+
+
 def test_fixture_deck_parses(tmp_path):
     g = load_golden("tl_n16")
     xml = write_fixture_deck(g, str(tmp_path))

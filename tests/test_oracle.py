@@ -260,7 +260,7 @@ def _elem(seed=0):
     return Xe
 
 
-@pytest.mark.parametrize("kind", [O.TL, O.Q1P0, O.TL_STVENANT])
+@pytest.mark.parametrize("kind", [O.TL, O.Q1P0, O.TL_STVENANT, O.Q1P0_STVENANT])
 def test_element_stiffness_is_residual_derivative(kind):
     """K = d f_int / du = -dR/du, by central differences of the element residual."""
     mu, ka = O.mu_kappa(1000.0, 0.3)
@@ -277,13 +277,14 @@ def test_element_stiffness_is_residual_derivative(kind):
         Rp, _ = O.element(kind, Xe, up.reshape(8, 3), mu, ka, want_K=False)
         Rm, _ = O.element(kind, Xe, um.reshape(8, 3), mu, ka, want_K=False)
         Kfd[:, j] = -(Rp - Rm) / (2 * h)
-    tol = 1e-5 if kind == O.Q1P0 else 1e-6
+    q1p0 = kind in (O.Q1P0, O.Q1P0_STVENANT)
+    tol = 1e-5 if q1p0 else 1e-6
     assert np.abs(K - Kfd).max() <= tol * np.abs(K).max()
-    if kind != O.Q1P0:
+    if not q1p0:
         assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
 
 
-@pytest.mark.parametrize("kind", [O.TL, O.Q1P0, O.TL_STVENANT])
+@pytest.mark.parametrize("kind", [O.TL, O.Q1P0, O.TL_STVENANT, O.Q1P0_STVENANT])
 def test_element_rigid_motion_and_translation(kind):
     """A rigid rotation of the element produces no stress, so no force; a
     rigid translation leaves R and K unchanged."""
@@ -298,7 +299,7 @@ def test_element_rigid_motion_and_translation(kind):
     R1, K1 = O.element(kind, Xe, ue, mu, ka)
     R2, K2 = O.element(kind, Xe, ue + np.array([0.3, -0.1, 0.2]), mu, ka)
     assert np.abs(R1 - R2).max() <= 1e-12 * np.abs(R1).max()
-    tol = 1e-7 if kind == O.Q1P0 else 1e-12
+    tol = 1e-7 if kind in (O.Q1P0, O.Q1P0_STVENANT) else 1e-12
     assert np.abs(K1 - K2).max() <= tol * np.abs(K1).max()
 
 
@@ -368,6 +369,50 @@ def test_beam_stvenant_force_matches_reference_run():
     for name in ("beam_newton_totlag", "beam_newton"):
         ref = np.load(os.path.join(GOLDEN, name + "_run.npz"))["u"][0]
         assert np.abs(u - ref).max() <= 1e-10 * np.abs(ref).max(), name
+
+
+def test_q1p0_stvenant_matches_reference_run():
+    """The reference's Q1P0.3D.a deck (the beam's 2 hexahedra,
+    updated_lagrangian_Q1P0 with large_strain_StVenant E = 100, nu = 0.49,
+    node set 1 fixed, force_BC -0.5 in x on node set 2 ramped over 5 steps):
+    the oracle's Newton with its SimoQ1P0 restatement reaches the
+    displacements of the reference's committed .run at every step."""
+    from conftest import GOLDEN
+    from host_decks import read_geom
+    X, conn, sets = read_geom(os.path.join(GOLDEN, "geometry", "beam.1.geom"))
+    kn = np.repeat(sets[1], 3)
+    kd = np.tile(np.arange(3), len(sets[1]))
+    eq, n = O.equation_numbers(X.shape[0], kn, kd)
+    b = O.msr_pattern(n, conn, eq)
+    mu, ka = O.mu_kappa(100.0, 0.49)
+    ref = np.load(os.path.join(GOLDEN, "q1p0_3d_a_run.npz"))
+    u = np.zeros_like(X)
+    free = eq > 0
+    mat = np.zeros(len(conn), dtype=np.int32)
+    for k, t in enumerate(ref["time"]):
+        fext = np.zeros(n)
+        for node in sets[2]:
+            fext[eq[node, 0] - 1] += -0.5 * t
+        R = O.residual(O.Q1P0_STVENANT, X, conn, mat, mu, ka, u, eq, n) + fext
+        r0 = np.linalg.norm(R)
+        for it in range(25):
+            val = O.stiffness(O.Q1P0_STVENANT, X, conn, mat, mu, ka, u, eq, n, b)
+            du, its, rr, rc = O.pcg_jacobi(b, val, R, rtol=1e-14)
+            assert rc == 0
+            u[free] += du[eq[free] - 1]
+            R = O.residual(O.Q1P0_STVENANT, X, conn, mat, mu, ka, u, eq, n) + fext
+            # the deck's test: abs_tolerance 1e-9 or rel_tolerance 1e-12
+            # (NLSolver::ExitIteration); the absolute one ends each step here
+            if np.linalg.norm(R) < 1e-9 or np.linalg.norm(R) < 1e-12 * r0:
+                break
+        assert it == 3, (k, it)  # four corrections per step
+        # the last correction starts from a residual 1e-4 of the first and
+        # carries the forward-difference tangent's error (eps = 1e-7), so the
+        # two codes' final iterates agree to 3.5e-11, not to print precision
+        assert np.abs(u - ref["u"][k]).max() <= 1e-10 * np.abs(ref["u"][k]).max(), k
+    # the Simo material on the same deck ends elsewhere (the switch is live)
+    R = O.residual(O.Q1P0, X, conn, mat, mu, ka, u, eq, n) + fext
+    assert np.linalg.norm(R) > 1e-6 * np.linalg.norm(fext)
 
 
 def test_gmres_reference_unit_cases():
