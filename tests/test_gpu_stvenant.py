@@ -1,4 +1,4 @@
-"""large_strain_StVenant on the total-Lagrangian kernels and nodal force
+"""large_strain_StVenant on the total-Lagrangian and Q1P0 kernels and nodal force
 cards (the reference's beam decks): residual and stiffness against the
 oracle's index-form restatement, both assembly paths and precisions, and
 the reference's beam.Newton / beam.Newton.TotLag displacements."""
@@ -140,3 +140,37 @@ def test_q1p0_residual_and_stiffness_match_oracle():
     Rs = O.residual(O.Q1P0, m.X, m.conn, m.mat, mu, ka, u, eq, n)
     assert np.abs(Rs - Ro).max() > 1e-6 * np.abs(Ro).max()
     s.close()
+
+
+def test_q1p0_newton_both_precisions():
+    """Q1P0 + St.Venant Newton, two uniaxial load steps: FP64 reaches the
+    oracle's Newton (with the same forward-difference tangent) and the mixed
+    path (FP32 operator, FP64 residual and Newton test) reaches FP64, both
+    within 1e-9."""
+    from oracle import oracle as O
+    from tahoe_amd import api
+    out = []
+    for precision in (0, 1):
+        m, kbc, s = cube_solid(n=4, precision=precision, formulation=api.Q1P0)
+        us, iters = api.static_solve(s, 2, s.opts(lin_rtol=1e-12))
+        out.append(us)
+        s.close()
+    mu, ka = O.mu_kappa(100.0, 0.25)
+    eq, n = O.equation_numbers(m.nnd, kbc.nodes, kbc.dofs)
+    b = O.msr_pattern(n, m.conn, eq)
+    u = np.zeros((m.nnd, 3))
+    free = eq > 0
+    for k in (1, 2):
+        u[kbc.nodes, kbc.dofs] = kbc.values * k / 2
+        R = O.residual(O.Q1P0_STVENANT, m.X, m.conn, m.mat, mu, ka, u, eq, n)
+        r0 = np.linalg.norm(R)
+        for _ in range(20):
+            val = O.stiffness(O.Q1P0_STVENANT, m.X, m.conn, m.mat, mu, ka, u, eq, n, b)
+            du, *_ = O.pcg_jacobi(b, val, R, rtol=1e-13)
+            u[free] += du[eq[free] - 1]
+            R = O.residual(O.Q1P0_STVENANT, m.X, m.conn, m.mat, mu, ka, u, eq, n)
+            if np.linalg.norm(R) < 1e-10 * r0:
+                break
+        scale = np.abs(u).max()
+        assert np.abs(out[0][k - 1] - u).max() <= 1e-9 * scale, k
+        assert np.abs(out[1][k - 1] - out[0][k - 1]).max() <= 1e-9 * scale, k
