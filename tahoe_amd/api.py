@@ -187,12 +187,16 @@ class Solid:
 
     # ---- definition ----
     def set_mesh(self, X, conn, mat=None):
-        self._X = np.ascontiguousarray(X, dtype=np.float64)
-        self._conn = np.ascontiguousarray(conn, dtype=np.int32)
-        self.nnd, self.nel = self._X.shape[0], self._conn.shape[0]
-        m = None if mat is None else np.ascontiguousarray(mat, dtype=np.int32)
-        self._ok(lib().tg_set_mesh(self._h, self.nnd, self.nel, _d(self._X), _i(self._conn),
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
+        conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 8)
+        m = None if mat is None else np.ascontiguousarray(mat, dtype=np.int32).ravel()
+        if m is not None and m.shape[0] != conn.shape[0]:
+            raise ValueError("one material id per element")
+        self._ok(lib().tg_set_mesh(self._h, X.shape[0], conn.shape[0], _d(X), _i(conn),
                                    None if m is None else _i(m)))
+        # kept only once the library has accepted the mesh
+        self._X, self._conn = X, conn
+        self.nnd, self.nel = X.shape[0], conn.shape[0]
 
     def set_material(self, mid, mu, kappa):
         self._ok(lib().tg_set_material(self._h, mid, float(mu), float(kappa)))

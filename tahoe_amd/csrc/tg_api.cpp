@@ -247,18 +247,21 @@ int tg_set_mesh(tg_ctx* c, int nnd, int nel, const double* X, const int* conn, c
                 for (int b = a + 1; b < 8; b++)
                     if (conn[8L * e + a] == conn[8L * e + b])
                         fail(TG_BAD_INPUT, "degenerate element (repeated node) is not supported");
+        if (mat)
+            for (int e = 0; e < nel; e++)
+                if (mat[e] < 0) fail(TG_BAD_INPUT, "negative material id");
+        // every check above runs before the context changes: a rejected mesh
+        // leaves the previous one (or none) in place
         c->nnd = nnd;
         c->nel = nel;
         c->X.assign(X, X + 3L * nnd);
         c->seg.nseg = 1;
         c->seg.node_begin = {0, nnd};
         c->conn.assign(conn, conn + 8L * nel);
-        c->mat.assign((size_t)nel, 0);
         if (mat)
-            for (int e = 0; e < nel; e++) {
-                if (mat[e] < 0) fail(TG_BAD_INPUT, "negative material id");
-                c->mat[e] = mat[e];
-            }
+            c->mat.assign(mat, mat + nel);
+        else
+            c->mat.assign((size_t)nel, 0);
         c->initialized = false;
     });
 }

@@ -424,3 +424,42 @@ def test_every_dof_prescribed_and_isolated_node():
         assert np.abs(us[k][:-1] - ref).max() <= TOL_U * np.abs(ref).max()
         assert np.all(us[k][-1] == 0.0)
     s.close()
+
+
+def test_input_errors_are_reported():
+    """Host-side checks turn malformed input into Tahoe status codes before
+    any kernel runs (6 kBadInputValue, 4 kOutOfRange, 1 kGeneralFail), and a
+    rejected call leaves the handle usable."""
+    from tahoe_amd import api, mesh as M
+    m = M.cube(2, seed=1)
+    kbc = M.kbc_uniaxial(m, 0.05)
+    s = api.Solid(0)
+
+    def code(fn, *a):
+        with pytest.raises(api.TahoeGpuError) as e:
+            fn(*a)
+        return e.value.code
+
+    assert code(s.set_mesh, np.zeros((0, 3)), np.zeros((0, 8), dtype=np.int32)) == 6  # empty mesh
+    bad = m.conn.copy()
+    bad[3, 5] = m.nnd  # a node that does not exist
+    assert code(s.set_mesh, m.X, bad) == 4
+    rep = m.conn.copy()
+    rep[0, 1] = rep[0, 0]  # repeated node
+    assert code(s.set_mesh, m.X, rep) == 6
+    assert code(s.set_mesh, m.X, m.conn, np.full(m.nel, -1)) == 6
+    assert code(s.initialize) == 1  # no mesh yet
+    s.set_mesh(m.X, m.conn, m.mat)
+    assert code(s.set_material, 2000, 1.0, 1.0) == 4
+    assert code(s.set_material, 0, -1.0, 1.0) == 1
+    assert code(s.set_formulation, 7) == 6
+    assert code(s.set_material_model, 9) == 6
+    assert code(s.set_kbc, [m.nnd], [0], [0.0]) == 4
+    assert code(s.set_kbc, [0], [3], [0.0]) == 4
+    assert code(s.initialize) == 6  # material 0 not defined
+    s.set_material(0, *api.mu_kappa(1000.0, 0.3))
+    s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
+    s.initialize()
+    us, iters = api.static_solve(s, 1, s.opts())
+    assert np.isfinite(us).all() and iters[0] > 0
+    s.close()
