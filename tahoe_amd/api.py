@@ -214,6 +214,8 @@ class Solid:
         nodes = np.ascontiguousarray(nodes, dtype=np.int32)
         dofs = np.ascontiguousarray(dofs, dtype=np.int32)
         values = np.ascontiguousarray(values, dtype=np.float64)
+        if not len(nodes) == len(dofs) == len(values):
+            raise ValueError("nodes, dofs and values must have one entry per force card")
         self._ok(lib().tg_set_forces(self._h, len(nodes), _i(nodes), _i(dofs), _d(values)))
 
     def apply_forces(self, scale):
@@ -230,6 +232,8 @@ class Solid:
         n = np.ascontiguousarray(nodes, dtype=np.int32)
         d = np.ascontiguousarray(dofs, dtype=np.int32)
         v = np.ascontiguousarray(values, dtype=np.float64)
+        if not len(n) == len(d) == len(v):
+            raise ValueError("nodes, dofs and values must have one entry per kinematic card")
         self._ok(lib().tg_set_kbc(self._h, len(n), _i(n), _i(d), _d(v)))
 
     def set_segments(self, node_begin):

@@ -528,13 +528,14 @@ int tg_set_forces(tg_ctx* c, int n, const int* nodes, const int* dofs, const dou
 {
     if (!c || n < 0 || (n > 0 && (!nodes || !dofs || !values))) return TG_BAD_INPUT;
     return guarded(c, [&] {
-        c->fbc_idx.resize((size_t)n);
-        c->fbc_vals.assign(values, values + n);
-        for (int k = 0; k < n; k++) {
+        // validate every card before the context changes: a rejected list
+        // leaves the previous cards in place
+        for (int k = 0; k < n; k++)
             if (nodes[k] < 0 || nodes[k] >= c->nnd || dofs[k] < 0 || dofs[k] > 2)
                 fail(TG_OUT_OF_RANGE, "force card out of range");
-            c->fbc_idx[k] = 3 * nodes[k] + dofs[k];
-        }
+        c->fbc_idx.resize((size_t)n);
+        for (int k = 0; k < n; k++) c->fbc_idx[k] = 3 * nodes[k] + dofs[k];
+        c->fbc_vals.assign(values, values + n);
         c->dfext.release();
     });
 }
