@@ -59,7 +59,7 @@ def lib():
         L.or_gmres_msr.argtypes = [C.c_int, I, D, D, D, C.c_int, C.c_double, C.c_int, I]
         L.or_pcg_jacobi.argtypes = [C.c_int, I, D, D, D, C.c_double, C.c_int, I, D]
         L.or_static_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, I, I, D, D, C.c_int, I, I, D,
-                                      C.c_int, C.POINTER(NewtonOpts), D, D, I, I]
+                                      C.c_int, I, I, D, C.c_int, C.POINTER(NewtonOpts), D, D, I, I]
         _lib = L
     return _lib
 
@@ -241,9 +241,16 @@ def pcg_jacobi(bindx, val, b, rtol=1e-12, maxit=100000):
 
 def static_solve(kind, X, conn, mat, mu, kappa, kbc_nodes, kbc_dofs, kbc_values, num_steps,
                  rel_tol=1e-10, abs_tol=1e-14, div_tol=10.0, max_iterations=20, lin_rtol=1e-12,
-                 lin_maxit=200000):
+                 lin_maxit=200000, force_nodes=(), force_dofs=(), force_values=()):
+    """FEManagerT::Solve's static load path: num_steps steps of t = s /
+    num_steps, kinematic cards at value * t, force cards adding value * t to
+    the residual, Newton per step.  Returns status, final u, u and Newton
+    count per step, total PCG iterations."""
     X, conn, mat = f64(X), i32(conn), i32(mat)
     kn, kd, kv = i32(kbc_nodes), i32(kbc_dofs), f64(kbc_values)
+    fn, fd, fv = i32(force_nodes), i32(force_dofs), f64(force_values)
+    if not (len(fn) == len(fd) == len(fv)):
+        raise ValueError("force card arrays differ in length")
     mu, kappa = _mats(mu, kappa)
     nnd = X.shape[0]
     u = np.zeros((nnd, 3))
@@ -252,7 +259,8 @@ def static_solve(kind, X, conn, mat, mu, kappa, kbc_nodes, kbc_dofs, kbc_values,
     lin = C.c_int()
     o = NewtonOpts(rel_tol, abs_tol, div_tol, max_iterations, lin_rtol, lin_maxit)
     rc = lib().or_static_solve(kind, nnd, conn.shape[0], _d(X), _i(conn), _i(mat), _d(mu), _d(kappa),
-                               len(kn), _i(kn), _i(kd), _d(kv), num_steps, C.byref(o), _d(u), _d(step_u),
+                               len(kn), _i(kn), _i(kd), _d(kv), len(fn), _i(fn), _i(fd), _d(fv), num_steps,
+                               C.byref(o), _d(u), _d(step_u),
                                _i(iters), C.byref(lin))
     return dict(status=rc, u=u, step_u=step_u, step_iters=iters, lin_iters=lin.value)
 

@@ -1009,15 +1009,29 @@ int or_gmres_msr(int n, const int* bindx, const double* val, const double* b, do
     return gmres(n, msr_mv, &d, val, b, x, m, tol, maxit > 0 ? maxit : 3 * n, steps);
 }
 
+/* R += t * f at the active dofs of the force cards (FieldT::FormRHS,
+ * FieldT.cpp:390-401, with FBC_CardT::CurrentValue, FBC_CardT.cpp:57-63,
+ * on the schedule (0,0)-(1,1)). */
+static void add_forces(int nf, const int* fn, const int* fd, const double* fv, double t, const int* eqnos,
+                       double* R)
+{
+    for (int k = 0; k < nf; k++) {
+        const int e = eqnos[3 * fn[k] + fd[k]];
+        if (e > 0) R[e - 1] += fv[k] * t;
+    }
+}
+
 /* FEManagerT::Solve (FEManagerT.cpp:138-269) with the static integrator:
  * per step t = s/num_steps, nStaticIntegrator::ConsistentKBC
  * (nStaticIntegrator.cpp:18-46) sets u = value * t on each card (schedule
- * (0,0)-(1,1)); NLSolver::Solve (NLSolver.cpp:57-264) iterates
+ * (0,0)-(1,1)) and the force cards contribute value * t to R;
+ * NLSolver::Solve (NLSolver.cpp:57-264) iterates
  * R -> ||R|| -> ExitIteration (NLSolver.cpp:675-757) -> K -> solve -> u += du
  * (nStaticIntegrator::Corrector, nStaticIntegrator.cpp:69-87). */
 int or_static_solve(int type, int nnd, int nel, const double* X, const int* conn, const int* mat,
                     const double* mu, const double* kappa, int nkbc, const int* kbc_nodes,
-                    const int* kbc_dofs, const double* kbc_values, int num_steps,
+                    const int* kbc_dofs, const double* kbc_values, int nforce, const int* f_nodes,
+                    const int* f_dofs, const double* f_values, int num_steps,
                     const or_newton_opts* o, double* u, double* step_u, int* step_iters,
                     int* lin_iters_total)
 {
@@ -1034,6 +1048,7 @@ int or_static_solve(int type, int nnd, int nel, const double* X, const int* conn
         double t = (double)s / (double)num_steps;
         for (int k = 0; k < nkbc; k++) u[3 * kbc_nodes[k] + kbc_dofs[k]] = kbc_values[k] * t;
         if (or_residual(type, nnd, nel, X, conn, mat, mu, kappa, u, eqnos, n_eq, R)) { status = 7; break; }
+        add_forces(nforce, f_nodes, f_dofs, f_values, t, eqnos, R);
         double err0 = sqrt(dot(n_eq, R, R));
         int iter = -1, done = (err0 < o->abs_tol);
         while (!done) {
@@ -1045,6 +1060,7 @@ int or_static_solve(int type, int nnd, int nel, const double* X, const int* conn
                 if (eqnos[k] > 0) u[k] += du[eqnos[k] - 1];
             iter++;
             if (or_residual(type, nnd, nel, X, conn, mat, mu, kappa, u, eqnos, n_eq, R)) { status = 7; break; }
+            add_forces(nforce, f_nodes, f_dofs, f_values, t, eqnos, R);
             double err = sqrt(dot(n_eq, R, R));
             double rel = err / err0;
             if (rel > o->div_tol) { status = 1; break; }

@@ -92,7 +92,8 @@ typedef struct {
 
 int or_static_solve(int type, int nnd, int nel, const double* X, const int* conn, const int* mat,
                     const double* mu, const double* kappa, int nkbc, const int* kbc_nodes,
-                    const int* kbc_dofs, const double* kbc_values, int num_steps,
+                    const int* kbc_dofs, const double* kbc_values, int nforce, const int* f_nodes,
+                    const int* f_dofs, const double* f_values, int num_steps,
                     const or_newton_opts* opts, double* u /* in: start, out: final */,
                     double* step_u /* [num_steps][nnd][3] or NULL */, int* step_iters,
                     int* lin_iters_total);

@@ -56,10 +56,24 @@ def parse_run(path):
 def read_geom(path):
     """Nodes [nnd][3], hexahedra [nel][8] (0-based) and node sets {ID: 0-based
     nodes} of a TahoeII text geometry file (the sections *nodesets,
-    *elements, *nodes; '#' starts a comment)."""
+    *elements, *nodes; '#' starts a comment).  A block whose first token is a
+    file name instead of a count is read from that file, beside the
+    geometry file (TahoeII's external blocks, e.g. cube.1.elem)."""
+    folder = os.path.dirname(path)
+
+    def tokens(p):
+        out = []
+        for line in open(p):
+            out += line.split("#")[0].split()
+        return out
+
     toks = []
-    for line in open(path):
-        toks += line.split("#")[0].split()
+    for t in tokens(path):
+        if not t.startswith("*") and not t.lstrip("+-").replace(".", "", 1).replace("e", "", 1).replace(
+                "-", "").replace("+", "").isdigit() and os.path.exists(os.path.join(folder, t)):
+            toks += tokens(os.path.join(folder, t))
+        else:
+            toks.append(t)
     i = 0
     sets, X, conn = [], None, []
 

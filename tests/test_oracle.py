@@ -40,6 +40,73 @@ def test_equation_numbers(name):
     assert np.array_equal(eq, g["eqnos"])
 
 
+def deck_kbc(rec):
+    """Geometry and kinematic cards of a reference deck in deck_equations.json."""
+    from conftest import GOLDEN
+    from host_decks import read_geom
+    X, conn, sets = read_geom(os.path.join(GOLDEN, "geometry", rec["geometry"]))
+    kn = np.concatenate([sets[s] for s, d in rec["kbc"]]).astype(np.int32)
+    kd = np.concatenate([np.full(len(sets[s]), d - 1) for s, d in rec["kbc"]]).astype(np.int32)
+    return X, conn, kn, kd
+
+
+def deck_equations():
+    import json
+    from conftest import GOLDEN
+    return json.load(open(os.path.join(GOLDEN, "deck_equations.json")))
+
+
+@pytest.mark.parametrize("deck", ["mat.5.a", "traction.a", "beam.Newton.TotLag", "Q1P0.3D.a"])
+def test_equation_count_matches_reference_out(deck):
+    """The reference's own .out files print the number of equations of each
+    deck (tests/golden/deck_equations.json, read from benchmark_XML by
+    tests/golden/make_deck_goldens.py); the oracle's numbering of the same
+    geometry and kinematic cards gives that count."""
+    rec = deck_equations()[deck]
+    X, conn, kn, kd = deck_kbc(rec)
+    eq, n = O.equation_numbers(X.shape[0], kn, kd)
+    assert n == rec["n_eq"] and int(eq.max()) == n
+
+
+def test_equation_number_rule_properties():
+    """The numbering rule of NodeManagerT::SetEquationNumbers and
+    FieldT::InitEquations (NodeManagerT.cpp:712-760, FieldT.cpp:635-658):
+    the free dofs carry 1..n_eq exactly once, consecutive in node-major /
+    dof-minor order, and exactly the prescribed dofs carry -1.  A dof-major
+    numbering, a 0-based one or one that also numbers the prescribed dofs
+    fails these checks (each is applied below to show it)."""
+    rng = np.random.default_rng(11)
+
+    def rule_holds(e, fixed):
+        n = int((~fixed).sum())
+        return (np.all(e[fixed] == -1) and np.array_equal(e[~fixed], np.arange(1, n + 1)))
+
+    for trial in range(20):
+        nnd = int(rng.integers(1, 60))
+        nk = int(rng.integers(0, 3 * nnd + 1))
+        flat = rng.choice(3 * nnd, size=nk, replace=False)
+        kn, kd = (flat // 3).astype(np.int32), (flat % 3).astype(np.int32)
+        # a dof named by two cards is prescribed once
+        kn, kd = np.concatenate([kn, kn[:2]]), np.concatenate([kd, kd[:2]])
+        eq, n = O.equation_numbers(nnd, kn, kd)
+        fixed = np.zeros(3 * nnd, dtype=bool)
+        fixed[flat] = True
+        assert n == 3 * nnd - nk
+        assert rule_holds(eq.ravel(), fixed)
+        if 0 < nk < 3 * nnd - 1 and nnd > 1:
+            # dof-major (x of every node, then y, then z) breaks the order
+            dm = -np.ones(3 * nnd, dtype=np.int64)
+            free_dm = np.flatnonzero(~fixed.reshape(nnd, 3).T.ravel())
+            dmt = dm.copy()
+            dmt[free_dm] = np.arange(1, n + 1)
+            dm = dmt.reshape(3, nnd).T.ravel()
+            assert not rule_holds(dm, fixed) or np.array_equal(dm, eq.ravel())
+            zero = eq.ravel() - (eq.ravel() > 0)  # 0-based
+            assert not rule_holds(zero, fixed)
+            allnum = np.arange(1, 3 * nnd + 1)  # prescribed dofs numbered too
+            assert not rule_holds(allnum, fixed)
+
+
 @pytest.mark.parametrize("name", TL_CASES + Q_CASES)
 def test_msr_pattern_matches_reference_dump(name):
     """On perturbed meshes no stored entry is an exact zero, so the printed
@@ -334,85 +401,115 @@ def test_pcg_and_multx_against_dense():
     assert np.abs(xs - xd).max() <= 1e-9 * np.abs(xd).max()
 
 
-def test_beam_stvenant_force_matches_reference_run():
-    """The reference's beam.Newton.TotLag deck (2 hexahedra, node set 1
-    fixed, force_BC -0.1 in x on node set 2, large_strain_StVenant E = 100,
-    nu = 0.25, one step) solved by the oracle's Newton with the force card
-    in the residual (FieldT::FormRHS): the displacements of the reference's
-    committed .run (13 significant digits).  Both the total- and the
-    updated-Lagrangian runs of the reference print the same values."""
+def beam_deck(force):
+    """The beam decks' geometry (tests/golden/geometry/beam.1.geom, 2
+    hexahedra): node set 1 fixed in all dofs, `force` in x on each node of
+    node set 2."""
     from conftest import GOLDEN
     from host_decks import read_geom
     X, conn, sets = read_geom(os.path.join(GOLDEN, "geometry", "beam.1.geom"))
-    kn = np.repeat(sets[1], 3)
-    kd = np.tile(np.arange(3), len(sets[1]))
+    kn = np.repeat(sets[1], 3).astype(np.int32)
+    kd = np.tile(np.arange(3), len(sets[1])).astype(np.int32)
+    fn = sets[2].astype(np.int32)
+    return X, conn, kn, kd, fn, np.zeros(len(fn), dtype=np.int32), np.full(len(fn), force)
+
+
+def equilibrium_residual(kind, X, conn, mu, ka, kn, kd, kv, fn, fd, fv, u, t):
+    """||R(u) + t f_ext|| over the free dofs, after checking that the
+    prescribed dofs of u hold t * value."""
     eq, n = O.equation_numbers(X.shape[0], kn, kd)
-    b = O.msr_pattern(n, conn, eq)
+    assert np.array_equal(u[kn, kd], np.asarray(kv) * t)
+    R = O.residual(kind, X, conn, np.zeros(len(conn), dtype=np.int32), mu, ka, u, eq, n)
+    for a, d, v in zip(fn, fd, fv):
+        R[eq[a, d] - 1] += v * t
+    return np.linalg.norm(R)
+
+
+def test_static_solve_beam_stvenant_matches_reference_run():
+    """The reference's beam.Newton.TotLag deck (2 hexahedra, node set 1
+    fixed, force_BC -0.1 in x on node set 2, large_strain_StVenant E = 100,
+    nu = 0.25, one step, abs_tolerance = rel_tolerance = 1e-12) solved by the
+    oracle's static load path with the force card (or_static_solve): the
+    displacements of the reference's committed .run (13 significant digits),
+    which the total- and the updated-Lagrangian runs print alike; at the
+    converged state the residual vanishes (equilibrium with the load)."""
+    from conftest import GOLDEN
+    X, conn, kn, kd, fn, fd, fv = beam_deck(-0.1)
     mu, ka = O.mu_kappa(100.0, 0.25)
-    fext = np.zeros(n)
-    for node in sets[2]:
-        fext[eq[node, 0] - 1] += -0.1
-    u = np.zeros_like(X)
-    free = eq > 0
     mat = np.zeros(len(conn), dtype=np.int32)
-    R = O.residual(O.TL_STVENANT, X, conn, mat, mu, ka, u, eq, n) + fext
-    r0 = np.linalg.norm(R)
-    for it in range(10):
-        val = O.stiffness(O.TL_STVENANT, X, conn, mat, mu, ka, u, eq, n, b)
-        du, its, rr, rc = O.pcg_jacobi(b, val, R, rtol=1e-14)
-        assert rc == 0
-        u[free] += du[eq[free] - 1]
-        R = O.residual(O.TL_STVENANT, X, conn, mat, mu, ka, u, eq, n) + fext
-        if np.linalg.norm(R) < 1e-12 * r0:
-            break
-    assert np.linalg.norm(R) < 1e-12 * r0
+    r = O.static_solve(O.TL_STVENANT, X, conn, mat, mu, ka, kn, kd, np.zeros(len(kn)), 1, rel_tol=1e-12,
+                       abs_tol=1e-12, div_tol=10.0, max_iterations=10, lin_rtol=1e-14, force_nodes=fn,
+                       force_dofs=fd, force_values=fv)
+    assert r["status"] == 0
+    u = r["step_u"][0]
     for name in ("beam_newton_totlag", "beam_newton"):
         ref = np.load(os.path.join(GOLDEN, name + "_run.npz"))["u"][0]
         assert np.abs(u - ref).max() <= 1e-10 * np.abs(ref).max(), name
+    assert equilibrium_residual(O.TL_STVENANT, X, conn, mu, ka, kn, kd, np.zeros(len(kn)), fn, fd, fv, u, 1.0) \
+        <= 1e-12 * 0.1 * len(fn)
+    # without the force card the beam does not move: the cards are live
+    r0 = O.static_solve(O.TL_STVENANT, X, conn, mat, mu, ka, kn, kd, np.zeros(len(kn)), 1)
+    assert r0["status"] == 0 and np.abs(r0["u"]).max() == 0.0
 
 
-def test_q1p0_stvenant_matches_reference_run():
+def test_static_solve_q1p0_stvenant_matches_reference_run():
     """The reference's Q1P0.3D.a deck (the beam's 2 hexahedra,
     updated_lagrangian_Q1P0 with large_strain_StVenant E = 100, nu = 0.49,
-    node set 1 fixed, force_BC -0.5 in x on node set 2 ramped over 5 steps):
-    the oracle's Newton with its SimoQ1P0 restatement reaches the
-    displacements of the reference's committed .run at every step."""
+    node set 1 fixed, force_BC -0.5 in x on node set 2 ramped over 5 steps,
+    abs_tolerance 1e-9, rel_tolerance 1e-12, max_iterations 25) solved by
+    or_static_solve: at every step the displacements of the reference's
+    committed .run, and equilibrium with that step's load.  The deck's
+    absolute test ends each step; the last correction starts from a
+    residual 1e-4 of the first and carries the forward-difference tangent's
+    error (eps = 1e-7), so the two codes' iterates agree to 3.5e-11, not to
+    print precision."""
     from conftest import GOLDEN
-    from host_decks import read_geom
-    X, conn, sets = read_geom(os.path.join(GOLDEN, "geometry", "beam.1.geom"))
-    kn = np.repeat(sets[1], 3)
-    kd = np.tile(np.arange(3), len(sets[1]))
-    eq, n = O.equation_numbers(X.shape[0], kn, kd)
-    b = O.msr_pattern(n, conn, eq)
+    X, conn, kn, kd, fn, fd, fv = beam_deck(-0.5)
     mu, ka = O.mu_kappa(100.0, 0.49)
-    ref = np.load(os.path.join(GOLDEN, "q1p0_3d_a_run.npz"))
-    u = np.zeros_like(X)
-    free = eq > 0
     mat = np.zeros(len(conn), dtype=np.int32)
+    ref = np.load(os.path.join(GOLDEN, "q1p0_3d_a_run.npz"))
+    assert np.allclose(ref["time"], np.arange(1, 6) / 5)
+    r = O.static_solve(O.Q1P0_STVENANT, X, conn, mat, mu, ka, kn, kd, np.zeros(len(kn)), 5, rel_tol=1e-12,
+                       abs_tol=1e-9, div_tol=1e5, max_iterations=25, lin_rtol=1e-14, force_nodes=fn,
+                       force_dofs=fd, force_values=fv)
+    assert r["status"] == 0
+    assert r["step_iters"].tolist() == [4] * 5
     for k, t in enumerate(ref["time"]):
-        fext = np.zeros(n)
-        for node in sets[2]:
-            fext[eq[node, 0] - 1] += -0.5 * t
-        R = O.residual(O.Q1P0_STVENANT, X, conn, mat, mu, ka, u, eq, n) + fext
-        r0 = np.linalg.norm(R)
-        for it in range(25):
-            val = O.stiffness(O.Q1P0_STVENANT, X, conn, mat, mu, ka, u, eq, n, b)
-            du, its, rr, rc = O.pcg_jacobi(b, val, R, rtol=1e-14)
-            assert rc == 0
-            u[free] += du[eq[free] - 1]
-            R = O.residual(O.Q1P0_STVENANT, X, conn, mat, mu, ka, u, eq, n) + fext
-            # the deck's test: abs_tolerance 1e-9 or rel_tolerance 1e-12
-            # (NLSolver::ExitIteration); the absolute one ends each step here
-            if np.linalg.norm(R) < 1e-9 or np.linalg.norm(R) < 1e-12 * r0:
-                break
-        assert it == 3, (k, it)  # four corrections per step
-        # the last correction starts from a residual 1e-4 of the first and
-        # carries the forward-difference tangent's error (eps = 1e-7), so the
-        # two codes' final iterates agree to 3.5e-11, not to print precision
+        u = r["step_u"][k]
         assert np.abs(u - ref["u"][k]).max() <= 1e-10 * np.abs(ref["u"][k]).max(), k
+        assert equilibrium_residual(O.Q1P0_STVENANT, X, conn, mu, ka, kn, kd, np.zeros(len(kn)), fn, fd, fv, u,
+                                    t) < 1e-9, k
     # the Simo material on the same deck ends elsewhere (the switch is live)
-    R = O.residual(O.Q1P0, X, conn, mat, mu, ka, u, eq, n) + fext
-    assert np.linalg.norm(R) > 1e-6 * np.linalg.norm(fext)
+    u = r["u"]
+    assert equilibrium_residual(O.Q1P0, X, conn, mu, ka, kn, kd, np.zeros(len(kn)), fn, fd, fv, u, 1.0) > \
+        1e-6 * 0.5 * len(fn)
+
+
+def test_static_solve_equilibrium_every_step():
+    """or_static_solve on a uniaxial cube (TL and Q1P0, Simo): after every
+    load step the prescribed dofs hold t_s * value and the residual at the
+    returned displacement meets the Newton test it was given (relative to
+    the step's first residual, which or_static_solve starts from the last
+    converged state)."""
+    m = M.cube(3, seed=4)
+    kbc = M.kbc_uniaxial(m, 0.05)
+    mu, ka = O.mu_kappa(1000.0, 0.3)
+    for kind in (O.TL, O.Q1P0):
+        r = O.static_solve(kind, m.X, m.conn, m.mat, mu, ka, kbc.nodes, kbc.dofs, kbc.values, 3, rel_tol=1e-10,
+                           abs_tol=1e-14)
+        assert r["status"] == 0
+        eq, n = O.equation_numbers(m.nnd, kbc.nodes, kbc.dofs)
+        prev = np.zeros((m.nnd, 3))
+        for s in range(3):
+            t = (s + 1) / 3
+            u = r["step_u"][s]
+            assert np.array_equal(u[kbc.nodes, kbc.dofs], kbc.values * t)
+            start = prev.copy()
+            start[kbc.nodes, kbc.dofs] = kbc.values * t
+            R0 = np.linalg.norm(O.residual(kind, m.X, m.conn, m.mat, mu, ka, start, eq, n))
+            R = np.linalg.norm(O.residual(kind, m.X, m.conn, m.mat, mu, ka, u, eq, n))
+            assert R < 1e-10 * R0, (kind, s, R / R0)
+            prev = u
 
 
 def test_gmres_reference_unit_cases():
