@@ -1,20 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the continuum-solid Newton step on one MI355X.
 
-Workload: 128^3 trilinear Q1 hexahedra (2,097,152 elements) in the unit
-cube, interior nodes perturbed by U(+-5%) of the element size from seed 2024,
-total-Lagrangian compressible neo-Hookean (Simo_isotropic, E = 1000,
-nu = 0.3), z = 0 face fixed, z = 1 face u_z = 0.05 t at load step 1 of 5
-(t = 0.2), FP64 throughout.  One step = one Newton correction from that
-state: residual R, ||R||, stiffness K assembled into the upper MSR matrix,
-expansion to the solver's block operator, Jacobi-PCG to ||r|| <= 1e-10 ||R||,
-u += du.  Every step restarts from the same state (ResetStep + KBC), so all
-steps do identical work.  Mesh, matrix and vectors are resident in HBM
-before the timed region starts.
+Default workload (`cube`): 128^3 trilinear Q1 hexahedra (2,097,152 elements)
+in the unit cube, interior nodes perturbed by U(+-5%) of the element size
+from seed 2024, total-Lagrangian compressible neo-Hookean (Simo_isotropic,
+E = 1000, nu = 0.3), z = 0 face fixed, z = 1 face u_z = 0.05 t at load step
+1 of 5 (t = 0.2), FP64 throughout.
 
+One step = one R + K assembly into the inner solver's operator at the state
+of the first Newton correction: residual R, ||R||, stiffness K (into the MSR
+and expanded, or straight into the block operator), 1/diag.  Mesh, matrix
+and vectors are resident in HBM before the timed region starts.
 value = elements assembled per second = nel / (device time of the R pass +
 device time of the K pass), from HIP events recorded on the solver stream
 around every launch inside the timed region.
+
+Beside the timed loop: one complete Newton correction (the same assembly,
+Jacobi-PCG to 1e-10, u += du) timed once by wall clock (newton_solves_per_s),
+a short event-timed correction for the PCG SpMV's launch duration (the
+roofline line), and the C oracle's R + K passes on a 64^3 sample on the
+host cores (cpu_baseline).  A watchdog prints the line before --time-budget
+seconds even if the long correction is still running.
 """
 from __future__ import annotations
 
@@ -25,6 +31,7 @@ import json
 import os
 import re
 import sys
+import threading
 import time
 
 import numpy as np
@@ -44,8 +51,8 @@ STIFF_FLOP_PER_EL = 83.0e3       # stiffness: 8 x (2*24*6*24 + 2*6*6*24) materia
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=128, help="elements per cube edge")
     ap.add_argument("--lin-rtol", type=float, default=1e-10)
     ap.add_argument("--cpu-n", type=int, default=64, help="edge of the CPU-baseline sample cube (0: skip)")
@@ -60,7 +67,18 @@ def parse_args():
     ap.add_argument("--mixed-checkpoints", type=int, default=10,
                     help="mixed-steps: solve the first K of the 10 load-step checkpoints (default all)")
     ap.add_argument("--profile", action="store_true",
-                    help="short run for profilers: 2 Newton corrections with 48 PCG iterations each")
+                    help="short run for profilers: every SpMV variant timed, the full correction capped at 480 "
+                         "PCG iterations, no CPU baseline")
+    ap.add_argument("--time-budget", type=float, default=420.0,
+                    help="seconds after which the line is printed with what has been measured (0: no limit)")
+    ap.add_argument("--autotune", type=int, choices=[0, 1], default=1,
+                    help="1: time the launch-shape candidates on the loaded problem first (a few seconds)")
+    ap.add_argument("--spmv-sample", type=int, default=256,
+                    help="PCG iterations of the event-timed correction that measures the SpMV launch")
+    ap.add_argument("--correction", type=int, choices=[0, 1], default=1,
+                    help="1: time one complete Newton correction (PCG to --lin-rtol) by wall clock")
+    ap.add_argument("--correction-maxit", type=int, default=200000,
+                    help="PCG iteration cap of the complete correction")
     ap.add_argument("--step-n", type=int, default=128, help="newton-step: cube edge")
     ap.add_argument("--step-stretch", type=float, default=0.005,
                     help="newton-step: top-face displacement of the one load step, fraction of the height")
@@ -158,6 +176,51 @@ def cpu_baseline(n):
     return one, par
 
 
+class Deadline:
+    """Prints the bench line before the time budget runs out.  The line is
+    filled in as the run goes; if the budget expires first (say inside the
+    long Newton correction, which runs in the library with the GIL
+    released), a watchdog thread prints what has been measured so far,
+    marked incomplete, and ends the process."""
+
+    def __init__(self, seconds):
+        self.lock = threading.Lock()
+        self.line = None
+        self.printed = False
+        self.t0 = time.perf_counter()
+        self.seconds = seconds
+        if seconds > 0:
+            threading.Thread(target=self._watch, daemon=True).start()
+
+    def left(self):
+        return self.seconds - (time.perf_counter() - self.t0)
+
+    def emit(self, line=None):
+        with self.lock:
+            if self.printed:
+                return
+            if line is not None:
+                self.line = line
+            if self.line is not None:
+                print(json.dumps(self.line), flush=True)
+                self.printed = True
+
+    def _watch(self):
+        while self.left() > 0:
+            time.sleep(min(1.0, max(0.01, self.left())))
+        with self.lock:
+            if self.printed:
+                return
+            if self.line is None:
+                print(f"bench.py: time budget of {self.seconds:.0f} s spent before the first measurement",
+                      file=sys.stderr, flush=True)
+                os._exit(1)
+            self.line["incomplete"] = f"time budget of {self.seconds:.0f} s reached; later parts not measured"
+            print(json.dumps(self.line), flush=True)
+            self.printed = True
+        os._exit(0)
+
+
 def main():
     faulthandler.enable()   # a host-side crash prints every thread's Python stack
     a = parse_args()
@@ -168,6 +231,9 @@ def main():
         print(json.dumps({"copy_kernel": "k_copy_calib", "bytes_read": nbytes, "bytes_written": nbytes,
                           "launches": 11, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
         return
+    if a.profile:
+        a.correction_maxit = min(a.correction_maxit, 480)
+        a.cpu_n = 0
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); a multi-GPU partition of the "
               "mesh is not built or measured here.", file=sys.stderr)
@@ -180,8 +246,24 @@ def main():
         return bench_rve_serve(a)
     if a.workload == "newton-step":
         return bench_newton_step(a)
+    return bench_cube(a)
+
+
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def bench_cube(a):
+    """The n^3 cube (default 128^3).  Timed step: one R + K assembly into the
+    inner solver's operator at the state of the first Newton correction of
+    load step 1 (R, ||R||, K through the MSR or straight into the block
+    operator, 1/diag).  Outside the timed loop: one complete Newton
+    correction (the same assembly + PCG to lin_rtol + u += du), timed once
+    by wall clock, and a short correction with event timing for the SpMV
+    launch duration."""
     from tahoe_amd import api, mesh as M
 
+    dl = Deadline(a.time_budget)
     n = a.n
     t_setup = time.perf_counter()
     m = M.cube(n, seed=2024)
@@ -194,19 +276,24 @@ def main():
     s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
     s.set_preconditioner(PRECOND[a.precond])
     n_eq, msr_len = s.initialize()
-    print(f"# setup {time.perf_counter() - t_setup:.1f} s: nel={m.nel} nnd={m.nnd} n_eq={n_eq} "
-          f"msr_upper_nnz={msr_len - 1}", file=sys.stderr, flush=True)
-    opts = s.opts(lin_rtol=a.lin_rtol, lin_maxit=48 if a.profile else 200000)
+    log(f"setup {time.perf_counter() - t_setup:.1f} s: nel={m.nel} nnd={m.nnd} n_eq={n_eq} "
+        f"msr_upper_nnz={msr_len - 1}")
     t_load = 0.2
     short = s.opts(lin_rtol=a.lin_rtol, lin_maxit=32)  # two full PCG chunks: no skipped launches
 
-    def short_pcg():
+    def at_state():
+        # the state of the first correction of load step 1: last converged
+        # u (zero) with the prescribed displacements at t = 0.2
         s.reset_step()
         s.apply_kbc(t_load)
+
+    def short_pcg():
+        at_state()
         s.newton_correction(short)
 
     from tahoe_amd import autotune
-    if a.stiff_chunk is not None or a.stiff_shape is not None or a.assembly is not None:
+    t_tune = time.perf_counter()
+    if a.stiff_chunk is not None or a.stiff_shape is not None or a.assembly is not None or not a.autotune:
         s.set_tuning(api.TUNE_STIFF_CHUNK, -1 if a.stiff_chunk is None else a.stiff_chunk)
         if a.stiff_shape is not None:
             s.set_tuning(api.TUNE_STIFF_SHAPE, a.stiff_shape)
@@ -215,60 +302,59 @@ def main():
                  "chunk": s.tuning(api.TUNE_STIFF_CHUNK), "ms": {}}
     else:
         tuned = autotune.tune_stiffness(s, reps=1, run_correction=short_pcg)
-    print(f"# Newton assembly {tuned['assembly']} stiffness shape {tuned['shape']} chunk {tuned['chunk']} "
-          f"(autotune ms: {tuned['ms']})", file=sys.stderr, flush=True)
-    # a profiling run times every variant too, so its trace and counters cover all of them
+    log(f"Newton assembly {tuned['assembly']} stiffness shape {tuned['shape']} chunk {tuned['chunk']} "
+        f"(autotune ms: {tuned['ms']})")
     explicit = a.spmv_xcd is not None or a.pcg_fused is not None
-    tuned_spmv = autotune.tune_spmv_xcd(s, short_pcg) if a.profile or not explicit else {"ms_per_iteration": {}}
+    tuned_spmv = (autotune.tune_spmv_xcd(s, short_pcg) if a.profile or (a.autotune and not explicit)
+                  else {"ms_per_iteration": {}})
     if a.spmv_xcd is not None:
         s.set_tuning(api.TUNE_SPMV_XCD, a.spmv_xcd)
     if a.pcg_fused is not None:
         s.set_tuning(api.TUNE_PCG_FUSED, a.pcg_fused)
     tuned_spmv.update(xcd=s.tuning(api.TUNE_SPMV_XCD), fused=s.tuning(api.TUNE_PCG_FUSED))
-    print(f"# PCG: SpMV variant {tuned_spmv['xcd']}, fused direction {tuned_spmv['fused']} "
-          f"(autotune ms per iteration: {tuned_spmv['ms_per_iteration']})", file=sys.stderr, flush=True)
+    log(f"PCG: SpMV variant {tuned_spmv['xcd']}, fused direction {tuned_spmv['fused']} "
+        f"(autotune ms per iteration: {tuned_spmv['ms_per_iteration']}); tuning {time.perf_counter() - t_tune:.1f} s")
 
     def step():
-        s.reset_step()
-        s.apply_kbc(t_load)
-        return s.newton_correction(opts)
+        s.form_rhs(want=False)      # R and ||R|| (the host reads the norm, as Newton's test does)
+        s.form_tangent()            # K into the solver operator + 1/diag
 
-    if a.profile:
-        s.timing_enable(True)
-        for _ in range(2):
-            s.reset_step()
-            s.apply_kbc(t_load)
-            s.newton_correction(opts)
-        s.synchronize()
-        print(json.dumps({"profile_run": s.timing()}), flush=True)
-        return
-
+    at_state()
     for w in range(a.warmup):
-        st = step()
-        s.synchronize()
-        print(f"# warmup {w}: ||R0||={st.error0:.6e} pcg_its={st.lin_iterations}", file=sys.stderr, flush=True)
+        step()
+    s.synchronize()
+    log(f"warmup: {a.warmup} assembly steps")
 
     s.timing_enable(True)
     s.timing_reset()
     s.synchronize()
-    lin_its = []
     t0 = time.perf_counter()
     for k in range(a.steps):
-        st = step()
-        lin_its.append(st.lin_iterations)
+        step()
     s.synchronize()
     t1 = time.perf_counter()
     tm = s.timing()
     s.timing_enable(False)
-    u = s.displacement()
-    assert np.all(np.isfinite(u))
-
     K = a.steps
     ms_per_step = (t1 - t0) * 1e3 / K
     r_ms = tm["residual_ms"] / K
     k_ms = tm["stiffness_ms"] / K
+    e_ms = tm["expand_ms"] / K
     value = m.nel / ((r_ms + k_ms) * 1e-3)
-    spmv_avg_ms = tm["spmv_ms"] / max(1, tm["spmv_launches"])
+    log(f"{K} assembly steps: {ms_per_step:.2f} ms per step (R {r_ms:.3f} ms, K {k_ms:.3f} ms, "
+        f"operator/diagonal {e_ms:.3f} ms) -> {value / 1e6:.1f} M elements/s")
+
+    # SpMV launch duration: a short correction with every launch event-timed
+    at_state()
+    s.timing_enable(True)
+    s.timing_reset()
+    st_s = s.newton_correction(s.opts(lin_rtol=a.lin_rtol, lin_maxit=a.spmv_sample))
+    s.synchronize()
+    ts = s.timing()
+    s.timing_enable(False)
+    spmv_avg_ms = ts["spmv_ms"] / max(1, ts["spmv_launches"])
+    its_s = max(1, ts["spmv_launches"])
+    pcg_it_ms = (ts["spmv_ms"] + ts["vector_ms"] + ts["precond_ms"]) / its_s
     nblocks = (n + 1 + 2 * n) ** 3  # (3n+1)^3 node-neighbour blocks of the structured cube
     vbytes = 8 if a.precision == "fp64" else 4
     fused = s.tuning(api.TUNE_PCG_FUSED) == 1 and a.precond != "block"
@@ -281,11 +367,13 @@ def main():
     profiled = n == 128 and a.precision == "fp64" and a.precond == "jacobi"
     spmv_bytes = nblocks * (9 * vbytes + 4) + node_vectors * 3 * 8 * m.nnd
     spmv_gbs = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9
+    log(f"SpMV {kname}: {spmv_avg_ms:.4f} ms per launch over {ts['spmv_launches']} launches "
+        f"-> {spmv_gbs:.0f} GB/s algorithmic")
     res_gflops = RES_FLOP_PER_EL * m.nel / (r_ms * 1e-3) / 1e9
     stf_gflops = STIFF_FLOP_PER_EL * m.nel / (k_ms * 1e-3) / 1e9
-    its = float(np.mean(lin_its))
-    pcg_ms = (tm["spmv_ms"] + tm["vector_ms"] + tm["precond_ms"]) / K
-    cpu_lines = cpu_baseline(a.cpu_n) if a.cpu_n > 0 else (None, None)
+    # least HBM bytes of a stiffness pass: every MSR value written once, the
+    # element's connectivity and its nodes' X and u read once
+    stf_bytes = msr_len * (8 if a.precision == "fp64" else 4) + 32 * m.nel + 48 * m.nnd
     line = {
         "metric": METRIC,
         "value": value,
@@ -300,36 +388,74 @@ def main():
         "dtype": "f64" if a.precision == "fp64" else "f32/f64",
         "data": "synthetic",
         "config": {"workload": f"{n}^3 Q1 hex TL neo-Hookean (Simo_isotropic E=1000 nu=0.3), +-5% perturbed, "
-                               "uniaxial 5% load step 1/5, one Newton correction: R + K into the solver operator + "
-                               f"{PC_NAME[a.precond]}-PCG to {a.lin_rtol:g}",
+                               "uniaxial 5% load step 1/5; step = one R + K assembly into the solver operator at "
+                               "the first Newton correction's state; one full Newton correction ("
+                               f"{PC_NAME[a.precond]}-PCG to {a.lin_rtol:g}) timed once beside it",
                    "elements": m.nel, "nodes": m.nnd, "equations": n_eq, "msr_upper_nnz": msr_len - 1,
                    "newton_assembly": "operator" if tuned["assembly"] == 1 else "msr+expansion",
                    "stiffness_shape_elements_per_workgroup": api.STIFF_SHAPE_EPB[tuned["shape"]],
-                   "stiffness_chunk": tuned["chunk"], "spmv_variant": tuned_spmv["xcd"], "pcg_fused": bool(fused)},
+                   "stiffness_chunk": tuned["chunk"], "spmv_variant": tuned_spmv["xcd"], "pcg_fused": bool(fused),
+                   "precision": a.precision, "preconditioner": a.precond},
+        "value_is": "elements / (residual pass + stiffness pass device time), HIP events on the solver stream",
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": spmv_gbs / (HBM_PEAK / 1e9),
                      "traffic": traffic_from_profiles(kname) if profiled else None,
                      "kernel": kname + (" (3x3-block sliced-ELL SpMV with the direction update p = z + beta p_old "
                                         "formed inside, + per-slice p.q partials)" if fused else
                                         " (3x3-block sliced-ELL SpMV + per-slice p.q partials)"),
-                     "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_avg_ms},
-        "cpu_baseline": cpu_lines[0],
-        "cpu_baseline_all_threads": cpu_lines[1],
-        "newton_solves_per_s": 1e3 / ms_per_step,
-        "elements_per_s_tangent_in_solver_operator": m.nel / ((r_ms + k_ms + tm["expand_ms"] / K) * 1e-3),
-        "residual": {"ms": r_ms, "elements_per_s": m.nel / (r_ms * 1e-3), "gflops": res_gflops,
-                     "frac_fp64_peak": res_gflops * 1e9 / FP64_PEAK},
-        "stiffness": {"ms": k_ms, "elements_per_s": m.nel / (k_ms * 1e-3), "gflops": stf_gflops,
-                      "frac_fp64_peak": stf_gflops * 1e9 / FP64_PEAK},
-        "pcg": {"iterations_per_step": its, "ms_per_step": pcg_ms,
-                "dof_iterations_per_s": n_eq * its / (pcg_ms * 1e-3) if pcg_ms > 0 else None,
-                "expand_ms": tm["expand_ms"] / K, "preconditioner_ms": tm["precond_ms"] / K,
-                "preconditioner": a.precond},
+                     "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_avg_ms,
+                     "launches_timed": ts["spmv_launches"],
+                     "why_this_kernel": "the PCG SpMV is the dominant kernel of a Newton solve (85% of a PCG "
+                                        "iteration, ~37k iterations per correction at 128^3)"},
+        "roofline_assembly": {
+            "stiffness": {"bound": "mfma", "achieved": stf_gflops / 1e3, "peak": FP64_PEAK / 1e12,
+                          "unit": "TFLOP/s", "frac": stf_gflops * 1e9 / FP64_PEAK, "avg_pass_ms": k_ms,
+                          "flop_per_element": STIFF_FLOP_PER_EL,
+                          "hbm_GBps_min_bytes": stf_bytes / (k_ms * 1e-3) / 1e9,
+                          "note": "FP64 dense 24x24 B^T C B count; FP64 vector and matrix peaks are equal on MI355X"},
+            "residual": {"bound": "fp64", "achieved": res_gflops / 1e3, "peak": FP64_PEAK / 1e12,
+                         "unit": "TFLOP/s", "frac": res_gflops * 1e9 / FP64_PEAK, "avg_pass_ms": r_ms,
+                         "flop_per_element": RES_FLOP_PER_EL},
+        },
+        "elements_per_s_tangent_in_solver_operator": m.nel / ((r_ms + k_ms + e_ms) * 1e-3),
+        "assembly_ms": {"residual": r_ms, "stiffness": k_ms, "operator_and_diagonal": e_ms,
+                        "wall_per_step": ms_per_step},
+        "pcg_iteration_ms": pcg_it_ms,
+        "dof_iterations_per_s": n_eq / (pcg_it_ms * 1e-3),
         "timing_ms": tm,
         "autotune_tangent_ms": {k: {str(x): y for x, y in v.items()} for k, v in tuned["ms"].items()},
         "autotune_pcg_ms_per_iteration": tuned_spmv["ms_per_iteration"],
     }
-    print(json.dumps(line), flush=True)
+    dl.line = line
+    if a.cpu_n > 0:
+        one, par = cpu_baseline(a.cpu_n)
+        line["cpu_baseline"] = one
+        line["cpu_baseline_all_threads"] = par
+        log(f"CPU baseline: {one['value']:.0f} elements/s on 1 thread, {par['value']:.0f} on {par['cores']}")
+    else:
+        line["cpu_baseline"] = None
+    if a.correction:
+        # one complete Newton correction from the same state, graph-replayed
+        # PCG (the default path), by wall clock
+        at_state()
+        s.synchronize()
+        log(f"full Newton correction ({PC_NAME[a.precond]}-PCG to {a.lin_rtol:g}, at most {a.correction_maxit} "
+            f"iterations); {dl.left():.0f} s of the time budget left")
+        t2 = time.perf_counter()
+        st = s.newton_correction(s.opts(lin_rtol=a.lin_rtol, lin_maxit=a.correction_maxit))
+        s.synchronize()
+        wall = time.perf_counter() - t2
+        u = s.displacement()
+        assert np.all(np.isfinite(u))
+        its = int(st.lin_iterations)
+        pcg_s = max(1e-9, wall - (r_ms + k_ms + e_ms) * 1e-3)
+        line["newton_solves_per_s"] = 1.0 / wall
+        line["newton_correction"] = {
+            "wall_s": wall, "pcg_iterations": its, "converged": its < a.correction_maxit,
+            "error0": st.error0, "pcg_s": pcg_s, "dof_iterations_per_s": n_eq * its / pcg_s,
+            "note": "R, ||R||, K into the operator, PCG (HIP-graph replayed chunks), u += du; wall clock"}
+        log(f"correction: {wall:.2f} s, {its} PCG iterations")
+    dl.emit()
 
 
 def base_line(value, unit, steps, warmup, ms_per_step, dtype, config, higher=True):

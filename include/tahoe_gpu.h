@@ -184,6 +184,14 @@ int tg_form_rhs(tg_ctx* ctx, double* R_out, double* norm_out);
  * SolidElementT::ElementLHSDriver SolidElementT.cpp:1100 -> MSRMatrixT::Assemble
  * MSRMatrixT.cpp:66-165, after GlobalMatrixT::Clear). */
 int tg_form_lhs(tg_ctx* ctx);
+/* The tangent half of a Newton correction: K(u) into the inner solver's
+ * operator (through the MSR or straight into the block operator, as
+ * TG_TUNE_NEWTON_ASSEMBLY selects) and the preconditioner's data for it --
+ * NLSolver::Solve's fFEManager.FormLHS (NLSolver.cpp:155) followed by the
+ * factorisation half of fLHS->Solve (NLSolver.cpp:762 -> GlobalMatrixT::
+ * Factorize, DiagonalMatrixT::Factorize DiagonalMatrixT.cpp:267-324 for
+ * Jacobi).  A following tg_newton_correction forms its own tangent again. */
+int tg_form_tangent(tg_ctx* ctx);
 
 /* ---- global matrix ---------------------------------------------------- */
 /* MSR values val[msr_length] (the data GenerateRCV/PrintLHS dump, MSRMatrixT.cpp:449-524). */

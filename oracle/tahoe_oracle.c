@@ -423,12 +423,10 @@ typedef struct {
  * (SimoQ1P0.cpp:359-387): current-configuration derivatives, Hughes (4.5.23)
  * mean gradient (SetMeanGradient, SimoQ1P0.cpp:661-686), and
  * F_bar = (v/(H J))^(1/3) F. */
-static int q1p0_state(const double* Xe, const double* ue, q1p0_state_t* st, int need_ref)
+static int q1p0_state(const double* Xe, const double* xe, const double* ue, q1p0_state_t* st, int need_ref)
 {
     const parent_t* P = parent();
     if (need_ref && shape_derivs(Xe, st->det0, st->dNX)) return 7;
-    double xe[24];
-    for (int k = 0; k < 24; k++) xe[k] = Xe[k] + ue[k];
     if (shape_derivs(xe, st->det, st->dnx)) return 7;
     st->H = 0.0;
     st->v = 0.0;
@@ -494,23 +492,35 @@ static int q1p0_element(stress_fn cauchy, const double Xe[24], const double ue[2
                         double Re[24], double Ke[576])
 {
     q1p0_state_t st;
-    if (q1p0_state(Xe, ue, &st, 1)) return 7;
+    /* current coordinates x = X + u, formed once per node as the node
+     * manager's current coordinates are */
+    double xe[24], uw[24];
+    for (int k = 0; k < 24; k++) {
+        xe[k] = Xe[k] + ue[k];
+        uw[k] = ue[k];
+    }
+    if (q1p0_state(Xe, xe, uw, &st, 1)) return 7;
     if (Re && q1p0_force(cauchy, &st, mu, kappa, -1.0, Re)) return 7;
     if (Ke) {
         const double eps = 1.0e-7;
-        double f0[24], fp[24], up[24];
+        double f0[24], fp[24];
         if (q1p0_force(cauchy, &st, mu, kappa, 1.0, f0)) return 7;
         memset(Ke, 0, 576 * sizeof(double));
         for (int col = 0; col < 24; col++) {
-            memcpy(up, ue, sizeof up);
-            up[col] += eps;
+            /* fLocDisp(a,i) += eps and fLocCurrCoords(a,i) += eps, undone
+             * by subtracting eps after the column (SimoQ1P0.cpp:493-513):
+             * the perturbed x is (X + u) + eps, and an entry keeps whatever
+             * rounding (v + eps) - eps leaves for the later columns */
+            uw[col] += eps;
+            xe[col] += eps;
             q1p0_state_t sp;
             memcpy(sp.det0, st.det0, sizeof st.det0);
             memcpy(sp.dNX, st.dNX, sizeof st.dNX);
-            /* current coordinates are X + u: perturbing u perturbs x by the same eps */
-            if (q1p0_state(Xe, up, &sp, 0)) return 7;
+            if (q1p0_state(Xe, xe, uw, &sp, 0)) return 7;
             if (q1p0_force(cauchy, &sp, mu, kappa, 1.0, fp)) return 7;
             for (int j = 0; j < 24; j++) Ke[24 * j + col] += (fp[j] - f0[j]) / eps;
+            uw[col] -= eps;
+            xe[col] -= eps;
         }
     }
     return 0;

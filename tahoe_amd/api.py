@@ -35,7 +35,7 @@ SYMBOLS = [
     "tg_set_material_model", "tg_set_precision", "tg_set_preconditioner", "tg_set_linear_solver", "tg_set_kbc", "tg_set_forces",
     "tg_apply_forces", "tg_set_segments", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
     "tg_get_displacement", "tg_apply_kbc", "tg_close_step", "tg_reset_step", "tg_form_rhs", "tg_form_lhs",
-    "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_batch", "tg_newton_correction",
+    "tg_form_tangent", "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_batch", "tg_newton_correction",
     "tg_synchronize", "tg_timing_enable", "tg_timing_reset", "tg_timing_get", "tg_calib_copy", "tg_set_tuning",
     "tg_get_tuning",
 ]
@@ -116,6 +116,7 @@ def lib():
         L.tg_reset_step.argtypes = [vp]
         L.tg_form_rhs.argtypes = [vp, D, D]
         L.tg_form_lhs.argtypes = [vp]
+        L.tg_form_tangent.argtypes = [vp]
         L.tg_get_msr_values.argtypes = [vp, D]
         L.tg_multx.argtypes = [vp, D, D]
         L.tg_copy_diagonal.argtypes = [vp, D]
@@ -289,6 +290,11 @@ class Solid:
 
     def form_lhs(self):
         self._ok(lib().tg_form_lhs(self._h))
+
+    def form_tangent(self):
+        """K(u) into the inner solver's operator plus the preconditioner's
+        data (the tangent half of a Newton correction)."""
+        self._ok(lib().tg_form_tangent(self._h))
 
     def msr_values(self):
         v = np.zeros(self.msr_len)

@@ -725,7 +725,7 @@ static void launch_stiffness_shape(tg_ctx* c, int v, int first, int count, const
 void launch_stiffness(tg_ctx* c, const double* u)
 {
     TimedRegion tr(c, T_STIFFNESS);
-    const bool mixed = c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN;
+    const bool mixed = mixed_stiffness(c);
     if (mixed)
         TG_HIP(hipMemsetAsync(c->dval32.p, 0, sizeof(float) * (size_t)c->msr_len, c->stream));
     else

@@ -294,7 +294,7 @@ static void gmres_t(tg_ctx* c, const MT* val, const double* b, double* x, double
 void gmres_solve(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
 {
     if (c->seg.nseg != 1) fail(TG_BAD_INPUT, "GMRES needs a single problem (no segments)");
-    if (c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN)
+    if (mixed_operator(c))
         gmres_t<float>(c, c->op.d_val32.p, b, x, rtol, maxit, st);
     else
         gmres_t<double>(c, c->op.d_val.p, b, x, rtol, maxit, st);

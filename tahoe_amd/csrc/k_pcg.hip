@@ -545,14 +545,16 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
     // segments' work a no-op).  The graph holds kernel nodes only.
     const bool graph = c->pcg_graph && !c->timing.enabled;
     if (graph) {
-        const void* key[12] = {val, dinv, x, r, p, q, S.d_part.p, S.d_scal.p, S.d_ndone.p,
+        // every address and size the chunk's launches take (tg_initialize
+        // also drops the graph, as buffers may come back at old addresses)
+        const void* key[24] = {val, dinv, x, r, p, q, S.d_part.p, S.d_scal.p, S.d_ndone.p,
                                (const void*)(long)(PC + 4 * c->spmv_xcd + 16 * (int)fused + 32L * c->amg.gen),
-                               (const void*)(long)sizeof(MT), (const void*)(long)c->nnd};
+                               (const void*)(long)sizeof(MT), (const void*)(long)c->nnd, (const void*)(long)nseg,
+                               (const void*)ns, S.d_seg_slice.p, S.d_slice_seg.p, c->op.d_slice_base.p,
+                               c->op.d_slice_width.p, c->op.d_col.p, pbuf[1], c->vbinv.p,
+                               c->amg.lv.empty() ? nullptr : (const void*)c->amg.lv[0]};
         if (!c->pg.exec || std::memcmp(key, c->pg.key, sizeof key) != 0) {
-            if (c->pg.exec) TG_HIP(hipGraphExecDestroy(c->pg.exec));
-            if (c->pg.graph) TG_HIP(hipGraphDestroy(c->pg.graph));
-            c->pg.exec = nullptr;
-            c->pg.graph = nullptr;
+            drop_pcg_graph(c);
             TG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             issue_chunk();
             TG_HIP(hipStreamEndCapture(c->stream, &c->pg.graph));

@@ -335,7 +335,7 @@ void launch_block_inverse(tg_ctx* c)
     if (c->precond != TG_PC_BLOCK_JACOBI) return;
     TimedRegion tr(c, T_EXPAND);
     const unsigned blocks = (unsigned)((c->nnd + VB - 1) / VB);
-    const bool f32 = c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN;
+    const bool f32 = mixed_operator(c);
     if (f32)
         hipLaunchKernelGGL(k_binv_op<float>, dim3(blocks), dim3(VB), 0, c->stream, c->nnd, c->deqn.p,
                            c->op.d_diag.p, c->op.d_val32.p, c->vbinv.p);

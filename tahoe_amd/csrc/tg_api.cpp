@@ -140,6 +140,15 @@ static void setup_segments(tg_ctx* c)
     TG_HIP(hipStreamSynchronize(c->stream));
 }
 
+void drop_pcg_graph(tg_ctx* c)
+{
+    if (c->pg.exec) (void)hipGraphExecDestroy(c->pg.exec);
+    if (c->pg.graph) (void)hipGraphDestroy(c->pg.graph);
+    c->pg.exec = nullptr;
+    c->pg.graph = nullptr;
+    std::memset(c->pg.key, 0, sizeof c->pg.key);
+}
+
 static void check_status(tg_ctx* c)
 {
     TG_HIP(hipMemcpyAsync(c->hstatus, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -222,8 +231,7 @@ int tg_destroy(tg_ctx* c)
             g_events.erase(it);
         }
     }
-    if (c->pg.exec) (void)hipGraphExecDestroy(c->pg.exec);
-    if (c->pg.graph) (void)hipGraphDestroy(c->pg.graph);
+    drop_pcg_graph(c);
     for (hipEvent_t e : c->poll_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->hscal) (void)hipHostFree(c->hscal);
@@ -385,7 +393,13 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
             if (c->mu[m] < 0.0) fail(TG_BAD_INPUT, "element refers to an undefined material");
         for (int k : c->fbc_idx)
             if (k >= 3 * c->nnd) fail(TG_OUT_OF_RANGE, "force card out of range");
+        // kinematic cards were checked against the mesh present when they
+        // were set; a mesh set afterwards may have fewer nodes
+        for (int kn : c->kbc_nodes)
+            if (kn < 0 || kn >= c->nnd) fail(TG_OUT_OF_RANGE, "KBC node out of range");
         c->initialized = false;
+        // a captured PCG chunk refers to buffers that are about to be replaced
+        drop_pcg_graph(c);
         c->dfext.release();
         // TG_SETUP_TRACE=1 prints the host-side setup phases to stderr
         static const bool trace = std::getenv("TG_SETUP_TRACE") != nullptr;
@@ -424,8 +438,8 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         c->dval.alloc((size_t)c->msr_len);
         c->dslot.alloc(300 * (size_t)c->nel);
         for (auto* v : {&c->vx, &c->vr, &c->vz, &c->vp, &c->vq, &c->vdinv, &c->vwork}) v->alloc(nn);
-        const bool mixed = c->precision == TG_MIXED;
-        c->dval32.alloc(mixed ? (size_t)c->msr_len : 0);
+        const bool mixed = mixed_operator(c);
+        c->dval32.alloc(mixed_stiffness(c) ? (size_t)c->msr_len : 0);
         const size_t sblocks = ((size_t)c->nnd + 255) / 256;
         c->dpart.alloc(2 * std::max<size_t>(sblocks, 1024) + 16);
         std::vector<int> eqn0(nn);
@@ -616,13 +630,12 @@ int tg_form_lhs(tg_ctx* c)
     });
 }
 
-static bool mixed_values(const tg_ctx* c) { return c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN; }
-
-// FP64 copy of the assembled values: in mixed precision the stiffness is
-// assembled in FP32 and widened here for the GlobalMatrixT queries.
+// FP64 copy of the assembled values: in mixed precision the total-Lagrangian
+// stiffness is assembled in FP32 and widened here for the GlobalMatrixT
+// queries.
 static void fp64_values(tg_ctx* c)
 {
-    if (mixed_values(c)) widen_values(c);
+    if (mixed_stiffness(c)) widen_values(c);
 }
 
 // the preconditioner's data for the current operator values
@@ -630,18 +643,22 @@ static void prepare_preconditioner(tg_ctx* c)
 {
     launch_block_inverse(c);
     if (c->precond == TG_PC_AMG) {
-        if (!mixed_values(c)) launch_narrow_operator(c);
+        if (!mixed_operator(c)) launch_narrow_operator(c);
         amg_numeric(c);
     }
 }
 
-// operator of the inner linear solve, in the solve's precision
+// operator of the inner linear solve, in the solve's precision: the FP32
+// total-Lagrangian stiffness is expanded as it is; the FP64 Q1P0 stiffness
+// is expanded (1/diag from the FP64 diagonal) and then narrowed
 static void prepare_operator(tg_ctx* c)
 {
-    if (mixed_values(c))
+    if (mixed_stiffness(c)) {
         launch_expand32(c);
-    else
+    } else {
         launch_expand(c);
+        if (mixed_operator(c)) launch_narrow_operator(c);
+    }
     prepare_preconditioner(c);
 }
 
@@ -659,13 +676,22 @@ static void newton_operator(tg_ctx* c)
     }
 }
 
+int tg_form_tangent(tg_ctx* c)
+{
+    if (!c) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        newton_operator(c);
+    });
+}
+
 static void solve_linear(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st)
 {
     if (c->linear_solver == TG_LIN_GMRES) {
         gmres_solve(c, b, x, rtol, maxit, st);
         return;
     }
-    if (mixed_values(c))
+    if (mixed_operator(c))
         pcg_solve_mixed(c, b, x, rtol, maxit, st);
     else
         pcg_solve(c, b, x, rtol, maxit, st);

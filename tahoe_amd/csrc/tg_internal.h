@@ -226,7 +226,7 @@ struct tg_ctx {
     struct {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        const void* key[12] = {};
+        const void* key[24] = {};
     } pg;                               // the captured chunk and the arguments it was captured with
     hipEvent_t poll_ev[2] = {nullptr, nullptr};
     long stiff_chunk_built = -2;
@@ -268,6 +268,13 @@ struct tg_ctx {
 
 namespace tg {
 
+// Mixed precision: the total-Lagrangian stiffness is assembled in FP32; the
+// Q1P0 stiffness (a forward difference of the FP64 internal force) stays
+// FP64 and is narrowed when it becomes the inner solve's operator, which is
+// FP32 for every element in mixed precision.
+inline bool mixed_stiffness(const tg_ctx* c) { return c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN; }
+inline bool mixed_operator(const tg_ctx* c) { return c->precision == TG_MIXED; }
+
 // host setup (tg_setup.cpp)
 int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
 void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist);
@@ -306,6 +313,8 @@ void eq_to_node(tg_ctx* c, const double* src_eq, double* dst_node);
 void node_to_eq(tg_ctx* c, const double* src_node, double* dst_eq);
 void launch_axpy(tg_ctx* c, double* y, const double* x, double a, long n);
 void launch_apply_kbc(tg_ctx* c, double t);
+// releases the captured PCG chunk (its kernels hold buffer addresses and sizes)
+void drop_pcg_graph(tg_ctx* c);
 
 // event-timed launch region
 struct TimedRegion {

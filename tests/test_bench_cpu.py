@@ -37,3 +37,26 @@ def test_cpu_baselines_small_sample():
     assert one["cores"] == 1 and one["kind"] == "port" and one["value"] > 0
     assert par["cores"] == bench.cpu_threads() and par["value"] > 0
     assert "8^3" in one["sample"] and "colours" in par["sample"]
+
+
+def test_deadline_prints_partial_line_once(tmp_path):
+    """The watchdog prints the line measured so far, marked incomplete, when
+    the time budget runs out inside a long call, and the process ends with
+    status 0; a line printed in time is printed once."""
+    import json
+    import subprocess
+    code = ("import time, bench\n"
+            "d = bench.Deadline(1.0)\n"
+            "d.line = {'value': 1.0}\n"
+            "time.sleep(30)\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = [json.loads(x) for x in r.stdout.splitlines() if x.startswith("{")]
+    assert len(out) == 1 and out[0]["value"] == 1.0 and "incomplete" in out[0]
+    code = ("import time, bench\n"
+            "d = bench.Deadline(2.0)\n"
+            "d.emit({'value': 2.0})\n"
+            "time.sleep(3)\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=60)
+    out = [json.loads(x) for x in r.stdout.splitlines() if x.startswith("{")]
+    assert r.returncode == 0 and out == [{"value": 2.0}]

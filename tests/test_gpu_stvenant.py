@@ -174,3 +174,5 @@ def test_q1p0_newton_both_precisions():
         scale = np.abs(u).max()
         assert np.abs(out[0][k - 1] - u).max() <= 1e-9 * scale, k
         assert np.abs(out[1][k - 1] - out[0][k - 1]).max() <= 1e-9 * scale, k
+    # the mixed leg ran its own (FP32-operator) arithmetic
+    assert not np.array_equal(out[1], out[0])

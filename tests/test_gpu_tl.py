@@ -474,3 +474,88 @@ def test_input_errors_are_reported():
     uf, itf = api.static_solve(f, 1, f.opts())
     f.close()
     assert iters[0] > 0 and list(iters) == list(itf) and np.array_equal(us, uf)
+
+
+def test_kbc_rechecked_against_a_later_mesh():
+    """Kinematic cards set on one mesh and a smaller mesh set afterwards:
+    tg_initialize rejects the cards that name missing nodes (kOutOfRange)
+    instead of numbering equations past the end of the node table."""
+    from tahoe_amd import api, mesh as M
+    big, small = M.cube(3, seed=1), M.cube(2, seed=1)
+    kbc = M.kbc_uniaxial(big, 0.05)
+    assert kbc.nodes.max() >= small.nnd
+    s = api.Solid(0)
+    s.set_mesh(big.X, big.conn, big.mat)
+    s.set_material(0, *api.mu_kappa(1000.0, 0.3))
+    s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
+    s.set_mesh(small.X, small.conn, small.mat)
+    with pytest.raises(api.TahoeGpuError) as e:
+        s.initialize()
+    assert e.value.code == 4
+    ks = M.kbc_uniaxial(small, 0.05)
+    s.set_kbc(ks.nodes, ks.dofs, ks.values)
+    s.initialize()
+    s.close()
+
+
+def test_reinitialized_context_solves_like_a_fresh_one():
+    """A context solved once (its PCG chunk captured as a graph) and then
+    re-initialized on another mesh with the same node count but other
+    segments solves exactly as a fresh context does: the captured chunk is
+    rebuilt for the new buffers and sizes, not replayed."""
+    from tahoe_amd import api, mesh as M
+    m1, m2 = M.cube(4, seed=3), M.cube(4, seed=4)
+    kbc = M.kbc_uniaxial(m1, 0.05)
+
+    def define(s, m, segs=None):
+        s.set_mesh(m.X, m.conn, m.mat)
+        s.set_material(0, *api.mu_kappa(1000.0, 0.3))
+        s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
+        s.initialize()
+
+    s = api.Solid(0)
+    define(s, m1)
+    api.static_solve(s, 1, s.opts())
+    define(s, m2)
+    u_re, it_re = api.static_solve(s, 2, s.opts())
+    s.close()
+    f = api.Solid(0)
+    define(f, m2)
+    u_f, it_f = api.static_solve(f, 2, f.opts())
+    f.close()
+    assert list(it_re) == list(it_f) and np.array_equal(u_re, u_f)
+
+
+def test_form_tangent_assembles_newtons_tangent():
+    """tg_form_tangent forms K(u) for the inner solve: through the MSR
+    (assembly 0), whose values then equal the reference's K at that iterate,
+    or straight into the block operator (assembly 1); either way it launches
+    the stiffness pass and the operator/diagonal pass and leaves a following
+    correction's result unchanged."""
+    from tahoe_amd import api
+    g = load_golden("tl_n4")
+    n = None
+    for assembly in (0, 1):
+        s = make_solid(g)
+        n = s.n_eq
+        s.set_tuning(api.TUNE_NEWTON_ASSEMBLY, assembly)
+        s.set_displacement(g["it_u"][1])
+        s.timing_enable(True)
+        s.timing_reset()
+        s.form_tangent()
+        t = s.timing()
+        s.timing_enable(False)
+        assert t["stiffness_launches"] > 0 and t["expand_launches"] > 0
+        if assembly == 0:
+            A = msr_dense(s.msr_bindx(), s.msr_values(), n)
+            K = golden_K_dense(g, 1, n)
+            assert np.abs(A - K).max() <= 1e-12 * np.abs(K).max()
+        st = s.newton_correction(s.opts(lin_rtol=1e-12))
+        u1 = s.displacement()
+        s.close()
+        f = make_solid(g)
+        f.set_tuning(api.TUNE_NEWTON_ASSEMBLY, assembly)
+        f.set_displacement(g["it_u"][1])
+        st_f = f.newton_correction(f.opts(lin_rtol=1e-12))
+        assert st.lin_iterations == st_f.lin_iterations and np.array_equal(u1, f.displacement())
+        f.close()

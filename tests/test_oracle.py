@@ -159,7 +159,11 @@ def test_stiffness_at_reference_iterates(name):
     n = int(g["eqnos"].max())
     b = O.msr_pattern(n, g["conn"], g["eqnos"])
     K0 = np.abs(golden_K_dense(g, 0, n)).max()
-    tol = 1e-12 if kind_of(g) == O.TL else 1e-8  # Q1P0: forward-difference tangent, see test_gpu_q1p0
+    # Q1P0: the reference's forward-difference tangent carries the rounding
+    # of its internal force amplified by 1/eps = 1e7; with the same
+    # perturbation ((X + u) + eps, undone by - eps) the oracle is within
+    # 1.2e-10 .. 9.9e-10 of max|K| (measured); bound 2x that
+    tol = 1e-12 if kind_of(g) == O.TL else 2e-9
     for k in range(len(g["lhs_ptr"]) - 1):
         val = O.stiffness(kind_of(g), g["X"], g["conn"], g["mat"], mu, ka, g["it_u"][k], g["eqnos"], n, b)
         err = np.abs(msr_dense(b, val, n) - golden_K_dense(g, k, n)).max() / K0
