@@ -109,7 +109,9 @@ int tg_set_formulation(tg_ctx* ctx, int formulation);
 enum { TG_MAT_SIMO_ISOTROPIC = 0, TG_MAT_STVENANT = 1 };
 int tg_set_material_model(tg_ctx* ctx, int model);
 /* TG_FP64 (default) or TG_MIXED; no counterpart in the reference, which is
- * FP64 only.  TG_MIXED applies to the total-Lagrangian element. */
+ * FP64 only.  TG_MIXED: the total-Lagrangian stiffness is assembled in
+ * FP32; the Q1P0 forward-difference stiffness is formed in FP64 and
+ * narrowed to the FP32 inner operator. */
 int tg_set_precision(tg_ctx* ctx, int precision);
 /* Preconditioner of the inner PCG: TG_PC_JACOBI (default) is the
  * reference's DiagonalMatrixT (1/diag, DiagonalMatrixT.cpp:267-324);
@@ -203,6 +205,23 @@ int tg_copy_diagonal(tg_ctx* ctx, double* diag);
 /* Solve K x = b by Jacobi-PCG; rhs holds b on entry and x on exit
  * (GlobalMatrixT::Solve, GlobalMatrixT.cpp:77-114). */
 int tg_solve(tg_ctx* ctx, double* rhs, double rtol, int maxit, tg_lin_stats* stats);
+
+/* tg_solve that allocates nothing -- no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  b and x are DEVICE
+ * arrays [n_eq] in equation space (x is written, b only read); the matrix is
+ * the context's MSR from the last tg_form_lhs, as for tg_solve.  The PCG's
+ * vectors, reduction partials, scalar slots and done count live in one
+ * static device array of TG_NOALLOC_SCRATCH_BYTES per GPU, kept until the
+ * process ends; a problem needing more (about 144 bytes per node) is
+ * rejected with TG_SIZE_MISMATCH.  Supports TG_LIN_PCG with the Jacobi and
+ * block-Jacobi preconditioners (TG_PC_AMG keeps its own buffers: rejected,
+ * TG_BAD_INPUT); every launch is direct (no graph), and no event timing is
+ * recorded during the call.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (two threads, two contexts) race and must
+ * be serialised by the caller.  Returns after the result is in x. */
+#define TG_NOALLOC_SCRATCH_BYTES (64L << 20)
+int tg_solve_noalloc(tg_ctx* ctx, const double* b_dev, double* x_dev, double rtol, int maxit, tg_lin_stats* stats);
 
 /* ---- solver ----------------------------------------------------------- */
 /* One load step's Newton loop, device resident: R, ||R||, K, PCG, u += du

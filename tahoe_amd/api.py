@@ -35,7 +35,7 @@ SYMBOLS = [
     "tg_set_material_model", "tg_set_precision", "tg_set_preconditioner", "tg_set_linear_solver", "tg_set_kbc", "tg_set_forces",
     "tg_apply_forces", "tg_set_segments", "tg_initialize", "tg_get_eqnos", "tg_get_msr_bindx", "tg_set_displacement",
     "tg_get_displacement", "tg_apply_kbc", "tg_close_step", "tg_reset_step", "tg_form_rhs", "tg_form_lhs",
-    "tg_form_tangent", "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_newton", "tg_newton_batch", "tg_newton_correction",
+    "tg_form_tangent", "tg_get_msr_values", "tg_multx", "tg_copy_diagonal", "tg_solve", "tg_solve_noalloc", "tg_newton", "tg_newton_batch", "tg_newton_correction",
     "tg_synchronize", "tg_timing_enable", "tg_timing_reset", "tg_timing_get", "tg_calib_copy", "tg_set_tuning",
     "tg_get_tuning",
 ]
@@ -121,6 +121,7 @@ def lib():
         L.tg_multx.argtypes = [vp, D, D]
         L.tg_copy_diagonal.argtypes = [vp, D]
         L.tg_solve.argtypes = [vp, D, C.c_double, C.c_int, C.POINTER(LinStats)]
+        L.tg_solve_noalloc.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.POINTER(LinStats)]
         L.tg_newton.argtypes = [vp, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)]
         L.tg_newton_correction.argtypes = [vp, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)]
         L.tg_newton_batch.argtypes = [vp, C.POINTER(NewtonOpts), C.POINTER(NewtonStats)]
@@ -317,6 +318,13 @@ class Solid:
         st = LinStats()
         self._ok(lib().tg_solve(self._h, _d(x), rtol, maxit, C.byref(st)))
         return x, st
+
+    def solve_noalloc(self, b_dev, x_dev, rtol=1e-12, maxit=100000):
+        """tg_solve_noalloc: b_dev and x_dev are device addresses (ints) of
+        [n_eq] FP64 arrays the caller owns; the library allocates nothing."""
+        st = LinStats()
+        self._ok(lib().tg_solve_noalloc(self._h, C.c_void_p(b_dev), C.c_void_p(x_dev), rtol, maxit, C.byref(st)))
+        return st
 
     # ---- solver ----
     @staticmethod

@@ -418,48 +418,59 @@ void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out)
 
 template <class MT, int PC>
 static void pcg_loop(tg_ctx* c, const MT* val, const double* m, const double* b, double* x, double* r, double* p,
-                     double* q, double rtol, int maxit, const int* d_active, tg_lin_stats* st);
+                     double* q, double rtol, int maxit, const int* d_active, const PcgWork& W, tg_lin_stats* st);
 
 template <class MT>
 void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, double* x, double* r, double* p,
-             double* q, double rtol, int maxit, const int* d_active, int nactive, tg_lin_stats* st)
+             double* q, double rtol, int maxit, const int* d_active, int nactive, tg_lin_stats* st,
+             const PcgWork* work)
 {
     (void)nactive;
+    PcgWork W;
+    if (work) {
+        W = *work;
+    } else {
+        // the context's own reductions, scalars and done count; the second
+        // direction buffer of the fused iteration is allocated on first use
+        if (c->pcg_fused && c->precond != TG_PC_BLOCK_JACOBI && c->vp2.n != 3 * (size_t)c->nnd)
+            c->vp2.alloc(3 * (size_t)c->nnd);
+        W = PcgWork{c->seg.d_part.p, c->seg.d_scal.p, c->seg.d_ndone.p, c->vp2.p, true};
+    }
     if (c->precond == TG_PC_BLOCK_JACOBI)
-        pcg_loop<MT, 1>(c, val, c->vbinv.p, b, x, r, p, q, rtol, maxit, d_active, st);
+        pcg_loop<MT, 1>(c, val, c->vbinv.p, b, x, r, p, q, rtol, maxit, d_active, W, st);
     else if (c->precond == TG_PC_AMG)
-        pcg_loop<MT, 2>(c, val, c->amg.lv[0]->d_x.p, b, x, r, p, q, rtol, maxit, d_active, st);
+        pcg_loop<MT, 2>(c, val, c->amg.lv[0]->d_x.p, b, x, r, p, q, rtol, maxit, d_active, W, st);
     else
-        pcg_loop<MT, 0>(c, val, dinv, b, x, r, p, q, rtol, maxit, d_active, st);
+        pcg_loop<MT, 0>(c, val, dinv, b, x, r, p, q, rtol, maxit, d_active, W, st);
 }
 
 template <class MT, int PC>
 static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double* b, double* x, double* r, double* p,
-                     double* q, double rtol, int maxit, const int* d_active, tg_lin_stats* st)
+                     double* q, double rtol, int maxit, const int* d_active, const PcgWork& W, tg_lin_stats* st)
 {
     Segments& S = c->seg;
     const long ns = S.nslices;
     const int nseg = S.nseg;
     const unsigned nb = node_blocks(c);
-    TG_HIP(hipMemsetAsync(S.d_ndone.p, 0, sizeof(int), c->stream));
+    TG_HIP(hipMemsetAsync(W.ndone, 0, sizeof(int), c->stream));
     // multigrid: dinv is the z vector; *done of the single segment gates the V-cycle
-    const double* done = S.d_scal.p + S_DONE * nseg;
+    const double* done = W.scal + S_DONE * nseg;
     if (PC == 2) {
         TimedRegion tr(c, T_PRECOND);
         amg_vcycle(c, b, (double*)dinv, nullptr);
     }
     {
         TimedRegion tr(c, T_VECTOR);
-        hipLaunchKernelGGL(k_seg_init<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, b, dinv, x, r, p, S.d_part.p, ns);
-        hipLaunchKernelGGL(k_seg_fin_init, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns, S.d_seg_slice.p,
-                           d_active, nseg, S.d_scal.p, rtol, maxit, S.d_ndone.p);
+        hipLaunchKernelGGL(k_seg_init<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, b, dinv, x, r, p, W.part, ns);
+        hipLaunchKernelGGL(k_seg_fin_init, dim3(nseg), dim3(FB), 0, c->stream, W.part, ns, S.d_seg_slice.p,
+                           d_active, nseg, W.scal, rtol, maxit, W.ndone);
     }
     const int CHUNK = PC == 2 ? 8 : 16;  // even: the fused iteration alternates its two direction buffers
     const bool fused = c->pcg_fused && PC != 1;
     double* pbuf[2] = {p, nullptr};
     if (fused) {
-        if (c->vp2.n != 3 * (size_t)c->nnd) c->vp2.alloc(3 * (size_t)c->nnd);
-        pbuf[1] = c->vp2.p;
+        if (!W.p2) fail(TG_GENERAL_FAIL, "fused PCG iteration without its second direction buffer");
+        pbuf[1] = W.p2;
     }
     auto issue_fused = [&]() {
         for (int k = 0; k < CHUNK; k++) {
@@ -470,7 +481,7 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
 #define TG_FSPMV(X, U)                                                                                        \
     hipLaunchKernelGGL((k_fused_spmv<MT, PC, X, U>), dim3(X ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream,  \
                        c->nnd, S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, \
-                       dinv, r, pold, pnew, q, S.d_part.p, ns, nseg, S.d_scal.p)
+                       dinv, r, pold, pnew, q, W.part, ns, nseg, W.scal)
                 switch (c->spmv_xcd) {
                 case 0: TG_FSPMV(false, false); break;
                 case 1: TG_FSPMV(true, false); break;
@@ -481,20 +492,20 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
             }
             {
                 TimedRegion tr(c, T_VECTOR);
-                hipLaunchKernelGGL(k_seg_fin_alpha, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
-                                   S.d_seg_slice.p, nseg, S.d_scal.p);
+                hipLaunchKernelGGL(k_seg_fin_alpha, dim3(nseg), dim3(FB), 0, c->stream, W.part, ns,
+                                   S.d_seg_slice.p, nseg, W.scal);
                 hipLaunchKernelGGL(k_seg_update<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, pnew,
-                                   q, dinv, x, r, S.d_part.p, ns, nseg, S.d_scal.p);
+                                   q, dinv, x, r, W.part, ns, nseg, W.scal);
             }
             if (PC == 2) {
                 TimedRegion tr(c, T_PRECOND);
                 amg_vcycle(c, r, (double*)dinv, done);
-                hipLaunchKernelGGL(k_seg_dot, dim3(nb), dim3(PB), 0, c->stream, c->nnd, r, dinv, S.d_part.p, ns);
+                hipLaunchKernelGGL(k_seg_dot, dim3(nb), dim3(PB), 0, c->stream, c->nnd, r, dinv, W.part, ns);
             }
             {
                 TimedRegion tr(c, T_VECTOR);
-                hipLaunchKernelGGL(k_seg_fin_beta, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
-                                   S.d_seg_slice.p, nseg, S.d_scal.p, S.d_ndone.p);
+                hipLaunchKernelGGL(k_seg_fin_beta, dim3(nseg), dim3(FB), 0, c->stream, W.part, ns,
+                                   S.d_seg_slice.p, nseg, W.scal, W.ndone);
             }
         }
     };
@@ -509,7 +520,7 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
 #define TG_SPMV(X, U)                                                                                      \
     hipLaunchKernelGGL((k_seg_spmv<MT, X, U>), dim3(X ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream, c->nnd, \
                        S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, p, q,  \
-                       S.d_part.p, ns, nseg, S.d_scal.p)
+                       W.part, ns, nseg, W.scal)
                 switch (c->spmv_xcd) {
                 case 0: TG_SPMV(false, false); break;
                 case 1: TG_SPMV(true, false); break;
@@ -520,22 +531,22 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
             }
             {
                 TimedRegion tr(c, T_VECTOR);
-                hipLaunchKernelGGL(k_seg_fin_alpha, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
-                                   S.d_seg_slice.p, nseg, S.d_scal.p);
+                hipLaunchKernelGGL(k_seg_fin_alpha, dim3(nseg), dim3(FB), 0, c->stream, W.part, ns,
+                                   S.d_seg_slice.p, nseg, W.scal);
                 hipLaunchKernelGGL(k_seg_update<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, p, q,
-                                   dinv, x, r, S.d_part.p, ns, nseg, S.d_scal.p);
+                                   dinv, x, r, W.part, ns, nseg, W.scal);
             }
             if (PC == 2) {
                 TimedRegion tr(c, T_PRECOND);
                 amg_vcycle(c, r, (double*)dinv, done);
-                hipLaunchKernelGGL(k_seg_dot, dim3(nb), dim3(PB), 0, c->stream, c->nnd, r, dinv, S.d_part.p, ns);
+                hipLaunchKernelGGL(k_seg_dot, dim3(nb), dim3(PB), 0, c->stream, c->nnd, r, dinv, W.part, ns);
             }
             {
                 TimedRegion tr(c, T_VECTOR);
-                hipLaunchKernelGGL(k_seg_fin_beta, dim3(nseg), dim3(FB), 0, c->stream, S.d_part.p, ns,
-                                   S.d_seg_slice.p, nseg, S.d_scal.p, S.d_ndone.p);
+                hipLaunchKernelGGL(k_seg_fin_beta, dim3(nseg), dim3(FB), 0, c->stream, W.part, ns,
+                                   S.d_seg_slice.p, nseg, W.scal, W.ndone);
                 hipLaunchKernelGGL(k_seg_direction<PC>, dim3(nb), dim3(PB), 0, c->stream, c->nnd, S.d_slice_seg.p, r,
-                                   dinv, p, ns, nseg, S.d_scal.p);
+                                   dinv, p, ns, nseg, W.scal);
             }
         }
     };
@@ -543,11 +554,11 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
     // serves every later solve of this context with the same vectors; the
     // iteration state lives in device memory (done flags make finished
     // segments' work a no-op).  The graph holds kernel nodes only.
-    const bool graph = c->pcg_graph && !c->timing.enabled;
+    const bool graph = W.graph && c->pcg_graph && !c->timing.enabled;
     if (graph) {
         // every address and size the chunk's launches take (tg_initialize
         // also drops the graph, as buffers may come back at old addresses)
-        const void* key[24] = {val, dinv, x, r, p, q, S.d_part.p, S.d_scal.p, S.d_ndone.p,
+        const void* key[24] = {val, dinv, x, r, p, q, W.part, W.scal, W.ndone,
                                (const void*)(long)(PC + 4 * c->spmv_xcd + 16 * (int)fused + 32L * c->amg.gen),
                                (const void*)(long)sizeof(MT), (const void*)(long)c->nnd, (const void*)(long)nseg,
                                (const void*)ns, S.d_seg_slice.p, S.d_slice_seg.p, c->op.d_slice_base.p,
@@ -562,8 +573,7 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
             std::memcpy(c->pg.key, key, sizeof key);
         }
     }
-    for (auto& e : c->poll_ev)
-        if (!e) TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // (tg_create made the two poll events)
     // Chunk k+1 is queued before chunk k's done count is read, so the GPU
     // never waits for the host; a chunk queued after every segment finished
     // does nothing.
@@ -576,7 +586,7 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
         TG_HIP(hipGetLastError());
         issued += CHUNK;
         const int slot = chunks & 1;
-        TG_HIP(hipMemcpyAsync(c->hstatus + 2 + slot, S.d_ndone.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipMemcpyAsync(c->hstatus + 2 + slot, W.ndone, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         TG_HIP(hipEventRecord(c->poll_ev[slot], c->stream));
         chunks++;
         if (issued >= maxit) break;
@@ -588,8 +598,8 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
     }
     TG_HIP(hipStreamSynchronize(c->stream));
     // per-segment results
-    S.h_scal.resize((size_t)S_NSCAL * nseg);
-    TG_HIP(hipMemcpyAsync(S.h_scal.data(), S.d_scal.p, sizeof(double) * S_NSCAL * nseg, hipMemcpyDeviceToHost,
+    if (S.h_scal.size() != (size_t)S_NSCAL * nseg) S.h_scal.resize((size_t)S_NSCAL * nseg);
+    TG_HIP(hipMemcpyAsync(S.h_scal.data(), W.scal, sizeof(double) * S_NSCAL * nseg, hipMemcpyDeviceToHost,
                           c->stream));
     TG_HIP(hipStreamSynchronize(c->stream));
     if (st) {
@@ -610,8 +620,8 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
 }
 
 template void pcg_run<double>(tg_ctx*, const double*, const double*, const double*, double*, double*, double*,
-                              double*, double, int, const int*, int, tg_lin_stats*);
+                              double*, double, int, const int*, int, tg_lin_stats*, const PcgWork*);
 template void pcg_run<float>(tg_ctx*, const float*, const double*, const double*, double*, double*, double*,
-                             double*, double, int, const int*, int, tg_lin_stats*);
+                             double*, double, int, const int*, int, tg_lin_stats*, const PcgWork*);
 
 }  // namespace tg

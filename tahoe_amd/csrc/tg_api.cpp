@@ -88,6 +88,34 @@ static void drain_events(tg_ctx* c)
 
 static void set_device(tg_ctx* c) { TG_HIP(hipSetDevice(c->device)); }
 
+// ---- scratch of tg_solve_noalloc ---------------------------------------------
+// One array per GPU for the life of the process.  In host code `scratch`
+// names a host-side stand-in, not device memory: its device address on each
+// GPU comes from hipGetSymbolAddress (resolved once per device, in
+// tg_create, with that device current) and only that address is handed to
+// kernels and copies.
+static __device__ __align__(256) unsigned char scratch[TG_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices] = {};
+
+static void resolve_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) fail(TG_OUT_OF_RANGE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return;
+    void* p = nullptr;
+    TG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(TG_GENERAL_FAIL, "no device address for the no-allocation scratch");
+    g_scratch_dev[device] = p;
+}
+
+static void* scratch_of(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch_dev[device] : nullptr;
+}
+
 static void require_init(const tg_ctx* c)
 {
     if (!c->initialized) fail(TG_GENERAL_FAIL, "tg_initialize has not been called");
@@ -125,6 +153,7 @@ static void setup_segments(tg_ctx* c)
     S.d_part.alloc(2 * (size_t)S.nslices);
     S.d_out.alloc((size_t)S.nseg);
     S.d_ndone.alloc(1);
+    S.h_scal.assign((size_t)S_NSCAL * S.nseg, 0.0);
     std::vector<int> ones((size_t)S.nseg, 1);
     auto up = [&](int* dst, const std::vector<int>& v) {
         TG_HIP(hipMemcpyAsync(dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, c->stream));
@@ -198,6 +227,7 @@ int tg_create(tg_ctx** out, int device)
         TG_HIP(hipGetDeviceCount(&n));
         if (device < 0 || device >= n) fail(TG_OUT_OF_RANGE, "no such device");
         TG_HIP(hipSetDevice(device));
+        resolve_scratch(device);
         TG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         TG_HIP(hipHostMalloc((void**)&c->hscal, sizeof(double) * 16, hipHostMallocDefault));
         TG_HIP(hipHostMalloc((void**)&c->hstatus, sizeof(int) * 4, hipHostMallocDefault));
@@ -205,6 +235,8 @@ int tg_create(tg_ctx** out, int device)
         TG_HIP(hipMemsetAsync(c->dstatus.p, 0, sizeof(int) * 4, c->stream));
         c->dscal.alloc(16);
         TG_HIP(hipMemsetAsync(c->dscal.p, 0, sizeof(double) * 16, c->stream));
+        // the PCG loop polls its done count through these two events
+        for (hipEvent_t& e : c->poll_ev) TG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         TG_HIP(hipStreamSynchronize(c->stream));
     });
     if (rc != TG_OK) {
@@ -749,6 +781,61 @@ int tg_solve(tg_ctx* c, double* rhs, double rtol, int maxit, tg_lin_stats* st)
         solve_linear(c, c->vz.p, c->vx.p, rtol, maxit, st);
         node_to_eq(c, c->vx.p, c->vwork.p);
         TG_HIP(hipMemcpyAsync(rhs, c->vwork.p, sizeof(double) * c->n_eq, hipMemcpyDeviceToHost, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int tg_solve_noalloc(tg_ctx* c, const double* b_dev, double* x_dev, double rtol, int maxit, tg_lin_stats* st)
+{
+    if (!c || !b_dev || !x_dev) return TG_BAD_INPUT;
+    return guarded(c, [&] {
+        require_init(c);
+        if (c->precond == TG_PC_AMG)
+            fail(TG_BAD_INPUT, "tg_solve_noalloc: the multigrid preconditioner keeps its own buffers");
+        if (c->linear_solver != TG_LIN_PCG) fail(TG_BAD_INPUT, "tg_solve_noalloc: PCG only");
+        double* base = (double*)scratch_of(c->device);
+        if (!base) fail(TG_GENERAL_FAIL, "tg_solve_noalloc: scratch not resolved for this device");
+        // layout (doubles): b, x, r, p, q, p2 [3 nnd each], part [2 nslices],
+        // scal [S_NSCAL nseg], then the done count (int)
+        const size_t n3 = 3 * (size_t)c->nnd;
+        const size_t ns = (size_t)c->seg.nslices, nseg = (size_t)c->seg.nseg;
+        const size_t ndoubles = 6 * n3 + 2 * ns + (size_t)S_NSCAL * nseg;
+        const size_t need = ndoubles * sizeof(double) + sizeof(int);
+        if (need > (size_t)TG_NOALLOC_SCRATCH_BYTES)
+            fail(TG_SIZE_MISMATCH, "tg_solve_noalloc: the problem needs " + std::to_string(need) +
+                                       " bytes of scratch, more than the " +
+                                       std::to_string((long)TG_NOALLOC_SCRATCH_BYTES) + " there are");
+        double* vb = base;
+        double* vx = vb + n3;
+        double* vr = vx + n3;
+        double* vp = vr + n3;
+        double* vq = vp + n3;
+        double* vp2 = vq + n3;
+        double* part = vp2 + n3;
+        double* scal = part + 2 * ns;
+        int* ndone = (int*)(scal + (size_t)S_NSCAL * nseg);
+        // no event timing inside (event creation is an allocation)
+        struct TimingOff {
+            tg_ctx* c;
+            bool was;
+            ~TimingOff() { c->timing.enabled = was; }
+        } off{c, c->timing.enabled};
+        c->timing.enabled = false;
+        check_status(c);
+        eq_to_node(c, b_dev, vb);
+        prepare_operator(c);
+        // every buffer the run reads before writing is set by the run itself
+        // (x, r, p, partials and scalars by the init kernels, the done count
+        // by a clear), so what an earlier call left does not matter
+        const PcgWork W{part, scal, ndone, vp2, false};
+        if (mixed_operator(c))
+            pcg_run<float>(c, c->op.d_val32.p, c->vdinv.p, vb, vx, vr, vp, vq, rtol, maxit, c->seg.d_active.p,
+                           c->seg.nseg, st, &W);
+        else
+            pcg_run<double>(c, c->op.d_val.p, c->vdinv.p, vb, vx, vr, vp, vq, rtol, maxit, c->seg.d_active.p,
+                            c->seg.nseg, st, &W);
+        check_status(c);
+        node_to_eq(c, vx, x_dev);
         TG_HIP(hipStreamSynchronize(c->stream));
     });
 }

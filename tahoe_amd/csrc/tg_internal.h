@@ -300,9 +300,22 @@ void widen_values(tg_ctx* c);
 void pcg_solve_mixed(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
 double device_dot(tg_ctx* c, const double* a, const double* b, long n);
 void pcg_solve(tg_ctx* c, const double* b_node, double* x_node, double rtol, int maxit, tg_lin_stats* st);
+// Device work space of one PCG run besides its vectors: per-slice reduction
+// partials [2][nslices], per-segment scalar slots [S_NSCAL][nseg], the done
+// count [1] and the second direction buffer of the fused iteration [3 nnd];
+// graph = false issues every launch directly (no capture, no instantiation).
+struct PcgWork {
+    double* part = nullptr;
+    double* scal = nullptr;
+    int* ndone = nullptr;
+    double* p2 = nullptr;
+    bool graph = true;
+};
+// work = nullptr: the context's own buffers
 template <class MT>
 void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, double* x, double* r, double* p,
-             double* q, double rtol, int maxit, const int* d_active, int nactive, tg_lin_stats* st);
+             double* q, double rtol, int maxit, const int* d_active, int nactive, tg_lin_stats* st,
+             const PcgWork* work = nullptr);
 void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out);
 void gmres_solve(tg_ctx* c, const double* b, double* x, double rtol, int maxit, tg_lin_stats* st);
 // aggregation multigrid (tg_amg.cpp: structure on the host; k_amg.hip: values and cycles)
