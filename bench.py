@@ -42,6 +42,7 @@ sys.path.insert(0, ROOT)
 METRIC = "elements/s assembled (R+K) + Newton solves/s; p50/p99 RVE latency vs plain FP64"
 HBM_PEAK = 8.0e12        # B/s, MI355X HBM3E
 FP64_PEAK = 78.6e12      # flop/s, MI355X FP64 vector and matrix (dense)
+FP32_MFMA_PEAK = 157.3e12  # flop/s, MI355X FP32 matrix (dense)
 
 # algorithmic work per element (flop counts of the kernels' arithmetic, not of the launch)
 RES_FLOP_PER_EL = 6.2e3          # residual: shape derivatives + kinematics + stress + B^T P
@@ -79,6 +80,9 @@ def parse_args():
                     help="1: time one complete Newton correction (PCG to --lin-rtol) by wall clock")
     ap.add_argument("--correction-maxit", type=int, default=200000,
                     help="PCG iteration cap of the complete correction")
+    ap.add_argument("--pcg-graph", type=int, choices=[0, 1], default=1,
+                    help="1: the PCG chunks are replayed as a captured HIP graph (default); 0: direct launches of "
+                         "the same kernels (for tracing tools that do not follow graph launches)")
     ap.add_argument("--step-n", type=int, default=128, help="newton-step: cube edge")
     ap.add_argument("--step-stretch", type=float, default=0.005,
                     help="newton-step: top-face displacement of the one load step, fraction of the height")
@@ -312,6 +316,7 @@ def bench_cube(a):
     if a.pcg_fused is not None:
         s.set_tuning(api.TUNE_PCG_FUSED, a.pcg_fused)
     tuned_spmv.update(xcd=s.tuning(api.TUNE_SPMV_XCD), fused=s.tuning(api.TUNE_PCG_FUSED))
+    s.set_tuning(api.TUNE_PCG_GRAPH, a.pcg_graph)
     log(f"PCG: SpMV variant {tuned_spmv['xcd']}, fused direction {tuned_spmv['fused']} "
         f"(autotune ms per iteration: {tuned_spmv['ms_per_iteration']}); tuning {time.perf_counter() - t_tune:.1f} s")
 
@@ -374,6 +379,7 @@ def bench_cube(a):
     # least HBM bytes of a stiffness pass: every MSR value written once, the
     # element's connectivity and its nodes' X and u read once
     stf_bytes = msr_len * (8 if a.precision == "fp64" else 4) + 32 * m.nel + 48 * m.nnd
+    stf_peak = FP64_PEAK if a.precision == "fp64" else FP32_MFMA_PEAK
     line = {
         "metric": METRIC,
         "value": value,
@@ -408,11 +414,15 @@ def bench_cube(a):
                      "why_this_kernel": "the PCG SpMV is the dominant kernel of a Newton solve (85% of a PCG "
                                         "iteration, ~37k iterations per correction at 128^3)"},
         "roofline_assembly": {
-            "stiffness": {"bound": "mfma", "achieved": stf_gflops / 1e3, "peak": FP64_PEAK / 1e12,
-                          "unit": "TFLOP/s", "frac": stf_gflops * 1e9 / FP64_PEAK, "avg_pass_ms": k_ms,
+            "stiffness": {"bound": "mfma", "achieved": stf_gflops / 1e3, "peak": stf_peak / 1e12,
+                          "unit": "TFLOP/s", "frac": stf_gflops * 1e9 / stf_peak, "avg_pass_ms": k_ms,
                           "flop_per_element": STIFF_FLOP_PER_EL,
                           "hbm_GBps_min_bytes": stf_bytes / (k_ms * 1e-3) / 1e9,
-                          "note": "FP64 dense 24x24 B^T C B count; FP64 vector and matrix peaks are equal on MI355X"},
+                          "kernel": ("k_stiffness_mfma (16x16x4 MFMA tiles)" if api.STIFF_SHAPE_MFMA[tuned["shape"]]
+                                     else "k_stiffness_tl / k_stiffness_db (VALU 3x3 node blocks)"),
+                          "note": "dense 24x24 B^T C B count per element; peak: " +
+                                  ("FP64 (vector and matrix peaks are equal on MI355X)" if a.precision == "fp64"
+                                   else "FP32 matrix cores")},
             "residual": {"bound": "fp64", "achieved": res_gflops / 1e3, "peak": FP64_PEAK / 1e12,
                          "unit": "TFLOP/s", "frac": res_gflops * 1e9 / FP64_PEAK, "avg_pass_ms": r_ms,
                          "flop_per_element": RES_FLOP_PER_EL},
@@ -453,7 +463,8 @@ def bench_cube(a):
         line["newton_correction"] = {
             "wall_s": wall, "pcg_iterations": its, "converged": its < a.correction_maxit,
             "error0": st.error0, "pcg_s": pcg_s, "dof_iterations_per_s": n_eq * its / pcg_s,
-            "note": "R, ||R||, K into the operator, PCG (HIP-graph replayed chunks), u += du; wall clock"}
+            "note": "R, ||R||, K into the operator, PCG (" + ("HIP-graph replayed chunks" if a.pcg_graph else
+                                                              "directly launched chunks") + "), u += du; wall clock"}
         log(f"correction: {wall:.2f} s, {its} PCG iterations")
     dl.emit()
 

@@ -260,7 +260,12 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      before the block products, 12..14 for 12, 8 or 4
  *                      and 15 for 16 block-major with the additions issued
  *                      as memory-side atomics (order per entry unchanged:
- *                      colours); default 9;
+ *                      colours); 16..17 for 8 or 4 elements per wave on the
+ *                      matrix cores (16x16x4 MFMA tiles, FP32 in TG_MIXED,
+ *                      FP64 otherwise; MSR assembly -- the straight-into-
+ *                      operator Newton path runs shape 9 instead), 18 as 16
+ *                      with registers bounded for 3 waves per SIMD; default
+ *                      17 for the FP32 (TG_MIXED) stiffness, 9 for FP64;
  * TG_TUNE_NEWTON_ASSEMBLY  where tg_newton / tg_newton_batch /
  *                      tg_newton_correction put the tangent: 0 (default)
  *                      into the MSR and then expanded into the inner solve's
@@ -289,8 +294,20 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      five, no separate direction pass) for the Jacobi and
  *                      multigrid preconditioners, in any SpMV variant;
  *                      0: the five-kernel iteration. */
+/* TG_TUNE_STIFF_GATHER 0 (default): the total-Lagrangian stiffness is
+ *                      added into the MSR element by element (colour-
+ *                      ordered read-modify-write through the slot map);
+ *                      1: each element's 300 upper entries are written to
+ *                      an element buffer in launch order, and one gather
+ *                      pass sums every MSR value's contributions in that
+ *                      order and writes it once.  Same sums in the same
+ *                      order: the values are bitwise equal.  Costs a
+ *                      buffer of 300 values per element and a list of
+ *                      contributions (4 bytes each, 300 per element) built
+ *                      on first use. */
 enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3, TG_TUNE_PCG_GRAPH = 4,
-       TG_TUNE_SPMV_XCD = 5, TG_TUNE_AMG_DEGREE = 6, TG_TUNE_AMG_WEIGHT = 7, TG_TUNE_PCG_FUSED = 8 };
+       TG_TUNE_SPMV_XCD = 5, TG_TUNE_AMG_DEGREE = 6, TG_TUNE_AMG_WEIGHT = 7, TG_TUNE_PCG_FUSED = 8,
+       TG_TUNE_STIFF_GATHER = 9 };
 int tg_set_tuning(tg_ctx* ctx, int knob, long value);
 int tg_get_tuning(tg_ctx* ctx, int knob, long* value);
 

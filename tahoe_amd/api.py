@@ -18,13 +18,16 @@ TOTAL_LAGRANGIAN, Q1P0 = 0, 1
 MAT_SIMO_ISOTROPIC, MAT_STVENANT = 0, 1
 FP64, MIXED = 0, 1
 TUNE_STIFF_CHUNK, TUNE_STIFF_SHAPE, TUNE_NEWTON_ASSEMBLY, TUNE_PCG_GRAPH, TUNE_SPMV_XCD = 1, 2, 3, 4, 5
-TUNE_AMG_DEGREE, TUNE_AMG_WEIGHT, TUNE_PCG_FUSED = 6, 7, 8
+TUNE_AMG_DEGREE, TUNE_AMG_WEIGHT, TUNE_PCG_FUSED, TUNE_STIFF_GATHER = 6, 7, 8, 9
 PC_JACOBI, PC_BLOCK_JACOBI, PC_AMG = 0, 1, 2
 LIN_PCG, LIN_GMRES = 0, 1
-STIFF_SHAPE_EPB = (16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4, 12, 8, 4, 16)
-STIFF_SHAPE_BLOCK_MAJOR = (False,) * 4 + (True, True) + (False,) * 9 + (True,)
-STIFF_SHAPE_DB_FORM = (False,) * 6 + (True,) * 3 + (False,) * 7
-STIFF_SHAPE_SCATTER = (0,) * 9 + (1,) * 3 + (2,) * 4   # 0 load-add-store, 1 loads first, 2 atomic adds
+STIFF_SHAPE_EPB = (16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4, 12, 8, 4, 16, 8, 4, 8)
+STIFF_SHAPE_BLOCK_MAJOR = (False,) * 4 + (True, True) + (False,) * 9 + (True,) + (False,) * 3
+STIFF_SHAPE_DB_FORM = (False,) * 6 + (True,) * 3 + (False,) * 10
+# 0 load-add-store, 1 loads first, 2 atomic adds, 3 matrix-core tiles (MSR path; elements per wave),
+# 4 matrix-core tiles with registers bounded for 3 waves per SIMD
+STIFF_SHAPE_SCATTER = (0,) * 9 + (1,) * 3 + (2,) * 4 + (3,) * 2 + (4,)
+STIFF_SHAPE_MFMA = tuple(x >= 3 for x in STIFF_SHAPE_SCATTER)
 
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)

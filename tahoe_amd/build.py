@@ -20,8 +20,11 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 HOST_SOURCES = ["TotalLagrangianHIPT.cpp", "HIPMSRMatrixT.cpp", "NLSolverHIP.cpp", "InputDeck.cpp",
                 "FEManagerHIP.cpp"]
 
-SOURCES = ["tg_setup.cpp", "tg_amg.cpp", "tg_api.cpp", "k_element.hip", "k_solver.hip", "k_pcg.hip", "k_amg.hip", "k_gmres.hip"]
-HEADERS = ["tg_internal.h", "tg_kinematics.h", "../../include/tahoe_gpu.h"]
+SOURCES = ["tg_setup.cpp", "tg_amg.cpp", "tg_api.cpp", "k_element.hip", "k_q1p0.hip", "k_gather.hip", "k_solver.hip",
+           "k_pcg.hip", "k_amg.hip", "k_gmres.hip"]
+HEADERS = ["tg_internal.h", "tg_kinematics.h", "tg_element.h", "../../include/tahoe_gpu.h"]
+# per-source extra flags
+EXTRA = {}
 
 ARCH = os.environ.get("TAHOE_GPU_ARCH", "gfx950")
 # packed FP32 instructions are disabled for all device code
@@ -44,7 +47,7 @@ def build(verbose: bool = False) -> str:
     for s in SOURCES:
         o = os.path.join(OBJ, os.path.splitext(s)[0] + ".o")
         objs.append(o)
-        cmd = [hipcc()] + FLAGS + ["-c", os.path.join(CSRC, s), "-o", o]
+        cmd = [hipcc()] + FLAGS + EXTRA.get(s, []) + ["-c", os.path.join(CSRC, s), "-o", o]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

@@ -7,6 +7,7 @@
 // element (lane q loads node q) into LDS and read by all 8 lanes.
 #include "tg_internal.h"
 #include "tg_kinematics.h"
+#include "tg_element.h"
 
 namespace tg {
 
@@ -16,33 +17,6 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // residual
 // ---------------------------------------------------------------------------
 
-// Sum of 8 lanes' 24 partials, scattered: lane q ends with node q's 3 sums.
-__device__ __forceinline__ void reduce_scatter8(const double r[24], int q, double out[3])
-{
-    double s12[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-        const bool hi = q & 4;
-        const double keep = hi ? r[12 + k] : r[k];
-        const double send = hi ? r[k] : r[12 + k];
-        s12[k] = keep + __shfl_xor(send, 4);
-    }
-    double s6[6];
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-        const bool hi = q & 2;
-        const double keep = hi ? s12[6 + k] : s12[k];
-        const double send = hi ? s12[k] : s12[6 + k];
-        s6[k] = keep + __shfl_xor(send, 2);
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const bool hi = q & 1;
-        const double keep = hi ? s6[3 + k] : s6[k];
-        const double send = hi ? s6[k] : s6[3 + k];
-        out[k] = keep + __shfl_xor(send, 1);
-    }
-}
 
 // R -= f_int for the elements of one colour.  Per integration point
 // (TotalLagrangianT::FormKd, TotalLagrangianT.cpp:107-143, called with
@@ -170,8 +144,6 @@ static void element_residual(tg_ctx* c, const double* u, double* R_node)
 // stiffness
 // ---------------------------------------------------------------------------
 
-// Position of (m, n), m <= n < 24, in the packed upper triangle.
-__device__ __forceinline__ int upper_index(int m, int n) { return m * 24 - (m * (m - 1)) / 2 + (n - m); }
 
 // Element stiffness (TotalLagrangianT::FormStiffness, TotalLagrangianT.cpp:40-104),
 // one lane per upper 3x3 node block (a <= b) of an element.  With
@@ -191,12 +163,14 @@ __device__ __forceinline__ int upper_index(int m, int n) { return m * 24 - (m * 
 // barrier and scatter waits.  EPB (elements per block) and the phase-2 lane
 // order are template parameters; the autotuner picks among the compiled
 // shapes.
-constexpr int STIFF_EPB[] = {16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4, 12, 8, 4, 16};
-constexpr bool STIFF_PM[] = {false, false, false, false, true, true, false, false,
-                             false, false, false, false, false, false, false, true};
-constexpr bool STIFF_DB[] = {false, false, false, false, false, false, true, true,
-                             true, false, false, false, false, false, false, false};
-constexpr int STIFF_SC[] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 2, 2};
+constexpr int STIFF_EPB[] = {16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4, 12, 8, 4, 16, 8, 4, 8};
+constexpr bool STIFF_PM[] = {false, false, false, false, true, true, false, false, false, false,
+                             false, false, false, false, false, true, false, false, false};
+constexpr bool STIFF_DB[] = {false, false, false, false, false, false, true, true, true, false,
+                             false, false, false, false, false, false, false, false, false};
+// scatter form; 3: the matrix-core kernel k_stiffness_mfma (EPB elements per
+// wave), 4: the same with registers bounded for 3 waves per SIMD
+constexpr int STIFF_SC[] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 2, 2, 2, 2, 3, 3, 4};
 constexpr int N_STIFF_SHAPES = sizeof(STIFF_EPB) / sizeof(STIFF_EPB[0]);
 
 // Voigt index of (i,k): (11,22,33,23,13,12)
@@ -259,11 +233,6 @@ __device__ __forceinline__ bool stiffness_ip(int q, bool valid, double2 mk, XU x
     return !bad;
 }
 
-// The element slot map holds, per element upper entry, the MSR value index
-// s, or -s-2 where that entry is the first contribution to s in the
-// stiffness launch order (the values are cleared before the passes, so the
-// read of a first contribution would return 0), or -1 for no entry.
-__device__ __forceinline__ int slot_target(int s) { return s < -1 ? -s - 2 : s; }
 
 // Scatter of one upper 3x3 node block (a <= b, local upper index p) of
 // element e: into the MSR values through the slot map (TO_OP = false,
@@ -351,7 +320,8 @@ __device__ __forceinline__ void upper_block(int p, int& a, int& b)
 // path) the slot indices and old values of the lane's 9 entries are loaded
 // before the block products, so the read half of the read-modify-write is in
 // flight during the arithmetic (no other element of the launch touches those
-// entries); 2 memory-side atomic adds with no return value.
+// entries); 2 memory-side atomic adds with no return value; 5 (MSR path) the
+// element buffer of the gather assembly instead of the MSR.
 template <class VT, int EPB, bool TO_OP, bool PM, int SC, int MAT = MAT_SIMO>
 __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict__ elems, int count,
                                                              const int* __restrict__ conn,
@@ -471,6 +441,16 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_tl(const int* __restrict
                 for (int j = 0; j < 3; j++) {
                     const int k = PRE ? 3 * i + j : 0;
                     if (sidx[k] >= 0) val[sidx[k]] = old[k] + K[i][j];
+                }
+        } else if (SC == 5) {
+            // element buffer: val = the row of the launch's first element
+            VT* eb = val + 300L * (base + le);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const int m = 3 * a + i, n = 3 * b + j;
+                    if (n >= m) eb[upper_index(m, n)] = K[i][j];
                 }
         } else {
             scatter_block<VT, TO_OP, SC == 2>(elems[base + le], a, b, p, K, conn, slot, eqn, val);
@@ -676,11 +656,253 @@ __global__ void __launch_bounds__(36 * EPB) k_stiffness_db(const int* __restrict
     if (bad) atomicOr(status, 7);
 }
 
+// Element stiffness on the matrix cores: K_e = sum_q B_q^T (D_q B_q) + the
+// geometric term, as 16x16x4 MFMA tiles (v_mfma_f32_16x16x4f32 in the mixed
+// path, v_mfma_f64_16x16x4f64 in FP64).  One wave per workgroup, EPW
+// elements per wave:
+//   - phase 1 (lane = element x integration point): g = dN/dx, s c, s sigma
+//     in FP64, rounded to VT into LDS, as in k_stiffness_tl;
+//   - per element, the 24 element dofs are taken dof-major (p = 8 i + a), so
+//     tile rows/columns 0..15 are the x and y dofs of the 8 nodes and 16..23
+//     the z dofs; the three upper tiles (0,0), (0,1), (1,1) cover every
+//     unordered dof pair once (136 + 128 + 36 = 300 entries).  The K
+//     dimension is (Voigt row v, integration point q), 48 = 6 x 8: lane l
+//     supplies A[p = l%16][k] = B_q[v][p] (a gradient entry or 0, Hughes
+//     2.8.21) and B[k][p] = (D_q B_q)[v][p] (3 products, the three nonzeros
+//     of a B column) for q = 4h + l/16; a lane's two dofs (l%16 and 16 +
+//     l%16) belong to the same node, so it reads 3 gradient entries and D_q.
+//   - the geometric term G_ab = sum_q g_q(a) . (T_q g_q(b)) is one more 16x16
+//     tile over K = 24 (q, k); it seeds tile (1,1) lane for lane and tile
+//     (0,0) through a lane permutation, on the blocks with equal dof.
+// Operands: lane l supplies A[l%16][l/16] and B[l/16][l%16] in both
+// precisions.  Results: lane l, register r holds row 4(l/16) + r (f32) or
+// row l/16 + 4r (f64), column l%16 (measured on gfx950); the entries go
+// through the slot map into the MSR (slot and old-value loads issued before
+// the products; first contributions stored without the read).
+template <class VT>
+struct Mfma;
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // tile row of result register r in lane group kq = l/16
+    static __device__ __forceinline__ constexpr int row(int kq, int r) { return 4 * kq + r; }
+};
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ constexpr int row(int kq, int r) { return kq + 4 * r; }
+};
+
+// local node-major index (3 a + i) of dof-major index p = 8 i + a
+__device__ __forceinline__ int node_major(int p) { return 3 * (p & 7) + (p >> 3); }
+
+// MINW > 1 asks the compiler for registers that fit MINW waves per SIMD.
+// GATHER: the entries go to the element buffer, val = its row of the
+// launch's first element (no slot map, no loads); else into the MSR.
+template <class VT, int EPW, int MAT = MAT_SIMO, int MINW = 1, bool GATHER = false>
+__global__ void __launch_bounds__(64, MINW) k_stiffness_mfma(const int* __restrict__ elems, int count,
+                                                        const int* __restrict__ conn, const int* __restrict__ emat,
+                                                        const double2* __restrict__ props,
+                                                        const double* __restrict__ X, const double* __restrict__ U,
+                                                        const int* __restrict__ slot, VT* __restrict__ val,
+                                                        int* __restrict__ status)
+{
+    using v4 = typename Mfma<VT>::v4;
+    __shared__ double sxu[EPW][8][6];
+    __shared__ VT sg[EPW][8][3][8];  // dN/dx [element][ip][direction][node]
+    __shared__ VT sd[EPW][8][21];    // s c, packed symmetric Voigt
+    __shared__ VT st[EPW][8][6];     // s sigma, Voigt (11,22,33,23,13,12)
+    const int l = threadIdx.x;
+    const long base = (long)blockIdx.x * EPW;
+    const int nel = (int)(count - base < EPW ? count - base : EPW);
+    bool bad = false;
+    if (l < 8 * EPW) {
+        const int le = l >> 3, q = l & 7;
+        if (le < nel) {
+            const int node = conn[8L * elems[base + le] + q];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                sxu[le][q][k] = X[3L * node + k];
+                sxu[le][q][3 + k] = U[3L * node + k];
+            }
+        }
+    }
+    __syncthreads();
+    if (l < 8 * EPW) {
+        const int le = l >> 3, q = l & 7;
+        if (le < nel) {
+            const double2 mk = props[emat[elems[base + le]]];
+            double g[3][8], D[21], T[6];
+            bad |= !stiffness_ip<MAT>(q, true, mk, [&](int a, int k) { return sxu[le][a][k]; }, g, D, T);
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int a = 0; a < 8; a++) sg[le][q][k][a] = (VT)g[k][a];
+#pragma unroll
+            for (int k = 0; k < 21; k++) sd[le][q][k] = (VT)D[k];
+#pragma unroll
+            for (int k = 0; k < 6; k++) st[le][q][k] = (VT)T[k];
+        }
+    }
+    __syncthreads();
+    const int kq = l >> 4, c = l & 15;
+    const int na = c & 7;        // node of both of this lane's dofs
+    const int i0 = c >> 3;       // dof of p = c (x or y); p = 16 + c is the z dof (valid when c < 8)
+    const bool lo8 = c < 8;
+    // this lane's output entries: tile t (0: (0,0), 1: (0,1), 2: (1,1)), register r
+    int ent[12];                 // packed upper index into the element's 300, or -1
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int m = Mfma<VT>::row(kq, r);  // row of tiles (0,0), (0,1); 16 + m of tile (1,1)
+        ent[r] = m <= c ? upper_index(min(node_major(m), node_major(c)), max(node_major(m), node_major(c))) : -1;
+        const int a01 = node_major(m), b01 = 3 * na + 2;
+        ent[4 + r] = lo8 ? upper_index(min(a01, b01), max(a01, b01)) : -1;
+        // tile (1,1): z dof of node m against the z dof of node na
+        ent[8 + r] = (lo8 && m < 8 && m <= c) ? upper_index(3 * m + 2, 3 * na + 2) : -1;
+    }
+    // the scatter's loads run one element ahead: the slot indices of element
+    // le + 1 are issued before the products of element le and its old values
+    // after them (elements of one launch share no entry)
+    // Loads are issued unconditionally (entry 0 of the element's slot row,
+    // value 0 of the MSR, both always present, stand in for a lane with
+    // nothing there) and their results selected, so the loop has no branches
+    // around loads.
+    int nsidx[12];
+    VT nold[12];
+    if (!GATHER) {
+        const int* es = slot + 300L * elems[base];
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            const int x = es[ent[k] >= 0 ? ent[k] : 0];
+            nsidx[k] = ent[k] >= 0 ? x : -1;
+        }
+        // a first contribution skips the read (the entry is still 0)
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            const VT x = val[nsidx[k] >= 0 ? nsidx[k] : 0];
+            nold[k] = nsidx[k] >= 0 ? x : VT(0);
+        }
+    }
+    for (int le = 0; le < nel; le++) {
+        int sidx[12];
+        VT old[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            sidx[k] = nsidx[k];
+            old[k] = nold[k];
+        }
+        const bool more = !GATHER && le + 1 < nel;
+        if (more) {
+            const int* es = slot + 300L * elems[base + le + 1];
+#pragma unroll
+            for (int k = 0; k < 12; k++) {
+                const int x = es[ent[k] >= 0 ? ent[k] : 0];
+                nsidx[k] = ent[k] >= 0 ? x : -1;
+            }
+        }
+        // geometric 8x8 tile: A[a][(q,k)] = g_qk(a), B[(q,k)][b] = (T_q g_q(b))_k
+        v4 accg = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            const int k = 4 * s + kq, q = k / 3, kk = k - 3 * q;
+            VT ga = 0, tb = 0;
+            if (lo8) {
+                const VT* T = st[le][q];
+                const VT tk0 = T[voigt(kk, 0)], tk1 = T[voigt(kk, 1)], tk2 = T[voigt(kk, 2)];
+                ga = sg[le][q][kk][na];
+                tb = tk0 * sg[le][q][0][na] + tk1 * sg[le][q][1][na] + tk2 * sg[le][q][2][na];
+            }
+            accg = Mfma<VT>::mac(ga, tb, accg);
+        }
+        // seed: tile (1,1) = G lane for lane; tile (0,0) gets G on its two
+        // equal-dof diagonal blocks: its row m (dof m >> 3, node m & 7) and
+        // column c take G[m & 7][c & 7], held by the lane and register whose
+        // result row is m & 7 in column c & 7
+        v4 acc11 = accg, acc00, acc01 = {0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = Mfma<VT>::row(kq, r);
+            constexpr bool f64 = sizeof(VT) == 8;
+            // f32: row 4 kq + r -> m & 7 = 4 (kq & 1) + r; f64: row kq + 4 r -> m & 7 = kq + 4 (r & 1)
+            const int src = (c & 7) | ((f64 ? kq : (kq & 1)) << 4);
+            const VT gv = __shfl(accg[f64 ? (r & 1) : r], src, 64);
+            acc00[r] = (m >> 3) == i0 ? gv : VT(0);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int q = 4 * h + kq;
+            VT D[21];
+#pragma unroll
+            for (int k = 0; k < 21; k++) D[k] = sd[le][q][k];
+            const VT g0 = sg[le][q][0][na], g1 = sg[le][q][1][na], g2 = sg[le][q][2][na];
+#pragma unroll
+            for (int v = 0; v < 6; v++) {
+                // B_q[v][(node, dof)] for dofs x, y, z (Hughes 2.8.21, Voigt 11,22,33,23,13,12)
+                const VT bx = v == 0 ? g0 : v == 4 ? g2 : v == 5 ? g1 : VT(0);
+                const VT by = v == 1 ? g1 : v == 3 ? g2 : v == 5 ? g0 : VT(0);
+                const VT bz = v == 2 ? g2 : v == 3 ? g1 : v == 4 ? g0 : VT(0);
+                // (D_q B_q)[v][(node, dof)]; both x and y are formed and one
+                // kept (a selected register index would become an indexed
+                // register access)
+                const VT dx = D[sym6(v, 0)] * g0 + D[sym6(v, 4)] * g2 + D[sym6(v, 5)] * g1;
+                const VT dy = D[sym6(v, 1)] * g1 + D[sym6(v, 3)] * g2 + D[sym6(v, 5)] * g0;
+                const VT dz = D[sym6(v, 2)] * g2 + D[sym6(v, 3)] * g1 + D[sym6(v, 4)] * g0;
+                const VT A0 = i0 ? by : bx, B0 = i0 ? dy : dx;
+                const VT A1 = lo8 ? bz : VT(0), B1 = lo8 ? dz : VT(0);
+                acc00 = Mfma<VT>::mac(A0, B0, acc00);
+                acc01 = Mfma<VT>::mac(A0, B1, acc01);
+                acc11 = Mfma<VT>::mac(A1, B1, acc11);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < 12; k++) {
+                const VT x = val[nsidx[k] >= 0 ? nsidx[k] : 0];
+                nold[k] = nsidx[k] >= 0 ? x : VT(0);
+            }
+        }
+        if (GATHER) {
+            VT* eb = val + 300L * (base + le);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (ent[r] >= 0) eb[ent[r]] = acc00[r];
+                if (ent[4 + r] >= 0) eb[ent[4 + r]] = acc01[r];
+                if (ent[8 + r] >= 0) eb[ent[8 + r]] = acc11[r];
+            }
+            continue;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int s0 = slot_target(sidx[r]), s1 = slot_target(sidx[4 + r]), s2 = slot_target(sidx[8 + r]);
+            if (s0 >= 0) val[s0] = old[r] + acc00[r];
+            if (s1 >= 0) val[s1] = old[4 + r] + acc01[r];
+            if (s2 >= 0) val[s2] = old[8 + r] + acc11[r];
+        }
+    }
+    if (bad) atomicOr(status, 7);
+}
+
 template <class VT, int EPB, bool TO_OP, bool PM, bool DB, int SC, int MAT = MAT_SIMO>
 static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u, VT* val)
 {
     const int blocks = (count + EPB - 1) / EPB;
-    if (DB)
+    if constexpr (SC >= 3 && !TO_OP)
+        hipLaunchKernelGGL((k_stiffness_mfma<VT, EPB, MAT, SC == 4 ? 3 : 1>), dim3(blocks), dim3(64), 0, c->stream,
+                           c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                           c->dslot.p, val, c->dstatus.p);
+    else if constexpr (SC >= 3)
+        // the matrix-core kernel scatters into the MSR only; the block
+        // operator gets the loads-first block kernel
+        launch_stiffness_v<VT, 12, TO_OP, false, false, 1, MAT>(c, first, count, u, val);
+    else if (DB)
         hipLaunchKernelGGL((k_stiffness_db<VT, EPB, TO_OP, MAT>), dim3(blocks), dim3(36 * EPB), 0, c->stream,
                            c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
                            TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
@@ -688,6 +910,32 @@ static void launch_stiffness_v(tg_ctx* c, int first, int count, const double* u,
         hipLaunchKernelGGL((k_stiffness_tl<VT, EPB, TO_OP, PM, SC, MAT>), dim3(blocks), dim3(36 * EPB), 0,
                            c->stream, c->dstiff_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p,
                            c->dX.p, u, TO_OP ? c->op.d_oslot.p : c->dslot.p, c->deqn.p, val, c->dstatus.p);
+}
+
+// Element-buffer pass of the gather assembly: the matrix-core shapes write
+// their tiles, every other shape runs the 12-element block kernel.
+template <class VT>
+static void launch_stiffness_gather(tg_ctx* c, int v, int first, int count, const double* u, VT* eb)
+{
+    const int blocks8 = (count + 7) / 8, blocks4 = (count + 3) / 4;
+    const int* el = c->dstiff_elems.p + first;
+    if (c->material_model == TG_MAT_STVENANT)
+        hipLaunchKernelGGL((k_stiffness_tl<VT, 12, false, false, 5, MAT_STVENANT>), dim3((count + 11) / 12),
+                           dim3(36 * 12), 0, c->stream, el, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                           c->dslot.p, c->deqn.p, eb, c->dstatus.p);
+    else if (v == 16)
+        hipLaunchKernelGGL((k_stiffness_mfma<VT, 8, MAT_SIMO, 1, true>), dim3(blocks8), dim3(64), 0, c->stream, el,
+                           count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->dslot.p, eb, c->dstatus.p);
+    else if (v == 17)
+        hipLaunchKernelGGL((k_stiffness_mfma<VT, 4, MAT_SIMO, 1, true>), dim3(blocks4), dim3(64), 0, c->stream, el,
+                           count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->dslot.p, eb, c->dstatus.p);
+    else if (v == 18)
+        hipLaunchKernelGGL((k_stiffness_mfma<VT, 8, MAT_SIMO, 3, true>), dim3(blocks8), dim3(64), 0, c->stream, el,
+                           count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->dslot.p, eb, c->dstatus.p);
+    else
+        hipLaunchKernelGGL((k_stiffness_tl<VT, 12, false, false, 5, MAT_SIMO>), dim3((count + 11) / 12),
+                           dim3(36 * 12), 0, c->stream, el, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
+                           c->dslot.p, c->deqn.p, eb, c->dstatus.p);
 }
 
 template <class VT, bool TO_OP>
@@ -717,15 +965,51 @@ static void launch_stiffness_shape(tg_ctx* c, int v, int first, int count, const
     case 12: TG_SHAPE(12); break;
     case 13: TG_SHAPE(13); break;
     case 14: TG_SHAPE(14); break;
-    default: TG_SHAPE(15); break;
+    case 15: TG_SHAPE(15); break;
+    case 16: TG_SHAPE(16); break;
+    case 17: TG_SHAPE(17); break;
+    default: TG_SHAPE(18); break;
     }
 #undef TG_SHAPE
 }
 
+// whether the element-buffer assembly runs; builds its lists and buffer for
+// the current launch order on first use (positions * 300 must fit an int)
+static bool gather_ready(tg_ctx* c)
+{
+    if (!c->stiff_gather || c->formulation != TG_TOTAL_LAGRANGIAN || 300L * c->nel >= (1L << 31)) return false;
+    stiffness_launches(c);
+    if (c->gather_built != c->stiff_chunk_built) {
+        build_gather_lists(c, c->dstiff_elems.p);
+        c->gather_built = c->stiff_chunk_built;
+    }
+    const size_t n = 300 * (size_t)c->nel;
+    if (mixed_stiffness(c)) {
+        if (c->debuf32.n != n) c->debuf32.alloc(n);
+    } else if (c->debuf.n != n) {
+        c->debuf.alloc(n);
+    }
+    return true;
+}
+
 void launch_stiffness(tg_ctx* c, const double* u)
 {
+    const bool gather = gather_ready(c);  // (setup on first use, outside the timed region)
     TimedRegion tr(c, T_STIFFNESS);
     const bool mixed = mixed_stiffness(c);
+    const int v = std::min(std::max(stiffness_shape(c), 0), N_STIFF_SHAPES - 1);
+    if (gather) {
+        // every MSR value is written by the gather: no clearing pass
+        for (const auto& fc : stiffness_launches(c)) {
+            if (mixed)
+                launch_stiffness_gather<float>(c, v, fc.first, fc.second, u, c->debuf32.p + 300L * fc.first);
+            else
+                launch_stiffness_gather<double>(c, v, fc.first, fc.second, u, c->debuf.p + 300L * fc.first);
+        }
+        TG_HIP(hipGetLastError());
+        launch_gather(c);
+        return;
+    }
     if (mixed)
         TG_HIP(hipMemsetAsync(c->dval32.p, 0, sizeof(float) * (size_t)c->msr_len, c->stream));
     else
@@ -734,7 +1018,6 @@ void launch_stiffness(tg_ctx* c, const double* u)
         launch_q1p0(c, u, c->dR.p, true);
         return;
     }
-    const int v = std::min(std::max(c->stiff_variant, 0), N_STIFF_SHAPES - 1);
     for (const auto& fc : stiffness_launches(c)) {
         if (mixed)
             launch_stiffness_shape<float, false>(c, v, fc.first, fc.second, u, c->dval32.p);
@@ -755,7 +1038,7 @@ void launch_stiffness_operator(tg_ctx* c, const double* u)
         TG_HIP(hipMemsetAsync(c->op.d_val32.p, 0, sizeof(float) * nv, c->stream));
     else
         TG_HIP(hipMemsetAsync(c->op.d_val.p, 0, sizeof(double) * nv, c->stream));
-    const int v = std::min(std::max(c->stiff_variant, 0), N_STIFF_SHAPES - 1);
+    const int v = std::min(std::max(stiffness_shape(c), 0), N_STIFF_SHAPES - 1);
     for (const auto& fc : stiffness_launches(c)) {
         if (mixed)
             launch_stiffness_shape<float, true>(c, v, fc.first, fc.second, u, c->op.d_val32.p);
@@ -962,194 +1245,6 @@ void launch_build_slots(tg_ctx* c)
     const int bs = 256;
     hipLaunchKernelGGL(k_build_slots, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nel,
                        c->dconn.p, c->deqn.p, c->dbindx.p, c->dslot.p);
-    TG_HIP(hipGetLastError());
-}
-
-}  // namespace tg
-
-// ---------------------------------------------------------------------------
-// Q1P0 mean-dilatation hexahedron (SimoQ1P0, tahoe/src/elements/continuum/solid/SimoQ1P0.cpp)
-// ---------------------------------------------------------------------------
-
-namespace tg {
-
-__device__ __forceinline__ double allreduce8(double v)
-{
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
-    v += __shfl_xor(v, 4);
-    return v;
-}
-
-// Per-lane (integration point q) share of the element internal force
-// f = sum_q w det_q (J_bar/J_q) B_bar_q^T sigma(F_bar_q) for the state in
-// LDS (x current coordinates, u displacements), ComputeInternalForce
-// (SimoQ1P0.cpp:390-439) with E = 0 and no phase field:
-//   SetMeanGradient (:661-686): H = sum w det0, v = sum w det, mean = sum (w det/v) dN/dx
-//   F_bar = (v/(H J))^(1/3) F, F from reference derivatives (:286-296)
-//   B_bar from SolidElementT::Set_B_bar (SolidElementT.cpp:1011-1041).
-// All 8 lanes of the element must call it together (wave-wide shuffles).
-template <int MAT, class XF, class UF>
-__device__ __forceinline__ bool q1p0_ip_force(int q, const double d[3][8], const double det0, const double g0[3][8],
-                                              XF xcur, UF u, double mu, double kappa, double f[24])
-{
-    bool ok = true;
-    double jc[9], jci[9], dnx[3][8];
-    jacobian(xcur, d, jc);
-    const double det = det3(jc);
-    ok &= det > 0.0;
-    inv3(jc, det, jci);
-    spatial_derivs(jci, d, dnx);
-    const double H = allreduce8(det0);  // weights 1
-    const double v = allreduce8(det);
-    const double wv = det / v;
-    double mean[3][8];
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int a = 0; a < 8; a++) mean[k][a] = allreduce8(wv * dnx[k][a]);
-    double F[9];
-    deformation_gradient(u, g0, F);
-    const double J = det3(F);
-    ok &= J > 0.0;
-    const double sc = cbrt(v / (H * J));
-#pragma unroll
-    for (int k = 0; k < 9; k++) F[k] *= sc;
-    // the material sees F_bar (SimoQ1P0 passes it on as the mechanical F)
-    const double Jbar = det3(F);
-    double Jk, kin[6], sig[6];
-    ok &= material_stress<MAT>(mu, kappa, F, Jbar, Jk, kin, sig);
-    const double s = det * (Jbar / J);  // w det_q J_bar/J_q, weight 1
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-        const double nx = dnx[0][a], ny = dnx[1][a], nz = dnx[2][a];
-        const double fx = (mean[0][a] - nx) / 3.0, fy = (mean[1][a] - ny) / 3.0, fz = (mean[2][a] - nz) / 3.0;
-        const double bx = (nx + fx) * sig[0] + fx * sig[1] + fx * sig[2] + nz * sig[4] + ny * sig[5];
-        const double by = fy * sig[0] + (ny + fy) * sig[1] + fy * sig[2] + nz * sig[3] + nx * sig[5];
-        const double bz = fz * sig[0] + fz * sig[1] + (nz + fz) * sig[2] + ny * sig[3] + nx * sig[4];
-        f[3 * a + 0] = s * bx;
-        f[3 * a + 1] = s * by;
-        f[3 * a + 2] = s * bz;
-    }
-    if (!ok)
-#pragma unroll
-        for (int k = 0; k < 24; k++) f[k] = 0.0;
-    return ok;
-}
-
-// Q1P0 residual R -= f (FormKd(-1), SimoQ1P0.cpp:528-582) and, when `stiff`,
-// the reference's forward-difference tangent (FormStiffness, SimoQ1P0.cpp:446-525):
-// for each element dof col, u_col += eps and x_col += eps, K(:,col) =
-// (f(u+eps) - f(u))/eps, then both are restored by subtracting eps, eps = 1e-7.
-// Upper entries (row <= col) are scattered through the slot map.
-template <bool STIFF, int MAT>
-__global__ void __launch_bounds__(256) k_q1p0(const int* __restrict__ elems, int count, const int* __restrict__ conn,
-                                              const int* __restrict__ emat, const double2* __restrict__ props,
-                                              const double* __restrict__ X, const double* __restrict__ U,
-                                              const int* __restrict__ eqn, double* __restrict__ R,
-                                              const int* __restrict__ slot, double* __restrict__ val,
-                                              int* __restrict__ status)
-{
-    __shared__ double sX[32][8][3], sx[32][8][3], su[32][8][3];
-    const int t = threadIdx.x, le = t >> 3, q = t & 7;
-    const long sl = (long)blockIdx.x * 32 + le;
-    const bool valid = sl < count;
-    int e = 0, node = 0;
-    if (valid) {
-        e = elems[sl];
-        node = conn[8L * e + q];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const double Xc = X[3L * node + c], uc = U[3L * node + c];
-            sX[le][q][c] = Xc;
-            su[le][q][c] = uc;
-            sx[le][q][c] = Xc + uc;
-        }
-    } else {
-        // inert unit element so the shuffled element reductions stay finite
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const double Xc = c == 0 ? (RA[q] + 1.0) * 0.5 : (c == 1 ? (SA[q] + 1.0) * 0.5 : (TA[q] + 1.0) * 0.5);
-            sX[le][q][c] = Xc;
-            su[le][q][c] = 0.0;
-            sx[le][q][c] = Xc;
-        }
-    }
-    __syncthreads();
-    double2 mk = make_double2(1.0, 1.0);
-    if (valid) mk = props[emat[e]];
-    double d[3][8], j0[9], ji[9], g0[3][8];
-    parent_derivs(q, d);
-    jacobian([&](int a, int c) { return sX[le][a][c]; }, d, j0);
-    const double det0 = det3(j0);
-    bool ok = det0 > 0.0;
-    inv3(j0, det0, ji);
-    spatial_derivs(ji, d, g0);
-    auto xa = [&](int a, int c) { return sx[le][a][c]; };
-    auto ua = [&](int a, int c) { return su[le][a][c]; };
-    double f[24], v0[3];
-    ok &= q1p0_ip_force<MAT>(q, d, det0, g0, xa, ua, mk.x, mk.y, f);
-    reduce_scatter8(f, q, v0);
-    if (!STIFF) {
-        if (valid)
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-                if (eqn[3L * node + i] >= 0) R[3L * node + i] -= v0[i];
-    } else {
-        const double eps = 1.0e-7;
-        const int* es = slot + 300L * e;
-        for (int col = 0; col < 24; col++) {
-            const int pa = col / 3, pi = col % 3;
-            if (q == pa) {
-                su[le][pa][pi] += eps;
-                sx[le][pa][pi] += eps;
-            }
-            __syncthreads();
-            double fp[24], vp[3];
-            ok &= q1p0_ip_force<MAT>(q, d, det0, g0, xa, ua, mk.x, mk.y, fp);
-            reduce_scatter8(fp, q, vp);
-            __syncthreads();
-            if (q == pa) {
-                su[le][pa][pi] -= eps;
-                sx[le][pa][pi] -= eps;
-            }
-            if (valid) {
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    const int m = 3 * q + i;
-                    if (m <= col) {
-                        const int s = slot_target(es[upper_index(m, col)]);
-                        if (s >= 0) val[s] += (vp[i] - v0[i]) / eps;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if (valid && !ok) atomicOr(status, 7);
-}
-
-void launch_q1p0(tg_ctx* c, const double* u, double* R_node, bool stiff)
-{
-    for (int col = 0; col < c->ncolors; col++) {
-        const int first = c->color_ptr[col], count = c->color_ptr[col + 1] - first;
-        if (count == 0) continue;
-        const int blocks = (count + 31) / 32;
-#define TG_Q1P0(S, M)                                                                                          \
-    hipLaunchKernelGGL((k_q1p0<S, M>), dim3(blocks), dim3(256), 0, c->stream, c->dcolor_elems.p + first, count, \
-                       c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->deqn.p, R_node, c->dslot.p, c->dval.p, \
-                       c->dstatus.p)
-        const bool stv = c->material_model == TG_MAT_STVENANT;
-        if (stiff && stv)
-            TG_Q1P0(true, MAT_STVENANT);
-        else if (stiff)
-            TG_Q1P0(true, MAT_SIMO);
-        else if (stv)
-            TG_Q1P0(false, MAT_STVENANT);
-        else
-            TG_Q1P0(false, MAT_SIMO);
-#undef TG_Q1P0
-    }
     TG_HIP(hipGetLastError());
 }
 

@@ -455,6 +455,11 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         c->ncolors = color_elements(c->nnd, c->nel, c->conn.data(), c->color_ptr, c->color_elems);
         phase("colouring");
         c->stiff_chunk_built = -2;
+        c->gather_built = -2;
+        c->dgptr.release();
+        c->dglist.release();
+        c->debuf.release();
+        c->debuf32.release();
         // device storage
         const size_t nn = 3 * (size_t)c->nnd;
         c->dX.alloc(nn);
@@ -1029,7 +1034,7 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
             if (value < -1) fail(TG_BAD_INPUT, "stiffness chunk must be -1, 0 or positive");
             c->stiff_chunk = value;
         } else if (knob == TG_TUNE_STIFF_SHAPE) {
-            if (value < 0 || value > 15) fail(TG_OUT_OF_RANGE, "stiffness shape must be 0..15");
+            if (value < 0 || value > 18) fail(TG_OUT_OF_RANGE, "stiffness shape must be 0..18");
             c->stiff_variant = (int)value;
         } else if (knob == TG_TUNE_NEWTON_ASSEMBLY) {
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "Newton assembly must be 0 or 1");
@@ -1043,6 +1048,9 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
         } else if (knob == TG_TUNE_PCG_FUSED) {
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "PCG fused must be 0 or 1");
             c->pcg_fused = (int)value;
+        } else if (knob == TG_TUNE_STIFF_GATHER) {
+            if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "stiffness gather must be 0 or 1");
+            c->stiff_gather = (int)value;
         } else if (knob == TG_TUNE_AMG_DEGREE) {
             if (value < 2 || value > 4) fail(TG_OUT_OF_RANGE, "multigrid smoother degree must be 2..4");
             c->amg.degree = (int)value;
@@ -1063,7 +1071,7 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
         if (knob == TG_TUNE_STIFF_CHUNK)
             *value = c->stiff_chunk;
         else if (knob == TG_TUNE_STIFF_SHAPE)
-            *value = c->stiff_variant;
+            *value = stiffness_shape(c);
         else if (knob == TG_TUNE_NEWTON_ASSEMBLY)
             *value = c->newton_assembly;
         else if (knob == TG_TUNE_PCG_GRAPH)
@@ -1072,6 +1080,8 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
             *value = c->spmv_xcd;
         else if (knob == TG_TUNE_PCG_FUSED)
             *value = c->pcg_fused;
+        else if (knob == TG_TUNE_STIFF_GATHER)
+            *value = c->stiff_gather;
         else if (knob == TG_TUNE_AMG_DEGREE)
             *value = c->amg.degree;
         else if (knob == TG_TUNE_AMG_WEIGHT)

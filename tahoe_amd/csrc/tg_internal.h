@@ -214,7 +214,8 @@ struct tg_ctx {
     // stiff_chunk (by rank inside the segment), all colours of range j
     // before range j+1 (0: whole colours)
     long stiff_chunk = -1;              // -1: chosen from the matrix footprint
-    int stiff_variant = 9;              // block shape of the stiffness kernel (9: 12 elements, loads first)
+    int stiff_variant = -1;             // shape of the stiffness kernel; -1: 17 (matrix cores, 4 elements
+                                        // per wave) for the FP32 stiffness, 9 (12 elements, loads first) for FP64
     int newton_assembly = 0;            // 0: Newton's K into the MSR then expanded, 1: straight into the operator
     int precond = TG_PC_JACOBI;         // preconditioner of the inner PCG
     int linear_solver = TG_LIN_PCG;     // inner linear solver of the Newton loop
@@ -230,6 +231,16 @@ struct tg_ctx {
     } pg;                               // the captured chunk and the arguments it was captured with
     hipEvent_t poll_ev[2] = {nullptr, nullptr};
     long stiff_chunk_built = -2;
+    // stiffness assembly through the element buffer (TG_TUNE_STIFF_GATHER):
+    // the element kernels write their 300 upper entries to ebuf[position in
+    // launch order][300]; one gather pass sums each MSR value's
+    // contributions, listed in launch order, and writes it once
+    int stiff_gather = 0;
+    long gather_built = -2;             // the launch order (chunk) the lists were built for
+    tg::DBuf<int> dgptr;                // [msr_len + 1] first list entry of each MSR value
+    tg::DBuf<int> dglist;               // [contributions] launch position * 300 + element entry
+    tg::DBuf<double> debuf;             // [nel][300] element upper entries (FP64)
+    tg::DBuf<float> debuf32;            // [nel][300] element upper entries (FP32, mixed precision)
     std::vector<std::pair<int, int>> stiff_launches;  // (first, count) into dstiff_elems
 
     // device storage
@@ -274,6 +285,8 @@ namespace tg {
 // FP32 for every element in mixed precision.
 inline bool mixed_stiffness(const tg_ctx* c) { return c->precision == TG_MIXED && c->formulation == TG_TOTAL_LAGRANGIAN; }
 inline bool mixed_operator(const tg_ctx* c) { return c->precision == TG_MIXED; }
+// the stiffness shape in effect (the measured fastest untuned choice at 128^3)
+inline int stiffness_shape(const tg_ctx* c) { return c->stiff_variant >= 0 ? c->stiff_variant : (mixed_stiffness(c) ? 17 : 9); }
 
 // host setup (tg_setup.cpp)
 int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
@@ -285,6 +298,10 @@ void build_bsell(tg_ctx* c);
 // kernels launched from the ABI (k_element.hip, k_solver.hip)
 void launch_residual(tg_ctx* c, const double* u, double* R_node);
 void launch_stiffness(tg_ctx* c, const double* u);
+// gather lists of the element-buffer assembly (k_gather.hip) for the
+// current launch order; dorder = elements in launch order
+void build_gather_lists(tg_ctx* c, const int* dorder);
+void launch_gather(tg_ctx* c);
 void launch_stiffness_operator(tg_ctx* c, const double* u);
 void launch_build_operator_slots(tg_ctx* c);
 void launch_build_src(tg_ctx* c);

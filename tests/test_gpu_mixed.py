@@ -25,11 +25,12 @@ def mixed_solid(g):
     return s
 
 
-@pytest.mark.parametrize("shape", [0, 6, 9, 15])
+@pytest.mark.parametrize("shape", [0, 6, 9, 15, 16, 17, 18])
 def test_mixed_stiffness_is_fp32_rounding_of_fp64(shape):
     """In several stiffness shapes: element-major lanes (0), the D*B form
     (6), loads issued before the block products (9), block-major lanes with
-    atomic additions (15)."""
+    atomic additions (15), FP32 matrix-core tiles with 8 and 4 elements per
+    wave (16, 17)."""
     from tahoe_amd import api
     g = load_golden("tl_n8")
     s64, s32 = make_solid(g), mixed_solid(g)
@@ -48,11 +49,15 @@ def test_mixed_stiffness_is_fp32_rounding_of_fp64(shape):
     assert np.array_equal(R64, R32)
 
 
+@pytest.mark.parametrize("shape", [9, 16])
 @pytest.mark.parametrize("name", ["tl_n8", "tl_n4_s20", "rve_n4", "tl_n16"])
-def test_mixed_newton_reaches_fp64_solution(name):
+def test_mixed_newton_reaches_fp64_solution(name, shape):
+    """Newton on the FP64 residual with the FP32 tangent of the block
+    kernel (shape 9) and of the matrix-core kernel (shape 16)."""
     from tahoe_amd import api
     g = load_golden(name)
     s = mixed_solid(g)
+    s.set_tuning(api.TUNE_STIFF_SHAPE, shape)
     opts = api.Solid.opts(lin_rtol=1e-6)
     step_u, iters = api.static_solve(s, int(g["num_steps"]), opts)
     for k in range(step_u.shape[0]):

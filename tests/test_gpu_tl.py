@@ -163,7 +163,7 @@ def test_large_mesh_against_oracle_and_invariants():
     assert abs(y @ s.multx(x) - x @ s.multx(y)) <= 1e-10 * np.linalg.norm(x) * np.linalg.norm(y) * np.abs(vo).max()
 
 
-@pytest.mark.parametrize("shape", list(range(16)))
+@pytest.mark.parametrize("shape", list(range(19)))
 @pytest.mark.parametrize("chunk", [0, 100])
 def test_stiffness_every_launch_shape(shape, chunk):
     """Every compiled workgroup shape and launch-range split assembles the
@@ -174,7 +174,7 @@ def test_stiffness_every_launch_shape(shape, chunk):
     from tahoe_amd import api
     g = load_golden("tl_n8")
     s = make_solid(g)
-    assert len(api.STIFF_SHAPE_EPB) == 16
+    assert len(api.STIFF_SHAPE_EPB) == 19
     with pytest.raises(api.TahoeGpuError):
         s.set_tuning(api.TUNE_STIFF_SHAPE, len(api.STIFF_SHAPE_EPB))
     s.set_tuning(api.TUNE_STIFF_SHAPE, shape)
@@ -194,7 +194,9 @@ def test_stiffness_every_launch_shape(shape, chunk):
 def test_operator_assembly_equals_msr_expansion(name, precision):
     """Newton's tangent assembled straight into the solver operator gives
     the operator that expanding the assembled MSR gives (same sums, same
-    order): the Newton corrections agree bitwise, prescribed dofs included."""
+    order): the Newton corrections agree bitwise, prescribed dofs included.
+    Both paths run the same kernel (shape 9; the matrix-core shapes assemble
+    the MSR only)."""
     from tahoe_amd import api
     g = load_golden(name)
     out = []
@@ -203,6 +205,7 @@ def test_operator_assembly_equals_msr_expansion(name, precision):
         if precision:
             s.set_precision(api.MIXED)
             s.initialize()
+        s.set_tuning(api.TUNE_STIFF_SHAPE, 9)
         s.set_tuning(api.TUNE_NEWTON_ASSEMBLY, mode)
         s.apply_kbc(1.0 / int(g["num_steps"]))
         st = s.newton_correction(s.opts(lin_rtol=1e-10))
@@ -559,3 +562,63 @@ def test_form_tangent_assembles_newtons_tangent():
         st_f = f.newton_correction(f.opts(lin_rtol=1e-12))
         assert st.lin_iterations == st_f.lin_iterations and np.array_equal(u1, f.displacement())
         f.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("shape", [0, 9, 16, 17, 18])
+@pytest.mark.parametrize("chunk", [0, 100])
+def test_gather_assembly_equals_scatter(precision, shape, chunk):
+    """The element-buffer assembly (TUNE_STIFF_GATHER = 1: element entries
+    written in launch order, one gather pass per MSR value) gives the MSR
+    values of the colour-ordered scatter bitwise, in FP64 and FP32, for the
+    block and the matrix-core kernels and for split launch ranges."""
+    from tahoe_amd import api
+    g = load_golden("tl_n8")
+    out = []
+    for gather in (0, 1):
+        s = make_solid(g) if precision == 0 else None
+        if s is None:
+            s = api.Solid(0)
+            s.set_mesh(g["X"], g["conn"], g["mat"])
+            s.set_material(0, *api.mu_kappa(1000.0, 0.3))
+            s.set_precision(api.MIXED)
+            s.set_kbc(g["kbc_nodes"], g["kbc_dofs"], g["kbc_values"])
+            s.initialize()
+        s.set_tuning(api.TUNE_STIFF_SHAPE, shape)
+        s.set_tuning(api.TUNE_STIFF_CHUNK, chunk)
+        s.set_tuning(api.TUNE_STIFF_GATHER, gather)
+        assert s.tuning(api.TUNE_STIFF_GATHER) == gather
+        s.set_displacement(g["step_u"][1])
+        s.form_lhs()
+        s.form_lhs()  # a second pass overwrites every value
+        out.append(s.msr_values())
+        s.close()
+    assert np.array_equal(out[0], out[1])
+
+
+def test_gather_assembly_newton_and_batch():
+    """Newton with the gather assembly reaches the reference's converged
+    displacements with its Newton counts; a batch of RVEs assembles the
+    same values with either assembly."""
+    from tahoe_amd import api, batch, rve
+    g = load_golden("tl_n16")
+    s = make_solid(g)
+    s.set_tuning(api.TUNE_STIFF_GATHER, 1)
+    s.set_tuning(api.TUNE_STIFF_SHAPE, 16)
+    step_u, iters = api.static_solve(s, int(g["num_steps"]), s.opts(lin_rtol=1e-12))
+    s.close()
+    for k in range(step_u.shape[0]):
+        ref = g["step_u"][k]
+        assert np.abs(step_u[k] - ref).max() <= 1e-9 * np.abs(ref).max(), k
+    assert list(iters) == g["step_iters"].tolist()
+    reqs = rve.batch_requests(12, seed=77)
+    pk = batch.pack([r.problem() for r in reqs])
+    vals = []
+    for gather in (0, 1):
+        b = batch.make_solid(pk, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64)
+        b.set_tuning(api.TUNE_STIFF_GATHER, gather)
+        b.apply_kbc(1.0)
+        b.form_lhs()
+        vals.append(b.msr_values())
+        b.close()
+    assert np.array_equal(vals[0], vals[1])
