@@ -67,6 +67,9 @@ def parse_args():
                          "FP64 residual)")
     ap.add_argument("--mixed-checkpoints", type=int, default=10,
                     help="mixed-steps: solve the first K of the 10 load-step checkpoints (default all)")
+    ap.add_argument("--fp64-lin-rtol", type=float, default=1e-10,
+                    help="mixed-steps: relative tolerance of the FP64 leg's inner PCG (the same as --mixed-lin-rtol "
+                         "compares the precisions like for like)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: every SpMV variant timed, the full correction capped at 480 "
                          "PCG iterations, no CPU baseline")
@@ -477,13 +480,21 @@ def base_line(value, unit, steps, warmup, ms_per_step, dtype, config, higher=Tru
 
 def bench_mixed_steps(a):
     """The 20% stretch in 10 load steps (static Newton to 1e-10 relative
-    residual every step), solved in FP64 and in mixed precision on the same
-    mesh; relative displacement error of mixed vs FP64 after every step."""
+    residual every step, step cutting), solved in FP64 and in mixed precision
+    on the same mesh with the same preconditioner; the two legs advance
+    checkpoint by checkpoint side by side, and the relative displacement
+    error of mixed vs FP64 is reported after every checkpoint.  The inner
+    PCG tolerance of each leg is an option (the same for both legs gives the
+    like-for-like precision comparison).  A time budget ends the run after
+    the last whole checkpoint, and the line says how many were solved."""
     from tahoe_amd import api, mesh as M
     n = a.mixed_n
+    dl = Deadline(0)
+    t_start = time.perf_counter()
     m = M.cube(n, seed=2024)
     kbc = M.kbc_uniaxial(m, 0.2)
-    res = {}
+    legs = {}
+    rtol = {"fp64": a.fp64_lin_rtol, "mixed": a.mixed_lin_rtol}
     for prec in ("fp64", "mixed"):
         s = api.Solid(0)
         s.set_mesh(m.X, m.conn, m.mat)
@@ -492,76 +503,98 @@ def bench_mixed_steps(a):
         s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
         s.set_preconditioner(PRECOND[a.precond])
         n_eq, _ = s.initialize()
-        print(f"# {prec}: context ready ({n_eq} equations)", file=sys.stderr, flush=True)
+        log(f"{prec}: context ready ({n_eq} equations)")
         # large meshes: an inner solve is capped at 15,000 iterations (about a
         # minute with the multigrid at 128^3), so no correction runs unbounded
-        opts = s.opts(lin_rtol=1e-10 if prec == "fp64" else a.mixed_lin_rtol, lin_maxit=15000 if n >= 96 else 200000)
+        opts = s.opts(lin_rtol=rtol[prec], lin_maxit=15000 if n >= 96 else 200000)
         # per-kernel event timing on small meshes only: at 96^3 and above the
         # solve launches millions of kernels, and the line reports wall time
         s.timing_enable(n < 96)
-        t0 = time.perf_counter()
-        lin = []
-
-        def on_step(k, st, prec=prec, t0=t0, lin=lin):
-            lin.append(st.lin_iterations)
-            print(f"# {prec} step {k}: Newton {st.iterations}, PCG {st.lin_iterations}, "
-                  f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-
-        def log(msg, prec=prec):
-            print(f"# {prec} {msg}", file=sys.stderr, flush=True)
-
-        # large meshes: report every Newton correction (each is a long solve)
-        us, its, subs = api.static_solve_adaptive(s, 10, opts, on_step=on_step, log=log, last=a.mixed_checkpoints,
-                                                  log_corrections=n >= 96)
-        s.synchronize()
-        wall = time.perf_counter() - t0
-        res[prec] = {"u": us, "newton": its, "substeps": subs, "pcg": lin, "wall_s": wall, "timing": s.timing()}
-        print(f"# {prec}: {wall:.2f} s, Newton per step {its}", file=sys.stderr, flush=True)
-        s.close()
-    err = [float(np.abs(um - ud).max() / np.abs(ud).max()) for um, ud in zip(res["mixed"]["u"], res["fp64"]["u"])]
-    tot = lambda r: r["timing"]["residual_ms"] + r["timing"]["stiffness_ms"]
-    kk = len(err)
+        path = api.LoadPath(s, 10, opts, log=lambda msg, prec=prec: log(f"{prec} {msg}"),
+                            log_corrections=n >= 96)
+        legs[prec] = {"solid": s, "path": path, "u": None, "newton": [], "substeps": [], "pcg": [], "wall_s": 0.0}
+    errs = []
+    last = min(a.mixed_checkpoints, 10)
+    per_checkpoint_s = []
+    for k in range(1, last + 1):
+        tk0 = time.perf_counter()
+        for prec in ("fp64", "mixed"):
+            L = legs[prec]
+            t0 = time.perf_counter()
+            u, nn, ns, lin = L["path"].advance()
+            L["solid"].synchronize()
+            L["wall_s"] += time.perf_counter() - t0
+            L["u"] = u
+            L["newton"].append(nn)
+            L["substeps"].append(ns)
+            L["pcg"].append(lin)
+            log(f"{prec} checkpoint {k}: Newton {nn}, sub-steps {ns}, PCG {lin}, leg total {L['wall_s']:.1f} s")
+        ud, um = legs["fp64"]["u"], legs["mixed"]["u"]
+        errs.append(float(np.abs(um - ud).max() / np.abs(ud).max()))
+        per_checkpoint_s.append(time.perf_counter() - tk0)
+        log(f"checkpoint {k}: mixed vs FP64 max relative error {errs[-1]:.3e} "
+            f"({time.perf_counter() - t_start:.0f} s since start)")
+        left = a.time_budget - (time.perf_counter() - t_start)
+        if k < last and left < 1.5 * per_checkpoint_s[-1]:
+            log(f"stopping after checkpoint {k}: {left:.0f} s of the time budget left, the last checkpoint "
+                f"took {per_checkpoint_s[-1]:.0f} s")
+            break
+    timing = {}
+    for prec in legs:
+        legs[prec]["solid"].synchronize()
+        timing[prec] = legs[prec]["solid"].timing()
+        legs[prec]["solid"].close()
+    kk = len(errs)
+    tot = lambda t: t["residual_ms"] + t["stiffness_ms"]
     timed = n < 96
+    mixed, fp64 = legs["mixed"], legs["fp64"]
     # elements/s assembled (R+K device time) when timed, else elements x Newton
     # corrections per second of wall time for the whole mixed solve
-    value = (m.nel * sum(res["mixed"]["newton"]) / (tot(res["mixed"]) * 1e-3) if timed else
-             m.nel * sum(res["mixed"]["newton"]) / res["mixed"]["wall_s"])
-    line = base_line(value, "elements/s", kk, 0,
-                     res["mixed"]["wall_s"] * 1e3 / kk, "f32/f64",
+    value = (m.nel * sum(mixed["newton"]) / (tot(timing["mixed"]) * 1e-3) if timed else
+             m.nel * sum(mixed["newton"]) / mixed["wall_s"])
+    line = base_line(value, "elements/s", kk, 0, mixed["wall_s"] * 1e3 / kk, "f32/f64",
                      {"workload": f"{n}^3 Q1 hex TL neo-Hookean, +-5% perturbed, uniaxial 20% in 10 load steps "
-                                  f"({'all' if kk == 10 else f'the first {kk}'} solved), "
-                                  "(a load step that fails is cut in half and retried), "
-                                  f"Newton to 1e-10; mixed: FP32 K and operator, PCG to {a.mixed_lin_rtol:g} inside FP64 "
-                                  f"residual/Newton; FP64: PCG to 1e-10; preconditioner {a.precond}",
-                      "elements": m.nel, "equations": n_eq, "preconditioner": a.precond})
+                                  f"({'all' if kk == 10 else f'the first {kk}'} solved; a load step that fails is "
+                                  f"cut in half and retried), Newton to 1e-10 on the FP64 residual; preconditioner "
+                                  f"{a.precond} in both legs; inner PCG to {a.fp64_lin_rtol:g} (FP64 leg) and "
+                                  f"{a.mixed_lin_rtol:g} (mixed leg: FP32 K and operator)",
+                      "elements": m.nel, "equations": n_eq, "preconditioner": a.precond,
+                      "inner_rtol": {"fp64": a.fp64_lin_rtol, "mixed": a.mixed_lin_rtol}})
     line.update({"value_is": ("elements x Newton corrections / (R + K device time), mixed" if timed else
                               "elements x Newton corrections / wall time of the whole mixed solve"),
-                 "rel_err_vs_fp64_per_step": err, "max_rel_err": max(err),
-                 "newton_per_step": {"mixed": res["mixed"]["newton"], "fp64": res["fp64"]["newton"]},
-                 "pcg_per_step": {"mixed": res["mixed"]["pcg"], "fp64": res["fp64"]["pcg"]},
-                 "substeps_per_step": {"mixed": res["mixed"]["substeps"], "fp64": res["fp64"]["substeps"]},
-                 "wall_s": {"mixed": res["mixed"]["wall_s"], "fp64": res["fp64"]["wall_s"]},
-                 "speedup_wall": res["fp64"]["wall_s"] / res["mixed"]["wall_s"],
-                 "timing_ms": {k: res[k]["timing"] for k in res}})
-    print(json.dumps(line), flush=True)
+                 "checkpoints_solved": kk,
+                 "rel_err_vs_fp64_per_step": errs, "max_rel_err": max(errs),
+                 "newton_per_step": {"mixed": mixed["newton"], "fp64": fp64["newton"]},
+                 "pcg_per_step": {"mixed": mixed["pcg"], "fp64": fp64["pcg"]},
+                 "substeps_per_step": {"mixed": mixed["substeps"], "fp64": fp64["substeps"]},
+                 "wall_s": {"mixed": mixed["wall_s"], "fp64": fp64["wall_s"]},
+                 "speedup_wall": fp64["wall_s"] / mixed["wall_s"],
+                 "timing_ms": timing})
+    dl.emit(line)
 
 
 def bench_newton_step(a):
     """One full Newton load step on the n^3 cube (Newton to 1e-10 relative
-    residual from u = 0, top face raised by --step-stretch of the height) on
-    the plain FP64 path (FP64 operator, Jacobi-PCG to 1e-10, the bench line's
-    solver) and on the mixed path (FP32 stiffness and operator, multigrid-PCG
-    to 1e-3 inside Newton on the FP64 residual); wall time of the whole step,
-    elements/s = elements / that time, relative displacement error of mixed
-    vs FP64."""
+    residual on the FP64 residual from u = 0, top face raised by
+    --step-stretch of the height) on four paths that separate what the
+    solver and the precision contribute:
+      plain_fp64      FP64 operator, Jacobi-PCG to 1e-10 (the bench line's solver);
+      fp64_jacobi_1e-3   FP64, Jacobi-PCG to 1e-3 (inexact Newton);
+      fp64_multigrid_1e-3  FP64, multigrid-PCG to 1e-3 (+ preconditioner);
+      mixed           FP32 stiffness and operator, multigrid-PCG to 1e-3 (+ precision).
+    Wall time of the whole step, elements/s = elements / that time, relative
+    displacement error of each path vs plain FP64."""
     from tahoe_amd import api, mesh as M
     n = a.step_n
     m = M.cube(n, seed=2024)
     kbc = M.kbc_uniaxial(m, a.step_stretch)
     res = {}
     # inner solves capped (100,000 Jacobi iterations is about 100 s at 128^3)
-    for leg, prec, pc, rt, cap in (("plain_fp64", api.FP64, api.PC_JACOBI, 1e-10, 100000),
-                                   ("mixed", api.MIXED, api.PC_AMG, 1e-3, 20000)):
+    legs = (("plain_fp64", api.FP64, api.PC_JACOBI, 1e-10, 100000),
+            ("fp64_jacobi_1e-3", api.FP64, api.PC_JACOBI, 1e-3, 100000),
+            ("fp64_multigrid_1e-3", api.FP64, api.PC_AMG, 1e-3, 20000),
+            ("mixed", api.MIXED, api.PC_AMG, 1e-3, 20000))
+    for leg, prec, pc, rt, cap in legs:
         s = api.Solid(0)
         s.set_mesh(m.X, m.conn, m.mat)
         s.set_material(0, *api.mu_kappa(1000.0, 0.3))
@@ -569,38 +602,39 @@ def bench_newton_step(a):
         s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
         s.set_preconditioner(pc)
         n_eq, _ = s.initialize()
-        print(f"# {leg}: context ready ({n_eq} equations)", file=sys.stderr, flush=True)
+        log(f"{leg}: context ready ({n_eq} equations)")
         s.apply_kbc(1.0)
         s.synchronize()
         t0 = time.perf_counter()
         st = s.newton_stepwise(
             s.opts(lin_rtol=rt, lin_maxit=cap),
-            log=lambda i, rel, pcg, leg=leg, t0=t0: print(
-                f"# {leg} correction {i}: |R|/|R0| {rel:.3e}, PCG {pcg}, {time.perf_counter() - t0:.1f} s",
-                file=sys.stderr, flush=True))
+            log=lambda i, rel, pcg, leg=leg, t0=t0: log(
+                f"{leg} correction {i}: |R|/|R0| {rel:.3e}, PCG {pcg}, {time.perf_counter() - t0:.1f} s"))
         s.synchronize()
         wall = time.perf_counter() - t0
         if st.status != 0:
             raise RuntimeError(f"{leg}: Newton did not converge ({st.iterations} corrections)")
         res[leg] = {"u": s.displacement(), "wall_s": wall, "newton": st.iterations, "pcg": st.lin_iterations}
         s.close()
-    ud, um = res["plain_fp64"]["u"], res["mixed"]["u"]
-    err = float(np.abs(um - ud).max() / np.abs(ud).max())
-    value = m.nel / res["mixed"]["wall_s"]
-    line = base_line(value, "elements/s", 1, 0, res["mixed"]["wall_s"] * 1e3, "f32/f64",
+    ud = res["plain_fp64"]["u"]
+    for leg in res:
+        res[leg]["rel_err_vs_plain_fp64"] = float(np.abs(res[leg]["u"] - ud).max() / np.abs(ud).max())
+        res[leg]["elements_per_s"] = m.nel / res[leg]["wall_s"]
+    w = {leg: res[leg]["wall_s"] for leg in res}
+    value = m.nel / w["mixed"]
+    line = base_line(value, "elements/s", 1, 0, w["mixed"] * 1e3, "f32/f64",
                      {"workload": f"{n}^3 Q1 hex TL neo-Hookean (Simo_isotropic E=1000 nu=0.3), +-5% perturbed, one "
                                   f"load step: top face raised by {a.step_stretch:g} of the height from u = 0, Newton "
-                                  "to 1e-10 relative residual. Mixed: FP32 K and operator, multigrid-PCG to 1e-3 "
-                                  "inside Newton on the FP64 residual; plain FP64: Jacobi-PCG to 1e-10",
+                                  "to 1e-10 relative residual on the FP64 residual; four solver paths (see legs)",
                       "elements": m.nel})
     line.update({"value_is": "elements / wall time of the whole Newton step, mixed path",
-                 "plain_fp64": {"elements_per_s": m.nel / res["plain_fp64"]["wall_s"],
-                                "wall_s": res["plain_fp64"]["wall_s"], "newton": res["plain_fp64"]["newton"],
-                                "pcg": res["plain_fp64"]["pcg"]},
-                 "mixed": {"wall_s": res["mixed"]["wall_s"], "newton": res["mixed"]["newton"],
-                           "pcg": res["mixed"]["pcg"]},
-                 "speedup_vs_plain_fp64": res["plain_fp64"]["wall_s"] / res["mixed"]["wall_s"],
-                 "rel_err_vs_fp64": err})
+                 "legs": {leg: {k: v for k, v in res[leg].items() if k != "u"} for leg in res},
+                 "speedup_vs_plain_fp64": w["plain_fp64"] / w["mixed"],
+                 "contributions": {
+                     "inexact_newton (Jacobi 1e-10 -> 1e-3)": w["plain_fp64"] / w["fp64_jacobi_1e-3"],
+                     "multigrid (Jacobi -> multigrid, both 1e-3, FP64)": w["fp64_jacobi_1e-3"] / w["fp64_multigrid_1e-3"],
+                     "precision (FP64 -> mixed, both multigrid 1e-3)": w["fp64_multigrid_1e-3"] / w["mixed"]},
+                 "rel_err_vs_fp64": res["mixed"]["rel_err_vs_plain_fp64"]})
     print(json.dumps(line), flush=True)
 
 
@@ -664,30 +698,45 @@ def bench_rve_batch(a):
         plain_el += r.elements
     print(f"# plain FP64: {len(sample)} RVEs, {plain_el} elements in {plain_wall:.2f} s", file=sys.stderr,
           flush=True)
-    # batched mixed precision over every request
+    # batched over every request, three ways: FP64 with the plain inner
+    # tolerance (what batching alone gives), FP64 with the inexact inner
+    # tolerance (the solver), mixed precision with it (the precision)
     batches = size_batches(reqs, a.batch_elements)
-    t0 = time.perf_counter()
-    solve_s = 0.0
-    errs = []
-    newton_max = 0
-    for b in batches:
-        pk = batch.pack([probs[r.rid] for r in b])
-        s = batch.make_solid(pk, formulation=api.TOTAL_LAGRANGIAN, precision=api.MIXED)
-        t1 = time.perf_counter()
-        us, st = batch.static_solve_batch(s, pk, 1, mixed)
-        s.synchronize()
-        solve_s += time.perf_counter() - t1
-        s.close()
-        newton_max = max(newton_max, max(x.iterations for x in st[0]))
-        for r, u in zip(b, us):
-            if r.rid in plain_u:
-                ref = plain_u[r.rid]
-                errs.append(float(np.abs(u - ref).max() / max(np.abs(ref).max(), 1e-300)))
-        print(f"# batch of {len(b)} (L {b[0].L}-{b[-1].L}): {time.perf_counter() - t0:.2f} s so far",
-              file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
+
+    def run_batched(tag, precision, opts):
+        t0 = time.perf_counter()
+        solve_s, errs, newton_max = 0.0, [], 0
+        for b in batches:
+            pk = batch.pack([probs[r.rid] for r in b])
+            s = batch.make_solid(pk, formulation=api.TOTAL_LAGRANGIAN, precision=precision)
+            t1 = time.perf_counter()
+            us, st = batch.static_solve_batch(s, pk, 1, opts)
+            s.synchronize()
+            solve_s += time.perf_counter() - t1
+            s.close()
+            newton_max = max(newton_max, max(x.iterations for x in st[0]))
+            for r, u in zip(b, us):
+                if r.rid in plain_u:
+                    ref = plain_u[r.rid]
+                    errs.append(float(np.abs(u - ref).max() / max(np.abs(ref).max(), 1e-300)))
+        wall = time.perf_counter() - t0
+        log(f"{tag}: {len(batches)} batches in {wall:.2f} s (Newton alone {solve_s:.2f} s)")
+        return {"wall_s": wall, "solve_s": solve_s, "elements_per_s": tot_el / wall,
+                "solve_only_elements_per_s": tot_el / solve_s,
+                "max_rel_err_vs_fp64": max(errs) if errs else None, "compared_rves": len(errs),
+                "newton_max": newton_max}
+
+    legs = {"batched_fp64_1e-10": run_batched("batched FP64, PCG to 1e-10", api.FP64, fp64),
+            f"batched_fp64_{a.batch_lin_rtol:g}": run_batched(f"batched FP64, PCG to {a.batch_lin_rtol:g}", api.FP64,
+                                                              mixed),
+            f"batched_mixed_{a.batch_lin_rtol:g}": run_batched(f"batched mixed, PCG to {a.batch_lin_rtol:g}",
+                                                               api.MIXED, mixed)}
+    main_leg = legs[f"batched_mixed_{a.batch_lin_rtol:g}"]
+    wall = main_leg["wall_s"]
     value = tot_el / wall
     plain_rate = plain_el / plain_wall
+    r64 = legs["batched_fp64_1e-10"]["elements_per_s"]
+    r64i = legs[f"batched_fp64_{a.batch_lin_rtol:g}"]["elements_per_s"]
     line = base_line(value, "elements/s", len(batches), 1, wall * 1e3 / len(batches), "f32/f64",
                      {"workload": f"{len(reqs)} two-phase RVE cubes, edge log-uniform 4-24 (seed 4242), "
                                   "Bernoulli(0.3) phase 2 at 10x stiffness, affine u = H X, |H_ij| <= 0.02, "
@@ -697,16 +746,20 @@ def bench_rve_batch(a):
                       "batch_elements": a.batch_elements})
     line.update({
         "rves_per_s": len(reqs) / wall,
-        "solve_only_elements_per_s": tot_el / solve_s,
+        "solve_only_elements_per_s": main_leg["solve_only_elements_per_s"],
         "plain_fp64": {"rves": len(sample), "elements": plain_el, "wall_s": plain_wall,
                        "elements_per_s": plain_rate, "solve_only_elements_per_s": plain_el / plain_solve,
                        "note": "every k-th request, one context per RVE, FP64 K + Jacobi-PCG to 1e-10"},
-        "timed": "both paths: pack + context setup (tg_initialize) + Newton to convergence; mesh generation "
-                 "excluded from both; solve_only: Newton alone",
-        "speedup_solve_only": (tot_el / solve_s) / (plain_el / plain_solve),
+        "legs": legs,
+        "contributions": {"batching (one at a time -> batched, FP64 PCG to 1e-10)": r64 / plain_rate,
+                          f"inexact inner solve (1e-10 -> {a.batch_lin_rtol:g}, batched FP64)": r64i / r64,
+                          "precision (FP64 -> mixed, batched, same tolerance)": value / r64i},
+        "timed": "every path: pack + context setup (tg_initialize) + Newton to convergence; mesh generation "
+                 "excluded; solve_only: Newton alone",
+        "speedup_solve_only": main_leg["solve_only_elements_per_s"] / (plain_el / plain_solve),
         "speedup_vs_plain_fp64": value / plain_rate,
-        "max_rel_err_vs_fp64": max(errs) if errs else None, "compared_rves": len(errs),
-        "newton_max": newton_max,
+        "max_rel_err_vs_fp64": main_leg["max_rel_err_vs_fp64"], "compared_rves": main_leg["compared_rves"],
+        "newton_max": main_leg["newton_max"],
     })
     print(json.dumps(line), flush=True)
 

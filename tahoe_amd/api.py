@@ -425,33 +425,39 @@ def static_solve(solid: Solid, num_steps: int, opts=None, on_step=None):
     return np.array(out_u), iters
 
 
-def static_solve_adaptive(solid: Solid, num_steps: int, opts=None, on_step=None, min_dt=1.0 / 4096, log=None,
-                          last=None, log_corrections=False):
-    """The static time loop to the checkpoints t = s/num_steps with step
-    cutting inside each interval: a sub-step whose Newton iteration fails
-    (J <= 0, divergence, iteration limit) is reset and retried at half the
-    increment, and a converged sub-step doubles the next one, never past the
-    next checkpoint (FEManagerT's time-step cutting, FEManagerT.cpp:243-266,
-    with ResetStep).  last: stop after that many checkpoints (default all).
-    log_corrections: Newton runs one correction per call (Solid.newton_stepwise)
-    and log reports each.
-    Returns the converged displacement at every checkpoint, the Newton
-    corrections summed per checkpoint and the sub-step counts."""
-    out_u, iters, subs = [], [], []
-    t, dt = 0.0, 1.0 / num_steps
-    for k in range(1, (num_steps if last is None else min(last, num_steps)) + 1):
-        tk = k / num_steps
+class LoadPath:
+    """FEManagerT::Solve's static time loop to the checkpoints t = k/num_steps
+    with step cutting inside each interval (FEManagerT.cpp:243-266, with
+    ResetStep): a sub-step whose Newton iteration fails (J <= 0, divergence,
+    iteration limit) is reset and retried at half the increment, and a
+    converged sub-step doubles the next one, never past the next checkpoint.
+    advance() solves to the next checkpoint, so several paths (say one per
+    precision) can be driven side by side.  log_corrections: Newton runs one
+    correction per call (Solid.newton_stepwise) and log reports each."""
+
+    def __init__(self, solid: Solid, num_steps: int, opts=None, min_dt=1.0 / 4096, log=None, log_corrections=False):
+        self.solid, self.num_steps, self.opts = solid, num_steps, opts
+        self.min_dt, self.log, self.log_corrections = min_dt, log, log_corrections
+        self.t, self.dt, self.k = 0.0, 1.0 / num_steps, 0
+
+    def advance(self):
+        """Solve to the next checkpoint; returns (u, Newton corrections,
+        sub-steps, PCG iterations) of that checkpoint."""
+        solid, log = self.solid, self.log
+        self.k += 1
+        tk = self.k / self.num_steps
         n_newton, n_sub, lin = 0, 0, 0
-        while t < tk - 1e-12:
-            dt = min(dt, tk - t)
+        while self.t < tk - 1e-12:
+            t, dt = self.t, min(self.dt, tk - self.t)
+            self.dt = dt
             solid.apply_kbc(t + dt)
             try:
-                if log_corrections and log:
+                if self.log_corrections and log:
                     st = solid.newton_stepwise(
-                        opts, log=lambda i, rel, pcg: log(f"t={t:.5g}+{dt:.5g} correction {i}: "
-                                                          f"|R|/|R0| {rel:.3e}, PCG {pcg}"))
+                        self.opts, log=lambda i, rel, pcg: log(f"t={t:.5g}+{dt:.5g} correction {i}: "
+                                                               f"|R|/|R0| {rel:.3e}, PCG {pcg}"))
                 else:
-                    st = solid.newton(opts)
+                    st = solid.newton(self.opts)
                 ok = st.status == 0
             except TahoeGpuError as e:
                 if e.code != 7:
@@ -461,20 +467,32 @@ def static_solve_adaptive(solid: Solid, num_steps: int, opts=None, on_step=None,
                 log(f"t={t:.5g}+{dt:.5g}: {'converged' if ok else 'cut'}")
             if ok:
                 solid.close_step()
-                t += dt
+                self.t += dt
                 n_newton += st.iterations
                 lin += st.lin_iterations
                 n_sub += 1
-                dt *= 2.0
+                self.dt = dt * 2.0
             else:
                 solid.reset_step()
-                dt *= 0.5
-                if dt < min_dt:
-                    raise TahoeGpuError(1, f"load step below {min_dt} at t = {t}")
-        t = tk
+                self.dt = dt * 0.5
+                if self.dt < self.min_dt:
+                    raise TahoeGpuError(1, f"load step below {self.min_dt} at t = {self.t}")
+        self.t = tk
+        return solid.displacement(), n_newton, n_sub, lin
+
+
+def static_solve_adaptive(solid: Solid, num_steps: int, opts=None, on_step=None, min_dt=1.0 / 4096, log=None,
+                          last=None, log_corrections=False):
+    """LoadPath to every checkpoint (or the first `last`).  Returns the
+    converged displacement at every checkpoint, the Newton corrections
+    summed per checkpoint and the sub-step counts."""
+    path = LoadPath(solid, num_steps, opts, min_dt, log, log_corrections)
+    out_u, iters, subs = [], [], []
+    for k in range(1, (num_steps if last is None else min(last, num_steps)) + 1):
+        u, n_newton, n_sub, lin = path.advance()
         iters.append(n_newton)
         subs.append(n_sub)
-        out_u.append(solid.displacement())
+        out_u.append(u)
         if on_step:
             on_step(k, NewtonStats(0, n_newton, 0.0, 0.0, lin))
     return np.array(out_u), iters, subs

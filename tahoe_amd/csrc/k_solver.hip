@@ -81,23 +81,67 @@ __device__ __forceinline__ void ordered_total(const double* part, int nparts, do
 // operator
 // ---------------------------------------------------------------------------
 
-template <class VT>
-__global__ void k_expand(long nb, const int* __restrict__ src, const int* __restrict__ tpos,
-                         const VT* __restrict__ val, VT* __restrict__ out)
+// pass 1: one workgroup per half slice (32 block rows, lanes 32 h .. 32 h +
+// 31 of slice s).  Those rows own one contiguous range of off-diagonal MSR
+// values and one of diagonal values (node-major equations); STAGED copies
+// both into LDS with coalesced reads, so every MSR value is read from HBM
+// once.  Then 8 groups of 32 threads take the block slots in turn, and a
+// lane of a slot with column >= row (or padding) fetches its 9 components
+// through the MSR map and writes them (32 lanes of one component are
+// consecutive).  Without STAGED (rows too long for LDS) the components are
+// gathered from HBM.
+template <class VT, bool STAGED>
+__global__ void __launch_bounds__(256) k_expand(const int* __restrict__ sbase, const int* __restrict__ swidth,
+                                                const int4* __restrict__ srange, const int* __restrict__ src,
+                                                const int* __restrict__ tpos, const VT* __restrict__ val,
+                                                VT* __restrict__ out)
 {
-    // pass 1: one thread per block slot (lane l of slot blk) with column >=
-    // row (and padding): its 9 components through the MSR map, the three
-    // entries of one MSR row segment from the same cache line
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (long)gridDim.x * blockDim.x) {
-        if (tpos[i] >= 0) continue;
-        const long blk = i >> 6, l = i & 63;
+    extern __shared__ unsigned char lds_raw[];
+    VT* buf = reinterpret_cast<VT*>(lds_raw);
+    const int s = blockIdx.x >> 1, h = blockIdx.x & 1;
+    const int4 rg = srange[blockIdx.x];  // off-diagonal [x, y), diagonal [z, w)
+    const int ne = rg.y - rg.x, nd = rg.w - rg.z;
+    if (STAGED) {
+        for (int x = threadIdx.x; x < ne; x += 256) buf[x] = val[(long)rg.x + x];
+        for (int x = threadIdx.x; x < nd; x += 256) buf[ne + x] = val[(long)rg.z + x];
+        __syncthreads();
+    }
+    const long b0 = sbase[s];
+    const int width = swidth[s];
+    const int l = 32 * h + (threadIdx.x & 31);
+    for (int k = threadIdx.x >> 5; k < width; k += 8) {
+        const long blk = b0 + k;
+        if (tpos[blk * 64 + l] >= 0) continue;
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const long o = (blk * 9 + k) * 64 + l;
-            const int s = src[o];
-            out[o] = s >= 0 ? val[s] : VT(0);
+        for (int c = 0; c < 9; c++) {
+            const long o = (blk * 9 + c) * 64 + l;
+            const int sv = src[o];
+            VT v = VT(0);
+            if (sv >= 0) {
+                if (STAGED && sv >= rg.x && sv < rg.y)
+                    v = buf[sv - rg.x];
+                else if (STAGED && sv >= rg.z && sv < rg.w)
+                    v = buf[ne + (sv - rg.z)];
+                else
+                    v = val[sv];
+            }
+            out[o] = v;
         }
     }
+}
+
+// the expansion's pass 1 for values of type VT
+template <class VT>
+static void launch_expand_pass1(tg_ctx* c, const VT* val, VT* out)
+{
+    const size_t lds = (size_t)c->op.lds_values * sizeof(VT);
+    const unsigned grid = 2u * (unsigned)c->op.nslices;
+    if (lds > 0 && lds <= 65536)
+        hipLaunchKernelGGL((k_expand<VT, true>), dim3(grid), dim3(256), lds, c->stream, c->op.d_slice_base.p,
+                           c->op.d_slice_width.p, c->op.d_srange.p, c->op.d_src.p, c->op.d_tpos.p, val, out);
+    else
+        hipLaunchKernelGGL((k_expand<VT, false>), dim3(grid), dim3(256), 0, c->stream, c->op.d_slice_base.p,
+                           c->op.d_slice_width.p, c->op.d_srange.p, c->op.d_src.p, c->op.d_tpos.p, val, out);
 }
 
 // pass 2: lower blocks as the transposes of their upper partners, written
@@ -293,8 +337,7 @@ void launch_expand(tg_ctx* c)
 {
     TimedRegion tr(c, T_EXPAND);
     const long n = c->op.nblocks_padded;
-    hipLaunchKernelGGL(k_expand<double>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p,
-                       c->op.d_tpos.p, c->dval.p, c->op.d_val.p);
+    launch_expand_pass1<double>(c, c->dval.p, c->op.d_val.p);
     hipLaunchKernelGGL(k_expand_lower<double>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_tpos.p,
                        c->op.d_val.p);
     const long nn = 3L * c->nnd;
@@ -307,8 +350,7 @@ void launch_expand32(tg_ctx* c)
 {
     TimedRegion tr(c, T_EXPAND);
     const long n = c->op.nblocks_padded;
-    hipLaunchKernelGGL(k_expand<float>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_src.p,
-                       c->op.d_tpos.p, c->dval32.p, c->op.d_val32.p);
+    launch_expand_pass1<float>(c, c->dval32.p, c->op.d_val32.p);
     hipLaunchKernelGGL(k_expand_lower<float>, dim3(grid_for(n) * 4), dim3(VB), 0, c->stream, n, c->op.d_tpos.p,
                        c->op.d_val32.p);
     const long nn = 3L * c->nnd;

@@ -85,6 +85,11 @@ struct Bsell {
     DBuf<int> d_slice_base, d_slice_width;
     DBuf<int> d_col;                       // [nblocks_padded] column node, -1 = padding
     DBuf<int> d_src;                       // [nblocks_padded * 9] MSR val index, -1 = zero
+    // per half slice (32 block rows): the contiguous MSR ranges its rows own
+    // -- off-diagonal values [e0, e1) and diagonal values [r0, r1) -- staged
+    // in LDS by the expansion; lds_values = the largest (e1 - e0 + r1 - r0)
+    DBuf<int4> d_srange;                   // [2 nslices] (e0, e1, r0, r1)
+    long lds_values = 0;
     DBuf<int> d_tpos;                      // [nblocks_padded] for a lower block (column < row): value
                                            // index of its transpose partner, else -1
     DBuf<double> d_val;                    // [nblocks_padded * 9]
@@ -256,7 +261,6 @@ struct tg_ctx {
     tg::DBuf<double> dval;
     tg::DBuf<float> dval32;             // mixed precision: FP32 stiffness values
     tg::DBuf<int> dslot;                // [nel*300] element upper entry -> val index (-1 none)
-    tg::DBuf<int> dperm;                // [n_eq] equation -> node-space index
     tg::Bsell op;
 
     // PCG work vectors in node space

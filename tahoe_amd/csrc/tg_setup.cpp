@@ -177,10 +177,11 @@ int color_elements(int nnd, int nel, const int* conn, std::vector<int>& cptr, st
 }
 
 // Solver operator structure: block row = node, block columns = neighbour
-// nodes; 64 block rows per slice, each slice padded to its widest row.
-// d_src (built on the device, launch_build_src) maps each stored value to
-// the MSR value it mirrors (upper entry (min(r,c), max(r,c))), -1 for
-// inactive dofs and padding.
+// nodes (ascending); 64 block rows per slice, each slice padded to its
+// widest row.  d_src (built on the device, launch_build_src) maps each
+// stored value to the MSR value it mirrors (upper entry (min(r,c), max(r,c))),
+// -1 for inactive dofs and padding; d_tpos gives each lower block its upper
+// partner.
 void build_bsell(tg_ctx* c)
 {
     Bsell& B = c->op;
@@ -226,7 +227,31 @@ void build_bsell(tg_ctx* c)
     TG_HIP(hipMemcpyAsync(B.d_slice_width.p, B.slice_width.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice, c->stream));
     TG_HIP(hipMemcpyAsync(B.d_col.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice, c->stream));
     launch_build_src(c);
-    TG_HIP(hipStreamSynchronize(c->stream));  // col (host) outlives its copy
+    // MSR ranges of the rows of each half slice (node-major equations: the
+    // active rows of 32 consecutive nodes are consecutive equations, their
+    // off-diagonal values consecutive)
+    std::vector<int4> rng(2 * (size_t)B.nslices);
+    long most = 0;
+    for (int s = 0; s < 2 * B.nslices; s++) {  // half slices of 32 nodes
+        int r0 = -1, r1 = -1;
+        for (int a = 32 * s; a < std::min(nnd, 32 * s + 32); a++)
+            for (int i = 0; i < 3; i++) {
+                const int e = c->eqnos[3L * a + i];
+                if (e <= 0) continue;
+                if (r0 < 0) r0 = e - 1;
+                r1 = e;
+            }
+        if (r0 < 0) {
+            rng[s] = make_int4(0, 0, 0, 0);
+            continue;
+        }
+        rng[s] = make_int4(c->bindx[r0], c->bindx[r1], r0, r1);
+        most = std::max(most, (long)(c->bindx[r1] - c->bindx[r0]) + (r1 - r0));
+    }
+    B.lds_values = most;
+    B.d_srange.alloc(rng.size());
+    TG_HIP(hipMemcpyAsync(B.d_srange.p, rng.data(), sizeof(int4) * rng.size(), hipMemcpyHostToDevice, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));  // col and rng (host) outlive their copies
 }
 
 }  // namespace tg
