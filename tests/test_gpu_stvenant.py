@@ -120,7 +120,8 @@ def test_beam_with_force_card(precision):
 def test_q1p0_residual_and_stiffness_match_oracle():
     """updated_lagrangian_Q1P0 with large_strain_StVenant (the reference's
     Q1P0.3D.a): the material sees F_bar.  R to 1e-12; the forward-difference
-    tangent (eps = 1e-7 on both sides) to 1e-7."""
+    tangent (eps = 1e-7 on both sides) to 6e-9 of max|K| (the forward
+    difference's rounding noise, measured at ~3e-9 against the reference)."""
     from oracle import oracle as O
     from tahoe_amd import api
     m, kbc, s = cube_solid(n=4, formulation=api.Q1P0)
@@ -135,7 +136,7 @@ def test_q1p0_residual_and_stiffness_match_oracle():
     s.form_lhs()
     b = s.msr_bindx()
     vo = O.stiffness(O.Q1P0_STVENANT, m.X, m.conn, m.mat, mu, ka, u, eq, n, b)
-    assert np.abs(s.msr_values() - vo).max() <= 1e-7 * np.abs(vo).max()
+    assert np.abs(s.msr_values() - vo).max() <= 6e-9 * np.abs(vo).max()
     # the Simo Q1P0 element on the same state differs (the switch is live)
     Rs = O.residual(O.Q1P0, m.X, m.conn, m.mat, mu, ka, u, eq, n)
     assert np.abs(Rs - Ro).max() > 1e-6 * np.abs(Ro).max()
