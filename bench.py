@@ -103,6 +103,9 @@ def parse_args():
                          "FP64 residual either way)")
     ap.add_argument("--serve-count", type=int, default=512)
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered load as a fraction of capacity")
+    ap.add_argument("--serve-raw-h", action="store_true",
+                    help="rve-serve: the Q1P0 requests take the raw macro F = I + H (as the served-stream "
+                         "configuration states) instead of its isochoric part; many then need cut load steps")
     ap.add_argument("--stiff-chunk", type=int, default=None,
                     help="elements per range of the colour-ordered stiffness launches (default: autotuned)")
     ap.add_argument("--spmv-xcd", type=int, default=None, help="PCG SpMV variant 0..3 (default: autotuned)")
@@ -772,7 +775,8 @@ def bench_rve_serve(a):
     from tahoe_amd import api, rve, serve
     # capacity: a burst of requests all arriving at once (the server's
     # throughput at saturation)
-    cal = rve.stream_requests(min(256, a.serve_count), rate=1e9, seed=555)
+    iso = not a.serve_raw_h
+    cal = rve.stream_requests(min(256, a.serve_count), rate=1e9, seed=555, isochoric=iso)
     def progress(bid, size, form, sec):
         what = "failed, re-solving one by one" if sec < 0 else f"in {sec:.2f} s"
         print(f"# batch {bid}: {size} requests ({'Q1P0' if form == api.Q1P0 else 'Q1'}) {what}",
@@ -790,7 +794,7 @@ def bench_rve_serve(a):
     cap_els = sum(r.elements for r in cal) / st.wall
     rate = a.serve_load * cap
     print(f"# capacity {cap:.1f} solves/s -> offered {rate:.1f}/s", file=sys.stderr, flush=True)
-    reqs = rve.stream_requests(a.serve_count, rate=rate)
+    reqs = rve.stream_requests(a.serve_count, rate=rate, isochoric=iso)
     for r in reqs:
         r.problem()
     srv = serve.RVEServer(precision=api.MIXED, on_batch=progress, log=log)
@@ -819,7 +823,9 @@ def bench_rve_serve(a):
     line = base_line(cap_els, "elements/s", len(reqs), 0, st.wall * 1e3 / len(reqs), "f32/f64",
                      {"workload": f"{len(reqs)} RVE requests, Poisson arrivals at {rate:.1f}/s "
                                   f"({a.serve_load:g} of measured capacity), edge log-uniform 4-32, 50% Q1 nu=0.3 "
-                                  "/ 50% Q1P0 nu=0.499 (isochoric macro F), two-phase ratio 10, |H_ij| <= 0.02, "
+                                  "/ 50% Q1P0 nu=0.499 (" + ("isochoric part of the macro F" if iso else
+                                                            "raw macro F = I + H") +
+                                  "), two-phase ratio 10, |H_ij| <= 0.02, "
                                   "mixed precision (FP32 K, FP64 R and Newton; PCG to 1e-3, block Jacobi for Q1P0), "
                                   "Newton to 1e-10; meshes built before arrival",
                       "requests": len(reqs), "elements": els})

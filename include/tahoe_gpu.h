@@ -229,7 +229,13 @@ int tg_solve_noalloc(tg_ctx* ctx, const double* b_dev, double* x_dev, double rto
 int tg_newton(tg_ctx* ctx, const tg_newton_opts* opts, tg_newton_stats* stats);
 /* tg_newton for every segment at once: per-segment ||R||, ExitIteration
  * tests and PCG; converged or failed segments stop updating while the rest
- * continue.  stats[nseg] receives each segment's outcome. */
+ * continue.  stats[nseg] receives each segment's outcome: status 0
+ * converged, 1 diverged or out of iterations, 7 an element of that problem
+ * inverted in a residual pass (the others carry on; the call itself then
+ * returns 0).  A failed segment's displacement goes back to its last
+ * converged state (tg_close_step) so that the element passes over the rest
+ * see no inverted element; an inverted element in a stiffness pass still
+ * fails the whole call with 7. */
 int tg_newton_batch(tg_ctx* ctx, const tg_newton_opts* opts, tg_newton_stats* stats);
 /* One Newton correction without the convergence loop: R, K, PCG, u += du.
  * stats->error0 gets ||R|| before the update (used by benchmarks). */

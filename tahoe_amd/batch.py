@@ -127,6 +127,20 @@ def static_solve_batch(s: api.Solid, pk: Packed, num_steps: int, opts=None):
     return unpack(pk, s.displacement()), stats
 
 
+def solve_one_step_partial(s: api.Solid, pk: Packed, opts=None):
+    """One load step to the full load for every sub-problem (from the
+    predictor when one is given).  A sub-problem that fails -- an inverted
+    element in a residual pass (status 7), divergence or the iteration limit
+    -- stops while the others go on to convergence.  Returns per problem its
+    displacement (None when it failed) and the Newton statistics."""
+    if pk.u0 is not None:
+        s.set_displacement(pk.u0)
+    s.apply_kbc(1.0)
+    st = s.newton_batch(opts)
+    u = unpack(pk, s.displacement())
+    return [u[i] if st[i].status == 0 else None for i in range(len(st))], st
+
+
 def solve_adaptive(s: api.Solid, pk: Packed, opts=None, min_dt=1.0 / 1024, log=None):
     """The static load path to t = 1 with step cutting: a step that fails
     (J <= 0 or Newton failure in any sub-problem) is reset and retried at

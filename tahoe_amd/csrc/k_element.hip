@@ -28,7 +28,8 @@ __global__ void __launch_bounds__(256) k_residual_tl(const int* __restrict__ ele
                                                      const double2* __restrict__ props,
                                                      const double* __restrict__ X, const double* __restrict__ U,
                                                      const int* __restrict__ eqn, double* __restrict__ R,
-                                                     int* __restrict__ status)
+                                                     int* __restrict__ status, const int* __restrict__ elem_seg,
+                                                     int* __restrict__ segstat)
 {
     __shared__ double sxu[32][8][6];
     const int t = threadIdx.x, le = t >> 3, q = t & 7;
@@ -90,7 +91,11 @@ __global__ void __launch_bounds__(256) k_residual_tl(const int* __restrict__ ele
         for (int i = 0; i < 3; i++)
             if (eqn[3L * node + i] >= 0) R[3L * node + i] += v[i];
     }
-    if (bad) atomicOr(status, 7);
+    if (bad) {
+        // the context's flag, and the flag of the element's problem in a batch
+        atomicOr(status, 7);
+        atomicOr(&segstat[elem_seg[e]], 7);
+    }
 }
 
 void launch_q1p0(tg_ctx* c, const double* u, double* R_node, bool stiff);
@@ -131,11 +136,11 @@ static void element_residual(tg_ctx* c, const double* u, double* R_node)
         if (c->material_model == TG_MAT_STVENANT)
             hipLaunchKernelGGL(k_residual_tl<MAT_STVENANT>, dim3(blocks), dim3(256), 0, c->stream,
                                c->dcolor_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
-                               c->deqn.p, R_node, c->dstatus.p);
+                               c->deqn.p, R_node, c->dstatus.p, c->seg.d_elem_seg.p, c->seg.d_segstat.p);
         else
             hipLaunchKernelGGL(k_residual_tl<MAT_SIMO>, dim3(blocks), dim3(256), 0, c->stream,
                                c->dcolor_elems.p + first, count, c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u,
-                               c->deqn.p, R_node, c->dstatus.p);
+                               c->deqn.p, R_node, c->dstatus.p, c->seg.d_elem_seg.p, c->seg.d_segstat.p);
     }
     TG_HIP(hipGetLastError());
 }

@@ -85,7 +85,8 @@ __global__ void __launch_bounds__(256) k_q1p0(const int* __restrict__ elems, int
                                               const double* __restrict__ X, const double* __restrict__ U,
                                               const int* __restrict__ eqn, double* __restrict__ R,
                                               const int* __restrict__ slot, double* __restrict__ val,
-                                              int* __restrict__ status)
+                                              int* __restrict__ status, const int* __restrict__ elem_seg,
+                                              int* __restrict__ segstat)
 {
     __shared__ double sX[32][8][3], sx[32][8][3], su[32][8][3];
     const int t = threadIdx.x, le = t >> 3, q = t & 7;
@@ -163,7 +164,11 @@ __global__ void __launch_bounds__(256) k_q1p0(const int* __restrict__ elems, int
             __syncthreads();
         }
     }
-    if (valid && !ok) atomicOr(status, 7);
+    if (valid && !ok) {
+        // the context's flag, and the flag of the element's problem in a batch
+        atomicOr(status, 7);
+        atomicOr(&segstat[elem_seg[e]], 7);
+    }
 }
 
 void launch_q1p0(tg_ctx* c, const double* u, double* R_node, bool stiff)
@@ -175,7 +180,7 @@ void launch_q1p0(tg_ctx* c, const double* u, double* R_node, bool stiff)
 #define TG_Q1P0(S, M)                                                                                          \
     hipLaunchKernelGGL((k_q1p0<S, M>), dim3(blocks), dim3(256), 0, c->stream, c->dcolor_elems.p + first, count, \
                        c->dconn.p, c->dmat.p, c->dmatprops.p, c->dX.p, u, c->deqn.p, R_node, c->dslot.p, c->dval.p, \
-                       c->dstatus.p)
+                       c->dstatus.p, c->seg.d_elem_seg.p, c->seg.d_segstat.p)
         const bool stv = c->material_model == TG_MAT_STVENANT;
         if (stiff && stv)
             TG_Q1P0(true, MAT_STVENANT);

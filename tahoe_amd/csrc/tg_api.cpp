@@ -153,6 +153,7 @@ static void setup_segments(tg_ctx* c)
     S.d_part.alloc(2 * (size_t)S.nslices);
     S.d_out.alloc((size_t)S.nseg);
     S.d_ndone.alloc(1);
+    S.d_segstat.alloc((size_t)S.nseg);
     S.h_scal.assign((size_t)S_NSCAL * S.nseg, 0.0);
     std::vector<int> ones((size_t)S.nseg, 1);
     auto up = [&](int* dst, const std::vector<int>& v) {
@@ -163,6 +164,7 @@ static void setup_segments(tg_ctx* c)
     up(S.d_active.p, ones);
     up(S.d_elem_seg.p, S.elem_seg);
     TG_HIP(hipMemsetAsync(S.d_scal.p, 0, sizeof(double) * S_NSCAL * S.nseg, c->stream));
+    TG_HIP(hipMemsetAsync(S.d_segstat.p, 0, sizeof(int) * S.nseg, c->stream));
     // every copy and clear of the context goes through its own stream (a
     // null-stream operation is not ordered with a non-blocking stream), and
     // the host vectors copied from must outlive the copies
@@ -178,16 +180,37 @@ void drop_pcg_graph(tg_ctx* c)
     std::memset(c->pg.key, 0, sizeof c->pg.key);
 }
 
+static void clear_status(tg_ctx* c)
+{
+    TG_HIP(hipMemsetAsync(c->dstatus.p, 0, sizeof(int), c->stream));
+    if (c->seg.d_segstat.p) TG_HIP(hipMemsetAsync(c->seg.d_segstat.p, 0, sizeof(int) * c->seg.nseg, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+}
+
 static void check_status(tg_ctx* c)
 {
     TG_HIP(hipMemcpyAsync(c->hstatus, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     TG_HIP(hipStreamSynchronize(c->stream));
     if (c->hstatus[0] != 0) {
         const int code = c->hstatus[0];
-        TG_HIP(hipMemsetAsync(c->dstatus.p, 0, sizeof(int), c->stream));
-        TG_HIP(hipStreamSynchronize(c->stream));
+        clear_status(c);
         fail(code, "zero or negative jacobian");
     }
+}
+
+// After a residual pass of a batch: the segments whose elements inverted
+// (per-segment flags of the residual kernels) get status 7 in bad[g]; the
+// flags are cleared.  Other failures still raise.
+static void check_status_segments(tg_ctx* c, std::vector<int>& bad)
+{
+    TG_HIP(hipMemcpyAsync(c->hstatus, c->dstatus.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TG_HIP(hipStreamSynchronize(c->stream));
+    bad.assign((size_t)c->seg.nseg, 0);
+    if (c->hstatus[0] == 0) return;
+    if (c->hstatus[0] != TG_BAD_JACOBIAN) check_status(c);  // raises
+    TG_HIP(hipMemcpyAsync(bad.data(), c->seg.d_segstat.p, sizeof(int) * bad.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+    clear_status(c);
 }
 
 }  // namespace tg
@@ -939,14 +962,28 @@ int tg_newton_batch(tg_ctx* c, const tg_newton_opts* o, tg_newton_stats* stats)
         const int G = c->seg.nseg;
         std::vector<double> e0(G), e(G);
         std::vector<int> state(G, 0), iter(G, -1);  // 0 active, 1 converged, 2 failed
+        std::vector<int> code(G, 0), bad;           // per segment: 7 once an element inverted
         std::vector<long> lin(G, 0);
+        // a failed segment goes back to its last converged displacement, so
+        // the element passes the rest still run over its elements see no
+        // inverted element (its result is "failed" either way)
+        auto restore = [&](int g) {
+            const size_t b = 3 * (size_t)c->seg.node_begin[g], n = 3 * (size_t)(c->seg.node_begin[g + 1] - c->seg.node_begin[g]);
+            TG_HIP(hipMemcpyAsync(c->du.p + b, c->du_last.p + b, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        };
         launch_residual(c, c->du.p, c->dR.p);
-        check_status(c);
+        check_status_segments(c, bad);
         segment_dots(c, c->dR.p, c->dR.p, e0.data());
         for (int g = 0; g < G; g++) {
             e0[g] = std::sqrt(e0[g]);
             e[g] = e0[g];
-            if (e0[g] < o->abs_tolerance) state[g] = 1;
+            if (bad[g]) {
+                state[g] = 2;
+                code[g] = TG_BAD_JACOBIAN;
+                restore(g);
+            } else if (e0[g] < o->abs_tolerance) {
+                state[g] = 1;
+            }
         }
         std::vector<int> active(G);
         for (;;) {
@@ -965,26 +1002,31 @@ int tg_newton_batch(tg_ctx* c, const tg_newton_opts* o, tg_newton_stats* stats)
                 if (active[g]) lin[g] += (long)c->seg.h_scal[(size_t)S_ITER * G + g];
             launch_axpy(c, c->du.p, c->vx.p, 1.0, 3L * c->nnd);
             launch_residual(c, c->du.p, c->dR.p);
-            check_status(c);
+            check_status_segments(c, bad);
             segment_dots(c, c->dR.p, c->dR.p, e.data());
             for (int g = 0; g < G; g++) {
                 e[g] = std::sqrt(e[g]);
                 if (!active[g]) continue;
                 iter[g]++;
                 const double rel = e[g] / e0[g];
-                if (rel > o->divergence_tolerance)
+                if (bad[g]) {
+                    // an element of this problem inverted: it fails, the others go on
+                    state[g] = 2;
+                    code[g] = TG_BAD_JACOBIAN;
+                } else if (rel > o->divergence_tolerance)
                     state[g] = 2;
                 else if (rel < o->rel_tolerance || e[g] < o->abs_tolerance)
                     state[g] = 1;
                 else if (iter[g] >= o->max_iterations)
                     state[g] = 2;
+                if (state[g] == 2) restore(g);
             }
         }
         std::vector<int> ones(G, 1);
         TG_HIP(hipMemcpyAsync(c->seg.d_active.p, ones.data(), sizeof(int) * G, hipMemcpyHostToDevice, c->stream));
         TG_HIP(hipStreamSynchronize(c->stream));
         for (int g = 0; g < G; g++) {
-            stats[g].status = state[g] == 1 ? 0 : 1;
+            stats[g].status = state[g] == 1 ? 0 : (code[g] ? code[g] : 1);
             stats[g].iterations = iter[g] + 1;
             stats[g].error0 = e0[g];
             stats[g].error = e[g];

@@ -181,6 +181,7 @@ struct Segments {
     DBuf<double> d_part;                  // [2][nslices]
     DBuf<double> d_out;                   // [nseg]
     DBuf<int> d_ndone;
+    DBuf<int> d_segstat;                  // [nseg] 7 when an element of the segment inverted in a residual pass
     std::vector<double> h_scal;
 };
 

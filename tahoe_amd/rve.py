@@ -78,10 +78,11 @@ def batch_requests(n=4096, lo=4, hi=24, seed=4242, nu=0.3):
     return [Request(i, int(L[i]), api.TOTAL_LAGRANGIAN, nu, seed * 100003 + i) for i in range(n)]
 
 
-def stream_requests(n, rate, lo=4, hi=32, seed=777, q1p0_fraction=0.5):
+def stream_requests(n, rate, lo=4, hi=32, seed=777, q1p0_fraction=0.5, isochoric=True):
     """Poisson arrivals (exponential gaps at `rate` per second), edge
     log-uniform in lo-hi, a fraction q1p0_fraction of Q1P0 mixed-pressure
-    requests at nu = 0.499 (isochoric macro F) and the rest Q1 at nu = 0.3."""
+    requests at nu = 0.499 (the isochoric part of the macro F, or the raw
+    F = I + H with isochoric=False) and the rest Q1 at nu = 0.3."""
     L = log_uniform_edges(n, lo, hi, seed)
     ug = M.uniform01(seed, np.uint64(1 << 42) + np.arange(n, dtype=np.uint64))
     uk = M.uniform01(seed, np.uint64(1 << 43) + np.arange(n, dtype=np.uint64))
@@ -90,5 +91,5 @@ def stream_requests(n, rate, lo=4, hi=32, seed=777, q1p0_fraction=0.5):
     for i in range(n):
         q1p0 = uk[i] < q1p0_fraction
         out.append(Request(i, int(L[i]), api.Q1P0 if q1p0 else api.TOTAL_LAGRANGIAN, 0.499 if q1p0 else 0.3,
-                           seed * 100003 + i, float(t[i]), isochoric=bool(q1p0)))
+                           seed * 100003 + i, float(t[i]), isochoric=bool(q1p0) and isochoric))
     return out

@@ -168,38 +168,46 @@ class RVEServer:
     # -- worker ---------------------------------------------------------
     def _solve_group(self, reqs):
         """Solve reqs as one batch context in one load step (from the affine
-        predictor); raises TahoeGpuError when any sub-problem fails."""
+        predictor).  Returns per request its displacement (None when that
+        sub-problem failed; the others are unaffected) and its Newton
+        corrections; raises TahoeGpuError when the batch as a whole fails."""
         pk = batch.pack([r.problem() for r in reqs])
         s = batch.make_solid(pk, formulation=reqs[0].formulation, precision=self.precision, device=self.device,
                              precond=precond_for(reqs[0]))
         try:
-            us, st = batch.static_solve_batch(s, pk, 1, self.opts)
+            us, st = batch.solve_one_step_partial(s, pk, self.opts)
         finally:
             s.close()
-        return us, [x.iterations for x in st[-1]]
+        return us, [x.iterations for x in st]
 
     def _solve(self, bid, reqs, t0):
-        """A mini-batch in one load step; when it fails, each request goes
-        through solve_request on its own.  A request that cannot be solved is
-        reported with its status and u None; a worker never dies on one."""
+        """A mini-batch in one load step.  Its converged members finish with
+        the batch; a member that failed (an inverted element, divergence)
+        goes through solve_request on its own afterwards, and when the batch
+        as a whole fails every member does.  A request that cannot be solved
+        is reported with its status and u None; a worker never dies on one."""
         t_adm = time.perf_counter() - t0
         done = {}
         try:
             us, its = self._solve_group(reqs)
-            done = {r.rid: (u, n, 0, 1) for r, u, n in zip(reqs, us, its)}
-        except api.TahoeGpuError as e:
-            if self.on_batch:
-                self.on_batch(bid, len(reqs), reqs[0].formulation, -1.0)  # -1: batch failed, solving one by one
-            for r in reqs:
-                try:
-                    u, n, steps = solve_request(r, self.precision, self.opts, self.device, self.log)
-                    done[r.rid] = (u, n, 0, steps)
-                except api.TahoeGpuError as e:
-                    done[r.rid] = (None, 0, e.code, 0)
-        t_end = time.perf_counter() - t0
+            t_batch = time.perf_counter() - t0
+            for r, u, n in zip(reqs, us, its):
+                if u is not None:
+                    done[r.rid] = (u, n, 0, 1, t_batch)
+        except api.TahoeGpuError:
+            pass
+        retry = [r for r in reqs if r.rid not in done]
+        if retry and self.on_batch:
+            self.on_batch(bid, len(retry), reqs[0].formulation, -1.0)  # -1: members re-solved one by one
+        for r in retry:
+            try:
+                u, n, steps = solve_request(r, self.precision, self.opts, self.device, self.log)
+                done[r.rid] = (u, n, 0, steps, time.perf_counter() - t0)
+            except api.TahoeGpuError as e:
+                done[r.rid] = (None, 0, e.code, 0, time.perf_counter() - t0)
         out = []
         for r in reqs:
-            u, n, code, steps = done[r.rid]
+            u, n, code, steps, t_end = done[r.rid]
             out.append(Result(r.rid, u if self.keep_u else None, n, r.arrival, t_adm, t_end, bid, len(reqs), code,
                               steps))
         return out
@@ -224,9 +232,10 @@ class RVEServer:
             with self._lock:
                 for x in res:
                     self.stats.results[x.rid] = x
-                self.stats.batches.append((bid, len(reqs), reqs[0].formulation, res[0].admitted, res[0].finished))
+                self.stats.batches.append((bid, len(reqs), reqs[0].formulation, res[0].admitted,
+                                           max(x.finished for x in res)))
             if self.on_batch:
-                self.on_batch(bid, len(reqs), reqs[0].formulation, res[0].finished - res[0].admitted)
+                self.on_batch(bid, len(reqs), reqs[0].formulation, max(x.finished for x in res) - res[0].admitted)
 
     # -- driver ---------------------------------------------------------
     def run(self, requests, realtime=True):

@@ -1,9 +1,9 @@
 """Continuous-batching server: requests of both formulations arriving over
 time are grouped into mini-batches on several contexts at once; every
 result equals the same request solved alone (bitwise: a sub-problem's
-arithmetic does not depend on the batch it lands in; a batch that fails is
-re-solved request by request through the same single-request path) and
-matches the plain FP64 solve within 1e-9."""
+arithmetic does not depend on the batch it lands in; a member that fails is
+re-solved on its own through the same single-request path) and matches the
+plain FP64 solve within 1e-9."""
 import numpy as np
 import pytest
 
@@ -51,3 +51,29 @@ def test_size_classes_batch_separately():
     assert all(len(v) == 1 for v in by_batch.values())
     alone, _, _ = serve.solve_request(big, api.MIXED, srv.opts)
     assert np.array_equal(st.results[big.rid].u, alone)
+
+
+def test_failed_member_does_not_fail_the_batch():
+    """A batch whose member inverts its elements (a predictor u0 = -2 X, so
+    F = -I) reports status 7 for that member alone; the other members
+    converge as they do alone (bitwise), so a server re-solves only the
+    failed request."""
+    import dataclasses
+    from tahoe_amd import api, batch, rve
+    reqs = rve.batch_requests(5, lo=4, hi=6, seed=31)
+    probs = [r.problem() for r in reqs]
+    probs[2] = dataclasses.replace(probs[2], u0=-2.0 * probs[2].mesh.X)
+    opts = api.Solid.opts(lin_rtol=1e-3)
+    pk = batch.pack(probs)
+    s = batch.make_solid(pk, precision=api.MIXED)
+    us, st = batch.solve_one_step_partial(s, pk, opts)
+    s.close()
+    assert st[2].status == 7 and us[2] is None
+    for i in (0, 1, 3, 4):
+        assert st[i].status == 0
+        one = batch.pack([probs[i]])
+        s1 = batch.make_solid(one, precision=api.MIXED)
+        u1, st1 = batch.solve_one_step_partial(s1, one, opts)
+        s1.close()
+        assert st1[0].status == 0 and st1[0].iterations == st[i].iterations
+        assert np.array_equal(us[i], u1[0]), i
