@@ -88,7 +88,16 @@ def make_solid(pk: Packed, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64,
                precond=api.PC_JACOBI) -> api.Solid:
     """A context for the packed problems (the multigrid preconditioner needs
     a single problem)."""
-    s = api.Solid(device)
+    return configure(api.Solid(device), pk, formulation, precision, precond)
+
+
+def configure(s: api.Solid, pk: Packed, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64,
+              precond=api.PC_JACOBI) -> api.Solid:
+    """Set up context s, new or used before, for the packed problems.  A used
+    context keeps its device allocations where they are large enough, so a
+    server that keeps one context per kind of batch allocates nothing once
+    its largest batch has been seen; the displacements start from zero as in
+    a new context."""
     s.set_mesh(pk.X, pk.conn, pk.mat)
     for m, (E, nu) in enumerate(pk.materials):
         s.set_material(m, *api.mu_kappa(E, nu))

@@ -139,12 +139,17 @@ static void setup_segments(tg_ctx* c)
     seg_slice[S.nseg] = (int)S.nslices;
     for (long s = 0; s < S.nslices; s++) slice_seg[s] = node_seg[64 * s];
     S.elem_seg.resize((size_t)c->nel);
-    for (int e = 0; e < c->nel; e++) {
-        const int g = node_seg[c->conn[8L * e]];
-        for (int a = 1; a < 8; a++)
-            if (node_seg[c->conn[8L * e + a]] != g) fail(TG_BAD_INPUT, "element connects two segments");
-        S.elem_seg[e] = g;
-    }
+    int straddle = 0;
+    parallel_for(c->nel, [&](long b, long end) {
+        int bad = 0;
+        for (long e = b; e < end; e++) {
+            const int g = node_seg[c->conn[8 * e]];
+            for (int a = 1; a < 8; a++) bad |= node_seg[c->conn[8 * e + a]] != g;
+            S.elem_seg[e] = g;
+        }
+        if (bad) __atomic_store_n(&straddle, 1, __ATOMIC_RELAXED);
+    });
+    if (straddle) fail(TG_BAD_INPUT, "element connects two segments");
     S.d_slice_seg.alloc((size_t)S.nslices);
     S.d_seg_slice.alloc((size_t)S.nseg + 1);
     S.d_active.alloc((size_t)S.nseg);
@@ -303,16 +308,23 @@ int tg_set_mesh(tg_ctx* c, int nnd, int nel, const double* X, const int* conn, c
     if (!c) return TG_BAD_INPUT;
     return guarded(c, [&] {
         if (nnd <= 0 || nel <= 0 || !X || !conn) fail(TG_BAD_INPUT, "empty mesh");
-        for (long k = 0; k < 8L * nel; k++)
-            if (conn[k] < 0 || conn[k] >= nnd) fail(TG_OUT_OF_RANGE, "connectivity refers to a missing node");
-        for (int e = 0; e < nel; e++)
-            for (int a = 0; a < 8; a++)
-                for (int b = a + 1; b < 8; b++)
-                    if (conn[8L * e + a] == conn[8L * e + b])
-                        fail(TG_BAD_INPUT, "degenerate element (repeated node) is not supported");
-        if (mat)
-            for (int e = 0; e < nel; e++)
-                if (mat[e] < 0) fail(TG_BAD_INPUT, "negative material id");
+        // per element: 1 missing node, 2 repeated node, 4 negative material
+        int found = 0;
+        parallel_for(nel, [&](long b, long end) {
+            int f = 0;
+            for (long e = b; e < end; e++) {
+                const int* n = conn + 8 * e;
+                for (int a = 0; a < 8; a++) {
+                    f |= (n[a] < 0 || n[a] >= nnd) ? 1 : 0;
+                    for (int k = a + 1; k < 8; k++) f |= n[a] == n[k] ? 2 : 0;
+                }
+                if (mat) f |= mat[e] < 0 ? 4 : 0;
+            }
+            if (f) __atomic_fetch_or(&found, f, __ATOMIC_RELAXED);
+        });
+        if (found & 1) fail(TG_OUT_OF_RANGE, "connectivity refers to a missing node");
+        if (found & 2) fail(TG_BAD_INPUT, "degenerate element (repeated node) is not supported");
+        if (found & 4) fail(TG_BAD_INPUT, "negative material id");
         // every check above runs before the context changes: a rejected mesh
         // leaves the previous one (or none) in place
         c->nnd = nnd;

@@ -9,6 +9,7 @@
 #include <memory>
 #include <utility>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/tahoe_gpu.h"
@@ -52,23 +53,63 @@ void hip_check(hipError_t e, const char* what);
 
 #define TG_HIP(call) ::tg::hip_check((call), #call)
 
-// Device buffer owned by a context.
+// Host threads for set-up passes (at most 16).
+inline int worker_count()
+{
+    unsigned n = std::thread::hardware_concurrency();
+    if (n == 0) n = 4;
+    return (int)std::min(n, 16u);
+}
+
+// f(begin, end) over [0, n) split among worker_count() threads; small
+// ranges run on the calling thread.
+template <class F>
+void parallel_for(long n, F f)
+{
+    const int nt = worker_count();
+    if (n < 4096 || nt == 1) {
+        f(0L, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const long chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; t++) {
+        const long b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([=] { f(b, e); });
+    }
+    for (auto& x : th) x.join();
+}
+
+// Device buffer owned by a context.  alloc keeps an allocation that is
+// already large enough (n shrinks, the memory stays), so a context set up
+// again for a mesh no larger than its largest so far calls no hipMalloc or
+// hipFree; alloc(0) and release free it.  The contents are undefined after
+// alloc either way.
 template <class T>
 struct DBuf {
     T* p = nullptr;
-    size_t n = 0;
+    size_t n = 0;    // elements in use
+    size_t cap = 0;  // elements allocated
     void alloc(size_t count)
     {
+        if (count == 0) {
+            release();
+            return;
+        }
+        if (count <= cap) {
+            n = count;
+            return;
+        }
         release();
-        if (count == 0) return;
         TG_HIP(hipMalloc((void**)&p, count * sizeof(T)));
-        n = count;
+        n = cap = count;
     }
     void release()
     {
         if (p) (void)hipFree(p);
         p = nullptr;
-        n = 0;
+        n = cap = 0;
     }
     ~DBuf() { release(); }
 };
@@ -82,6 +123,7 @@ struct Bsell {
     int nslices = 0;
     long nblocks_padded = 0;               // sum over slices of width * 64
     std::vector<int> slice_base, slice_width;
+    ivec h_col;  // host staging of d_col, kept across set-ups
     DBuf<int> d_slice_base, d_slice_width;
     DBuf<int> d_col;                       // [nblocks_padded] column node, -1 = padding
     DBuf<int> d_src;                       // [nblocks_padded * 9] MSR val index, -1 = zero

@@ -10,31 +10,6 @@
 
 namespace tg {
 
-static int worker_count()
-{
-    unsigned n = std::thread::hardware_concurrency();
-    if (n == 0) n = 4;
-    return (int)std::min(n, 16u);
-}
-
-template <class F>
-static void parallel_for(long n, F f)
-{
-    int nt = worker_count();
-    if (n < 4096 || nt == 1) {
-        f(0L, n);
-        return;
-    }
-    std::vector<std::thread> th;
-    long chunk = (n + nt - 1) / nt;
-    for (int t = 0; t < nt; t++) {
-        long b = t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        th.emplace_back([=] { f(b, e); });
-    }
-    for (auto& x : th) x.join();
-}
-
 // NodeManagerT::SetEquationNumbers (tahoe/src/nodes/NodeManagerT.cpp:712-760)
 // -> FieldT::InitEquations/SetEquations (tahoe/src/nodes/FieldT.cpp:635-690):
 // dofs with a kinematic card get -1, the others 1.. in node-major, dof-minor
@@ -49,16 +24,22 @@ int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>
     return n;
 }
 
-// node -> incident elements (CSR)
-static void node_elements(int nnd, int nel, const int* conn, std::vector<int>& ptr, std::vector<int>& list)
+// node -> incident elements (CSR).  Counted and filled by all workers with
+// atomic increments, so the order of a node's elements varies from run to
+// run; node_graph sorts what it derives from them.
+static void node_elements(int nnd, int nel, const int* conn, std::vector<int>& ptr, ivec& list)
 {
     ptr.assign((size_t)nnd + 1, 0);
-    for (long k = 0; k < (long)nel * 8; k++) ptr[conn[k] + 1]++;
+    int* cnt = ptr.data() + 1;
+    parallel_for(8L * nel, [&](long b, long e) {
+        for (long k = b; k < e; k++) __atomic_fetch_add(&cnt[conn[k]], 1, __ATOMIC_RELAXED);
+    });
     for (int a = 0; a < nnd; a++) ptr[a + 1] += ptr[a];
     list.resize((size_t)ptr[nnd]);
     std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-    for (int e = 0; e < nel; e++)
-        for (int a = 0; a < 8; a++) list[(size_t)fill[conn[8 * e + a]]++] = e;
+    parallel_for(8L * nel, [&](long b, long e) {
+        for (long k = b; k < e; k++) list[(size_t)__atomic_fetch_add(&fill[conn[k]], 1, __ATOMIC_RELAXED)] = (int)(k / 8);
+    });
 }
 
 // Node adjacency as CSR (sorted unique neighbours including the node itself),
@@ -67,7 +48,8 @@ static void node_elements(int nnd, int nel, const int* conn, std::vector<int>& p
 // neighbours are sorted.
 void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist)
 {
-    std::vector<int> ptr, list;
+    std::vector<int> ptr;
+    ivec list;
     node_elements(nnd, nel, conn, ptr, list);
     const int MAXNB = 8 * 64;
     for (int a = 0; a < nnd; a++)
@@ -204,7 +186,8 @@ void build_bsell(tg_ctx* c)
     }
     B.nblocks_padded = base * 64;
     if (B.nblocks_padded * 9 > 0x7fffffffL) fail(TG_OUT_OF_RANGE, "block operator exceeds 32-bit indexing");
-    ivec col((size_t)B.nblocks_padded);  // each slice fills its own range, padding -1
+    ivec& col = B.h_col;  // kept by the context; each slice fills its own range, padding -1
+    col.resize((size_t)B.nblocks_padded);
     parallel_for(B.nslices, [&](long sb, long se) {
         for (long s = sb; s < se; s++) {
             std::fill(col.begin() + (long)B.slice_base[s] * 64, col.begin() + ((long)B.slice_base[s] + B.slice_width[s]) * 64,
@@ -231,24 +214,26 @@ void build_bsell(tg_ctx* c)
     // active rows of 32 consecutive nodes are consecutive equations, their
     // off-diagonal values consecutive)
     std::vector<int4> rng(2 * (size_t)B.nslices);
-    long most = 0;
-    for (int s = 0; s < 2 * B.nslices; s++) {  // half slices of 32 nodes
-        int r0 = -1, r1 = -1;
-        for (int a = 32 * s; a < std::min(nnd, 32 * s + 32); a++)
-            for (int i = 0; i < 3; i++) {
-                const int e = c->eqnos[3L * a + i];
-                if (e <= 0) continue;
-                if (r0 < 0) r0 = e - 1;
-                r1 = e;
+    std::vector<long> most(2 * (size_t)B.nslices, 0);
+    parallel_for(2L * B.nslices, [&](long sb, long se) {
+        for (long s = sb; s < se; s++) {  // half slices of 32 nodes
+            int r0 = -1, r1 = -1;
+            for (long a = 32 * s; a < std::min((long)nnd, 32 * s + 32); a++)
+                for (int i = 0; i < 3; i++) {
+                    const int e = c->eqnos[3 * a + i];
+                    if (e <= 0) continue;
+                    if (r0 < 0) r0 = e - 1;
+                    r1 = e;
+                }
+            if (r0 < 0) {
+                rng[s] = make_int4(0, 0, 0, 0);
+                continue;
             }
-        if (r0 < 0) {
-            rng[s] = make_int4(0, 0, 0, 0);
-            continue;
+            rng[s] = make_int4(c->bindx[r0], c->bindx[r1], r0, r1);
+            most[s] = (long)(c->bindx[r1] - c->bindx[r0]) + (r1 - r0);
         }
-        rng[s] = make_int4(c->bindx[r0], c->bindx[r1], r0, r1);
-        most = std::max(most, (long)(c->bindx[r1] - c->bindx[r0]) + (r1 - r0));
-    }
-    B.lds_values = most;
+    });
+    B.lds_values = most.empty() ? 0 : *std::max_element(most.begin(), most.end());
     B.d_srange.alloc(rng.size());
     TG_HIP(hipMemcpyAsync(B.d_srange.p, rng.data(), sizeof(int4) * rng.size(), hipMemcpyHostToDevice, c->stream));
     TG_HIP(hipStreamSynchronize(c->stream));  // col and rng (host) outlive their copies

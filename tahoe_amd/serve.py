@@ -113,8 +113,17 @@ def solve_request(req, precision, opts, device=0, log=None, precond=None):
 
 class RVEServer:
     def __init__(self, precision=api.MIXED, workers=6, max_batch=32, budget_bytes=64 << 30, lin_rtol=None,
-                 device=0, keep_u=True, on_batch=None, log=None):
+                 device=0, keep_u=True, on_batch=None, log=None, reuse_contexts=True):
         self.precision = precision
+        # each worker keeps one context per (formulation, size class,
+        # preconditioner) alive between batches and sets it up again for the
+        # next batch (batch.configure): no allocation once the kind's largest
+        # batch has been seen, and no free per batch.  These contexts hold
+        # device memory beyond the admission budget, which counts the
+        # requests in flight: at most workers x 6 contexts, each the size of
+        # the largest batch of its kind so far.
+        self.reuse_contexts = reuse_contexts
+        self._local = threading.local()
         self.workers = workers
         self.max_batch = max_batch
         self.budget = budget_bytes
@@ -172,13 +181,36 @@ class RVEServer:
         sub-problem failed; the others are unaffected) and its Newton
         corrections; raises TahoeGpuError when the batch as a whole fails."""
         pk = batch.pack([r.problem() for r in reqs])
-        s = batch.make_solid(pk, formulation=reqs[0].formulation, precision=self.precision, device=self.device,
-                             precond=precond_for(reqs[0]))
+        form, pc = reqs[0].formulation, precond_for(reqs[0])
+        if not self.reuse_contexts:
+            s = batch.make_solid(pk, formulation=form, precision=self.precision, device=self.device, precond=pc)
+            try:
+                us, st = batch.solve_one_step_partial(s, pk, self.opts)
+            finally:
+                s.close()
+            return us, [x.iterations for x in st]
+        pool = self._pool()
+        key = (form, size_class(reqs[0]), pc)
+        s = pool.pop(key, None) or api.Solid(self.device)
         try:
+            batch.configure(s, pk, formulation=form, precision=self.precision, precond=pc)
             us, st = batch.solve_one_step_partial(s, pk, self.opts)
-        finally:
-            s.close()
+        except BaseException:
+            s.close()   # a context whose batch raised is not used again
+            raise
+        pool[key] = s
         return us, [x.iterations for x in st]
+
+    def _pool(self) -> dict:
+        """This worker thread's live contexts."""
+        if not hasattr(self._local, "pool"):
+            self._local.pool = {}
+        return self._local.pool
+
+    def _close_pool(self):
+        for s in self._pool().values():
+            s.close()
+        self._local.pool = {}
 
     def _solve(self, bid, reqs, t0):
         """A mini-batch in one load step.  Its converged members finish with
@@ -213,6 +245,12 @@ class RVEServer:
         return out
 
     def _worker(self, t0):
+        try:
+            self._serve_batches(t0)
+        finally:
+            self._close_pool()
+
+    def _serve_batches(self, t0):
         while True:
             with self._lock:
                 job = None

@@ -77,3 +77,38 @@ def test_failed_member_does_not_fail_the_batch():
         s1.close()
         assert st1[0].status == 0 and st1[0].iterations == st[i].iterations
         assert np.array_equal(us[i], u1[0]), i
+
+
+@pytest.mark.parametrize("form", ["Q1", "Q1P0"])
+def test_context_set_up_again_equals_a_new_one(form):
+    """One context set up for a batch, then for a smaller batch of other
+    sizes, then for the first batch again (its device buffers kept, not
+    reallocated) gives bitwise the results of a new context for each."""
+    from tahoe_amd import api, batch, rve
+    f = api.Q1P0 if form == "Q1P0" else api.TOTAL_LAGRANGIAN
+    nu = 0.499 if form == "Q1P0" else 0.3
+    def problems(edges, seed):
+        return [rve.Request(i, L, f, nu, seed + i, isochoric=form == "Q1P0").problem() for i, L in enumerate(edges)]
+    big, small = batch.pack(problems([7, 5, 6, 8], 700)), batch.pack(problems([4, 3], 900))
+    pc = api.PC_BLOCK_JACOBI if form == "Q1P0" else api.PC_JACOBI
+    opts = api.Solid.opts(lin_rtol=1e-3)
+
+    def fresh(pk):
+        s = batch.make_solid(pk, formulation=f, precision=api.MIXED, precond=pc)
+        try:
+            return batch.solve_one_step_partial(s, pk, opts)
+        finally:
+            s.close()
+
+    s = api.Solid(0)
+    try:
+        for pk in (big, small, big):
+            batch.configure(s, pk, formulation=f, precision=api.MIXED, precond=pc)
+            us, st = batch.solve_one_step_partial(s, pk, opts)
+            ref_u, ref_st = fresh(pk)
+            assert [x.iterations for x in st] == [x.iterations for x in ref_st]
+            assert [x.lin_iterations for x in st] == [x.lin_iterations for x in ref_st]
+            for a, b in zip(us, ref_u):
+                assert a is not None and np.array_equal(a, b)
+    finally:
+        s.close()
