@@ -108,7 +108,7 @@ def parse_args():
                          "configuration states) instead of its isochoric part; many then need cut load steps")
     ap.add_argument("--stiff-chunk", type=int, default=None,
                     help="elements per range of the colour-ordered stiffness launches (default: autotuned)")
-    ap.add_argument("--spmv-xcd", type=int, default=None, help="PCG SpMV variant 0..3 (default: autotuned)")
+    ap.add_argument("--spmv-xcd", type=int, default=None, help="PCG SpMV variant 0..7 (default: autotuned)")
     ap.add_argument("--pcg-fused", type=int, choices=[0, 1], default=None,
                     help="1: PCG direction formed inside the SpMV, 0: its own pass (default: autotuned)")
     ap.add_argument("--assembly", type=int, default=None,
@@ -126,8 +126,22 @@ PC_NAME = {"jacobi": "Jacobi", "block": "block-Jacobi", "amg": "multigrid"}
 
 def spmv_kernel_name(mt, fused, variant, pc=0):
     """Template instance of the PCG SpMV that runs (pc: 0 Jacobi, 2 multigrid)."""
-    x, u = ("true" if variant & 1 else "false"), ("true" if variant & 2 else "false")
-    return f"k_fused_spmv<{mt}, {pc}, {x}, {u}>" if fused else f"k_seg_spmv<{mt}, {x}, {u}>"
+    x, u, y = (("true" if variant & b else "false") for b in (1, 2, 4))
+    return f"k_fused_spmv<{mt}, {pc}, {x}, {u}, {y}>" if fused else f"k_seg_spmv<{mt}, {x}, {u}, {y}>"
+
+
+def spmv_operator_bytes(n, nnd, vbytes, symmetric):
+    """Algorithmic HBM bytes of the operator part of one SpMV on the
+    structured n^3 cube: (3n+1)^3 node-neighbour blocks, nnd of them
+    diagonal.  Every block's 9 values and column once; with the symmetric
+    read, a lower block's values are its upper partner's, so only the upper
+    blocks (diagonal included) bring values and a lower block brings its
+    column and the partner's index."""
+    nblocks = (n + 1 + 2 * n) ** 3
+    if not symmetric:
+        return nblocks * (9 * vbytes + 4)
+    lower = (nblocks - nnd) // 2
+    return (nblocks - lower) * (9 * vbytes + 4) + lower * 8
 
 
 def traffic_from_profiles(kernel):
@@ -368,16 +382,23 @@ def bench_cube(a):
     pcg_it_ms = (ts["spmv_ms"] + ts["vector_ms"] + ts["precond_ms"]) / its_s
     nblocks = (n + 1 + 2 * n) ** 3  # (3n+1)^3 node-neighbour blocks of the structured cube
     vbytes = 8 if a.precision == "fp64" else 4
+    variant = s.tuning(api.TUNE_SPMV_XCD)
+    symmetric = bool(variant & 4)
     fused = s.tuning(api.TUNE_PCG_FUSED) == 1 and a.precond != "block"
     # operator values + columns once; node vectors once: x and y (FP64); the
     # fused iteration reads p_old and z's sources (r and 1/diag for Jacobi, z
     # for the multigrid) instead of x, and writes the new p besides y
     node_vectors = (5 if a.precond == "jacobi" else 4) if fused else 2
-    kname = spmv_kernel_name("double" if a.precision == "fp64" else "float", fused, s.tuning(api.TUNE_SPMV_XCD),
+    kname = spmv_kernel_name("double" if a.precision == "fp64" else "float", fused, variant,
                              2 if a.precond == "amg" else 0)
     profiled = n == 128 and a.precision == "fp64" and a.precond == "jacobi"
-    spmv_bytes = nblocks * (9 * vbytes + 4) + node_vectors * 3 * 8 * m.nnd
+    # algorithmic bytes of the product on the full (both triangles) block
+    # operator: every block's values and column, the node vectors once
+    spmv_bytes = spmv_operator_bytes(n, m.nnd, vbytes, False) + node_vectors * 3 * 8 * m.nnd
     spmv_gbs = spmv_bytes / (spmv_avg_ms * 1e-3) / 1e9
+    # the symmetric read needs from HBM only the upper half and the lower
+    # blocks' indices; the rest is the partner rows' values again, from cache
+    unique_bytes = spmv_operator_bytes(n, m.nnd, vbytes, symmetric) + node_vectors * 3 * 8 * m.nnd
     log(f"SpMV {kname}: {spmv_avg_ms:.4f} ms per launch over {ts['spmv_launches']} launches "
         f"-> {spmv_gbs:.0f} GB/s algorithmic")
     res_gflops = RES_FLOP_PER_EL * m.nel / (r_ms * 1e-3) / 1e9
@@ -416,6 +437,9 @@ def bench_cube(a):
                                         "formed inside, + per-slice p.q partials)" if fused else
                                         " (3x3-block sliced-ELL SpMV + per-slice p.q partials)"),
                      "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": spmv_avg_ms,
+                     "symmetric_read": symmetric,
+                     "unique_hbm_bytes_per_launch": unique_bytes,
+                     "unique_hbm_GBps": unique_bytes / (spmv_avg_ms * 1e-3) / 1e9,
                      "launches_timed": ts["spmv_launches"],
                      "why_this_kernel": "the PCG SpMV is the dominant kernel of a Newton solve (85% of a PCG "
                                         "iteration, ~37k iterations per correction at 128^3)"},

@@ -288,7 +288,12 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
 /* TG_TUNE_SPMV_XCD     PCG SpMV variant, bit 0: each XCD (own L2) takes one
  *                      contiguous eighth of the rows instead of the hardware
  *                      order; bit 1: branch-free inner loop unrolled by 3
- *                      (padding slots multiply zeros).  0 (default) .. 3. */
+ *                      (padding slots multiply zeros); bit 2: each lower
+ *                      block is read as the transpose of its upper partner
+ *                      (the same values: the operator is symmetric), so HBM
+ *                      streams about half the operator and the partner rows'
+ *                      recent reads serve the rest from cache.  Every
+ *                      variant gives bitwise the same product.  0 .. 7. */
 /* TG_TUNE_AMG_DEGREE   multigrid: Chebyshev degree of each pre- and
  *                      post-smoothing pass, 2 (default) .. 4. */
 /* TG_TUNE_AMG_WEIGHT   multigrid: weight of the coarse-grid correction in

@@ -85,9 +85,10 @@ def tune_stiffness(solid: api.Solid, shapes=None, chunks=None, reps: int = 2, ru
             "ms": {"assembly_shape": {f"{m}/{v}": t for (m, v), t in t_shape.items()}, "chunk": t_chunk}}
 
 
-def tune_spmv_xcd(solid: api.Solid, run_pcg, fused=(1, 0), reps: int = 2) -> dict:
-    """Time the PCG iteration over the four SpMV variants (hardware or
-    XCD-contiguous row order, plain or branch-free unrolled inner loop), with
+def tune_spmv_xcd(solid: api.Solid, run_pcg, fused=(1, 0), reps: int = 2, variants=range(8)) -> dict:
+    """Time the PCG iteration over the SpMV variants (hardware or
+    XCD-contiguous row order, plain or branch-free unrolled inner loop, lower
+    blocks read directly or as their upper partners' transposes), with
     the direction formed inside the SpMV (fused) and as its own pass; run_pcg()
     must run a short PCG (e.g. one Newton correction with a small iteration
     cap) from a fixed state.  The figure compared is device time per
@@ -99,7 +100,7 @@ def tune_spmv_xcd(solid: api.Solid, run_pcg, fused=(1, 0), reps: int = 2) -> dic
     try:
         for f in fused:
             solid.set_tuning(api.TUNE_PCG_FUSED, f)
-            for v in (0, 1, 2, 3):
+            for v in variants:
                 solid.set_tuning(api.TUNE_SPMV_XCD, v)
                 run_pcg()
                 solid.synchronize()

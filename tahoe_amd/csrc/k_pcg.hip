@@ -14,6 +14,8 @@
 // (DiagonalMatrixT.cpp:267-324) and InnerProduct (SolverT.cpp:854-860).
 #include <cstring>
 
+#include <type_traits>
+
 #include "tg_internal.h"
 
 namespace tg {
@@ -148,14 +150,43 @@ __device__ __forceinline__ int row_block()
     return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
+// The 9 values of block slot blk of lane l, row-major (v[3 r + q]).  With
+// SYM a lower block (column node below the row node) is read as the
+// transpose of its upper partner, whose first value index tpos holds: the
+// operator is symmetric by construction (a lower block is a copy of its
+// partner), so the values are the same, while the partner's row, just
+// streamed by a nearby workgroup, serves them from L2 or the last-level
+// cache instead of HBM.  Only the lanes of lower blocks read tpos.
+template <class MT, bool SYM>
+__device__ __forceinline__ void block_values(const MT* __restrict__ bval, const int* __restrict__ tpos, long blk,
+                                             int l, bool lower, double v[9])
+{
+    long base = blk * 9 * 64 + l;
+    const bool tr = SYM && lower;
+    if (tr) base = tpos[blk * 64 + l];
+    double u[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) u[m] = (double)bval[base + m * 64];
+    v[0] = u[0];
+    v[4] = u[4];
+    v[8] = u[8];
+    v[1] = tr ? u[3] : u[1];
+    v[3] = tr ? u[1] : u[3];
+    v[2] = tr ? u[6] : u[2];
+    v[6] = tr ? u[2] : u[6];
+    v[5] = tr ? u[7] : u[5];
+    v[7] = tr ? u[5] : u[7];
+}
+
 // q = A p on the block operator, with per-slice partials of p.q.  MT is the
 // storage type of the operator values (double, or float in mixed
 // precision); products and sums are FP64.
-template <class MT, bool XCD, bool UNR>
+template <class MT, bool XCD, bool UNR, bool SYM>
 __global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict__ slice_seg,
                                                  const int* __restrict__ sbase, const int* __restrict__ swidth,
                                                  const int* __restrict__ bcol, const MT* __restrict__ bval,
-                                                 const double* __restrict__ x, double* __restrict__ y,
+                                                 const int* __restrict__ tpos, const double* __restrict__ x,
+                                                 double* __restrict__ y,
                                                  double* __restrict__ part, long nslices, int nseg,
                                                  const double* __restrict__ sc)
 {
@@ -180,10 +211,11 @@ __global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict_
                 const int cr = bcol[blk * 64 + l];
                 const int c = cr < 0 ? a : cr;
                 const double x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
-                const MT* v = bval + blk * 9 * 64 + l;
-                y0 += (double)v[0 * 64] * x0 + (double)v[1 * 64] * x1 + (double)v[2 * 64] * x2;
-                y1 += (double)v[3 * 64] * x0 + (double)v[4 * 64] * x1 + (double)v[5 * 64] * x2;
-                y2 += (double)v[6 * 64] * x0 + (double)v[7 * 64] * x1 + (double)v[8 * 64] * x2;
+                double v[9];
+                block_values<MT, SYM>(bval, tpos, blk, l, c < a, v);
+                y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+                y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+                y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
             }
         } else {
             for (int k = 0; k < w; k++) {
@@ -191,10 +223,11 @@ __global__ void __launch_bounds__(PB) k_seg_spmv(int nnd, const int* __restrict_
                 const int c = bcol[blk * 64 + l];
                 if (c < 0) continue;
                 const double x0 = x[3L * c], x1 = x[3L * c + 1], x2 = x[3L * c + 2];
-                const MT* v = bval + blk * 9 * 64 + l;
-                y0 += (double)v[0 * 64] * x0 + (double)v[1 * 64] * x1 + (double)v[2 * 64] * x2;
-                y1 += (double)v[3 * 64] * x0 + (double)v[4 * 64] * x1 + (double)v[5 * 64] * x2;
-                y2 += (double)v[6 * 64] * x0 + (double)v[7 * 64] * x1 + (double)v[8 * 64] * x2;
+                double v[9];
+                block_values<MT, SYM>(bval, tpos, blk, l, c < a, v);
+                y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+                y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+                y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
             }
         }
         y[3L * a] = y0;
@@ -323,10 +356,11 @@ __device__ __forceinline__ double zval(const double* __restrict__ m, const doubl
     return PC == 2 ? m[k] : m[k] * r[k];
 }
 
-template <class MT, int PC, bool XCD, bool UNR>
+template <class MT, int PC, bool XCD, bool UNR, bool SYM>
 __global__ void __launch_bounds__(PB) k_fused_spmv(int nnd, const int* __restrict__ slice_seg,
                                                    const int* __restrict__ sbase, const int* __restrict__ swidth,
                                                    const int* __restrict__ bcol, const MT* __restrict__ bval,
+                                                   const int* __restrict__ tpos,
                                                    const double* __restrict__ m, const double* __restrict__ r,
                                                    const double* __restrict__ pold, double* __restrict__ pnew,
                                                    double* __restrict__ y, double* __restrict__ part, long nslices,
@@ -354,10 +388,11 @@ __global__ void __launch_bounds__(PB) k_fused_spmv(int nnd, const int* __restric
             const double x0 = zval<PC>(m, r, 3L * c) + beta * pold[3L * c];
             const double x1 = zval<PC>(m, r, 3L * c + 1) + beta * pold[3L * c + 1];
             const double x2 = zval<PC>(m, r, 3L * c + 2) + beta * pold[3L * c + 2];
-            const MT* v = bval + blk * 9 * 64 + l;
-            y0 += (double)v[0 * 64] * x0 + (double)v[1 * 64] * x1 + (double)v[2 * 64] * x2;
-            y1 += (double)v[3 * 64] * x0 + (double)v[4 * 64] * x1 + (double)v[5 * 64] * x2;
-            y2 += (double)v[6 * 64] * x0 + (double)v[7 * 64] * x1 + (double)v[8 * 64] * x2;
+            double v[9];
+            block_values<MT, SYM>(bval, tpos, blk, l, c < a, v);
+            y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+            y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+            y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
         }
         double p3[3];
 #pragma unroll
@@ -401,6 +436,24 @@ __global__ void __launch_bounds__(FB) k_seg_fin_dot(const double* __restrict__ p
 // ---------------------------------------------------------------------------
 
 static unsigned node_blocks(const tg_ctx* c) { return (unsigned)((c->nnd + PB - 1) / PB); }
+
+// f(std::integral_constant<int, v>) for the SpMV variant v = 0..7 (bit 0
+// XCD row ranges, bit 1 unrolled loop, bit 2 symmetric read), so every
+// variant is its own kernel instance.
+template <class F>
+static void spmv_variant(int v, F f)
+{
+    switch (v & 7) {
+    case 0: f(std::integral_constant<int, 0>{}); break;
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    default: f(std::integral_constant<int, 7>{}); break;
+    }
+}
 
 void segment_dots(tg_ctx* c, const double* a, const double* b, double* host_out)
 {
@@ -472,23 +525,20 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
         if (!W.p2) fail(TG_GENERAL_FAIL, "fused PCG iteration without its second direction buffer");
         pbuf[1] = W.p2;
     }
+    const int variant = c->spmv_xcd;
     auto issue_fused = [&]() {
         for (int k = 0; k < CHUNK; k++) {
             double* pold = pbuf[k & 1];
             double* pnew = pbuf[(k + 1) & 1];
             {
                 TimedRegion tr(c, T_SPMV);
-#define TG_FSPMV(X, U)                                                                                        \
-    hipLaunchKernelGGL((k_fused_spmv<MT, PC, X, U>), dim3(X ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream,  \
-                       c->nnd, S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, \
-                       dinv, r, pold, pnew, q, W.part, ns, nseg, W.scal)
-                switch (c->spmv_xcd) {
-                case 0: TG_FSPMV(false, false); break;
-                case 1: TG_FSPMV(true, false); break;
-                case 2: TG_FSPMV(false, true); break;
-                default: TG_FSPMV(true, true); break;
-                }
-#undef TG_FSPMV
+spmv_variant(variant, [&](auto V) {
+                    constexpr int v = decltype(V)::value;
+                    hipLaunchKernelGGL((k_fused_spmv<MT, PC, (v & 1) != 0, (v & 2) != 0, (v & 4) != 0>),
+                                       dim3((v & 1) ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream, c->nnd,
+                                       S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val,
+                                       c->op.d_tpos.p, dinv, r, pold, pnew, q, W.part, ns, nseg, W.scal);
+                });
             }
             {
                 TimedRegion tr(c, T_VECTOR);
@@ -517,17 +567,13 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
         for (int k = 0; k < CHUNK; k++) {
             {
                 TimedRegion tr(c, T_SPMV);
-#define TG_SPMV(X, U)                                                                                      \
-    hipLaunchKernelGGL((k_seg_spmv<MT, X, U>), dim3(X ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream, c->nnd, \
-                       S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, p, q,  \
-                       W.part, ns, nseg, W.scal)
-                switch (c->spmv_xcd) {
-                case 0: TG_SPMV(false, false); break;
-                case 1: TG_SPMV(true, false); break;
-                case 2: TG_SPMV(false, true); break;
-                default: TG_SPMV(true, true); break;
-                }
-#undef TG_SPMV
+spmv_variant(variant, [&](auto V) {
+                    constexpr int v = decltype(V)::value;
+                    hipLaunchKernelGGL((k_seg_spmv<MT, (v & 1) != 0, (v & 2) != 0, (v & 4) != 0>),
+                                       dim3((v & 1) ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream, c->nnd,
+                                       S.d_slice_seg.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val,
+                                       c->op.d_tpos.p, p, q, W.part, ns, nseg, W.scal);
+                });
             }
             {
                 TimedRegion tr(c, T_VECTOR);
@@ -559,11 +605,11 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
         // every address and size the chunk's launches take (tg_initialize
         // also drops the graph, as buffers may come back at old addresses)
         const void* key[24] = {val, dinv, x, r, p, q, W.part, W.scal, W.ndone,
-                               (const void*)(long)(PC + 4 * c->spmv_xcd + 16 * (int)fused + 32L * c->amg.gen),
+                               (const void*)(long)(PC + 4L * (variant + 8L * ((int)fused + 2L * c->amg.gen))),
                                (const void*)(long)sizeof(MT), (const void*)(long)c->nnd, (const void*)(long)nseg,
                                (const void*)ns, S.d_seg_slice.p, S.d_slice_seg.p, c->op.d_slice_base.p,
                                c->op.d_slice_width.p, c->op.d_col.p, pbuf[1], c->vbinv.p,
-                               c->amg.lv.empty() ? nullptr : (const void*)c->amg.lv[0]};
+                               c->amg.lv.empty() ? nullptr : (const void*)c->amg.lv[0], c->op.d_tpos.p};
         if (!c->pg.exec || std::memcmp(key, c->pg.key, sizeof key) != 0) {
             drop_pcg_graph(c);
             TG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));

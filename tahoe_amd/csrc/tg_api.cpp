@@ -1097,7 +1097,7 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "PCG graph must be 0 or 1");
             c->pcg_graph = (int)value;
         } else if (knob == TG_TUNE_SPMV_XCD) {
-            if (value < 0 || value > 3) fail(TG_OUT_OF_RANGE, "SpMV variant must be 0..3");
+            if (value < 0 || value > 7) fail(TG_OUT_OF_RANGE, "SpMV variant must be 0..7");
             c->spmv_xcd = (int)value;
         } else if (knob == TG_TUNE_PCG_FUSED) {
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "PCG fused must be 0 or 1");

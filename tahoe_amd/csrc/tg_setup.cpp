@@ -239,4 +239,5 @@ void build_bsell(tg_ctx* c)
     TG_HIP(hipStreamSynchronize(c->stream));  // col and rng (host) outlive their copies
 }
 
+
 }  // namespace tg

@@ -14,22 +14,40 @@ import bench  # noqa: E402
 
 
 @pytest.mark.parametrize("fused", [False, True])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", range(8))
 def test_profile_covers_every_spmv_variant(fused, variant):
     name = bench.spmv_kernel_name("double", fused, variant)
     t = bench.traffic_from_profiles(name)
     assert t is not None, name
     # at least the algorithmic bytes of one launch (the per-node vectors of
     # the fused form included), and not absurdly more
-    alg = 4440104572 + (3 * 3 * 8 * 2146689 if fused else 0)
+    nnd = 129 ** 3
+    alg = bench.spmv_operator_bytes(128, nnd, 8, False) + (5 if fused else 2) * 24 * nnd
+    assert alg == 4440104572 + (3 * 3 * 8 * nnd if fused else 0)
     assert alg * 0.95 <= t <= alg * 2.0, (name, t)
 
 
 def test_kernel_names_and_threads():
-    assert bench.spmv_kernel_name("float", False, 3) == "k_seg_spmv<float, true, true>"
-    assert bench.spmv_kernel_name("double", True, 1, pc=2) == "k_fused_spmv<double, 2, true, false>"
-    assert bench.traffic_from_profiles("k_seg_spmv<float, false, false>") is None
+    assert bench.spmv_kernel_name("float", False, 3) == "k_seg_spmv<float, true, true, false>"
+    assert bench.spmv_kernel_name("double", True, 1, pc=2) == "k_fused_spmv<double, 2, true, false, false>"
+    assert bench.spmv_kernel_name("double", False, 5) == "k_seg_spmv<double, true, false, true>"
+    assert bench.traffic_from_profiles("k_seg_spmv<float, false, false, false>") is None
     assert 1 <= bench.cpu_threads() <= (os.cpu_count() or 1)
+
+
+def test_symmetric_operator_bytes():
+    """On the n^3 cube: (3n+1)^3 blocks, (n+1)^3 of them diagonal; the
+    symmetric read brings the values of the upper half and the diagonal, and
+    8 index bytes per lower block."""
+    n = 4
+    nnd = (n + 1) ** 3
+    nb = (3 * n + 1) ** 3
+    full = bench.spmv_operator_bytes(n, nnd, 8, False)
+    sym = bench.spmv_operator_bytes(n, nnd, 8, True)
+    assert full == nb * 76
+    lower = (nb - nnd) // 2
+    assert (nb - nnd) % 2 == 0 and sym == (nb - lower) * 76 + lower * 8
+    assert 0.5 < sym / full < 0.6
 
 
 def test_cpu_baselines_small_sample():

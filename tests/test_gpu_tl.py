@@ -239,14 +239,16 @@ def test_block_jacobi_newton(name):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_spmv_xcd_mapping_bitwise(precision):
-    """The SpMV variants change only which workgroup computes which rows and
-    how the inner loop is issued (padding slots add exact zeros), with the
-    direction formed inside the SpMV or in its own pass: the Newton
-    correction is bitwise equal for all eight."""
+    """The SpMV variants change only which workgroup computes which rows,
+    how the inner loop is issued (padding slots add exact zeros) and where a
+    lower block's values are read from (its own slot, or its upper partner's
+    transposed: the same values), with the direction formed inside the SpMV
+    or in its own pass: the Newton correction is bitwise equal for all
+    sixteen."""
     from tahoe_amd import api
     g = load_golden("rve_n4")
     out = []
-    for fused, xcd in [(f, v) for f in (1, 0) for v in (0, 1, 2, 3)]:
+    for fused, xcd in [(f, v) for f in (1, 0) for v in range(8)]:
         s = make_solid(g)
         if precision:
             s.set_precision(api.MIXED)
@@ -622,3 +624,4 @@ def test_gather_assembly_newton_and_batch():
         vals.append(b.msr_values())
         b.close()
     assert np.array_equal(vals[0], vals[1])
+
