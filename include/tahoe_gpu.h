@@ -293,18 +293,21 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      (the same values: the operator is symmetric), so HBM
  *                      streams about half the operator and the partner rows'
  *                      recent reads serve the rest from cache.  Every
- *                      variant gives bitwise the same product.  0 .. 7. */
+ *                      variant gives bitwise the same product.  0 .. 7, or
+ *                      -1 (default): 7 for a single problem of 2^17 nodes
+ *                      or more, else 0. */
 /* TG_TUNE_AMG_DEGREE   multigrid: Chebyshev degree of each pre- and
  *                      post-smoothing pass, 2 (default) .. 4. */
 /* TG_TUNE_AMG_WEIGHT   multigrid: weight of the coarse-grid correction in
  *                      percent, 100 .. 190 (default 150: unsmoothed
  *                      aggregation under-corrects; any weight in (0, 2)
  *                      keeps the V-cycle symmetric positive definite). */
-/* TG_TUNE_PCG_FUSED    1 (default): the direction update is formed inside
- *                      the SpMV (four kernels per PCG iteration instead of
- *                      five, no separate direction pass) for the Jacobi and
+/* TG_TUNE_PCG_FUSED    1: the direction update is formed inside the SpMV
+ *                      (four kernels per PCG iteration instead of five, no
+ *                      separate direction pass) for the Jacobi and
  *                      multigrid preconditioners, in any SpMV variant;
- *                      0: the five-kernel iteration. */
+ *                      0: the five-kernel iteration; -1 (default): 0 for a
+ *                      single problem of 2^17 nodes or more, else 1. */
 /* TG_TUNE_STIFF_GATHER 0 (default): the total-Lagrangian stiffness is
  *                      added into the MSR element by element (colour-
  *                      ordered read-modify-write through the slot map);

@@ -441,7 +441,7 @@ static unsigned node_blocks(const tg_ctx* c) { return (unsigned)((c->nnd + PB - 
 // XCD row ranges, bit 1 unrolled loop, bit 2 symmetric read), so every
 // variant is its own kernel instance.
 template <class F>
-static void spmv_variant(int v, F f)
+static void spmv_dispatch(int v, F f)
 {
     switch (v & 7) {
     case 0: f(std::integral_constant<int, 0>{}); break;
@@ -485,7 +485,7 @@ void pcg_run(tg_ctx* c, const MT* val, const double* dinv, const double* b, doub
     } else {
         // the context's own reductions, scalars and done count; the second
         // direction buffer of the fused iteration is allocated on first use
-        if (c->pcg_fused && c->precond != TG_PC_BLOCK_JACOBI && c->vp2.n != 3 * (size_t)c->nnd)
+        if (pcg_fused(c) && c->precond != TG_PC_BLOCK_JACOBI && c->vp2.n != 3 * (size_t)c->nnd)
             c->vp2.alloc(3 * (size_t)c->nnd);
         W = PcgWork{c->seg.d_part.p, c->seg.d_scal.p, c->seg.d_ndone.p, c->vp2.p, true};
     }
@@ -519,20 +519,20 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
                            d_active, nseg, W.scal, rtol, maxit, W.ndone);
     }
     const int CHUNK = PC == 2 ? 8 : 16;  // even: the fused iteration alternates its two direction buffers
-    const bool fused = c->pcg_fused && PC != 1;
+    const bool fused = pcg_fused(c) && PC != 1;
     double* pbuf[2] = {p, nullptr};
     if (fused) {
         if (!W.p2) fail(TG_GENERAL_FAIL, "fused PCG iteration without its second direction buffer");
         pbuf[1] = W.p2;
     }
-    const int variant = c->spmv_xcd;
+    const int variant = spmv_variant(c);
     auto issue_fused = [&]() {
         for (int k = 0; k < CHUNK; k++) {
             double* pold = pbuf[k & 1];
             double* pnew = pbuf[(k + 1) & 1];
             {
                 TimedRegion tr(c, T_SPMV);
-spmv_variant(variant, [&](auto V) {
+spmv_dispatch(variant, [&](auto V) {
                     constexpr int v = decltype(V)::value;
                     hipLaunchKernelGGL((k_fused_spmv<MT, PC, (v & 1) != 0, (v & 2) != 0, (v & 4) != 0>),
                                        dim3((v & 1) ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream, c->nnd,
@@ -567,7 +567,7 @@ spmv_variant(variant, [&](auto V) {
         for (int k = 0; k < CHUNK; k++) {
             {
                 TimedRegion tr(c, T_SPMV);
-spmv_variant(variant, [&](auto V) {
+spmv_dispatch(variant, [&](auto V) {
                     constexpr int v = decltype(V)::value;
                     hipLaunchKernelGGL((k_seg_spmv<MT, (v & 1) != 0, (v & 2) != 0, (v & 4) != 0>),
                                        dim3((v & 1) ? (nb + 7) / 8 * 8 : nb), dim3(PB), 0, c->stream, c->nnd,

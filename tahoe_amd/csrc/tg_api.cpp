@@ -1097,10 +1097,10 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "PCG graph must be 0 or 1");
             c->pcg_graph = (int)value;
         } else if (knob == TG_TUNE_SPMV_XCD) {
-            if (value < 0 || value > 7) fail(TG_OUT_OF_RANGE, "SpMV variant must be 0..7");
+            if (value < -1 || value > 7) fail(TG_OUT_OF_RANGE, "SpMV variant must be 0..7, or -1 for the default");
             c->spmv_xcd = (int)value;
         } else if (knob == TG_TUNE_PCG_FUSED) {
-            if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "PCG fused must be 0 or 1");
+            if (value < -1 || value > 1) fail(TG_OUT_OF_RANGE, "PCG fused must be 0 or 1, or -1 for the default");
             c->pcg_fused = (int)value;
         } else if (knob == TG_TUNE_STIFF_GATHER) {
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "stiffness gather must be 0 or 1");
@@ -1131,9 +1131,9 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
         else if (knob == TG_TUNE_PCG_GRAPH)
             *value = c->pcg_graph;
         else if (knob == TG_TUNE_SPMV_XCD)
-            *value = c->spmv_xcd;
+            *value = spmv_variant(c);
         else if (knob == TG_TUNE_PCG_FUSED)
-            *value = c->pcg_fused;
+            *value = pcg_fused(c) ? 1 : 0;
         else if (knob == TG_TUNE_STIFF_GATHER)
             *value = c->stiff_gather;
         else if (knob == TG_TUNE_AMG_DEGREE)

@@ -269,8 +269,8 @@ struct tg_ctx {
     int linear_solver = TG_LIN_PCG;     // inner linear solver of the Newton loop
     int gmres_restart = 30;
     int pcg_graph = 1;                  // replay PCG chunks as a captured HIP graph
-    int spmv_xcd = 0;                   // XCD-contiguous row ranges in the PCG SpMV
-    int pcg_fused = 1;                  // PCG direction formed inside the SpMV (four kernels per iteration)
+    int spmv_xcd = -1;                  // PCG SpMV variant (bits: XCD rows, unrolled, symmetric); -1 auto
+    int pcg_fused = -1;                 // PCG direction formed inside the SpMV; -1 auto
     tg::DBuf<double> vp2;               // second direction buffer of the fused iteration
     struct {
         hipGraph_t graph = nullptr;
@@ -334,6 +334,16 @@ inline bool mixed_stiffness(const tg_ctx* c) { return c->precision == TG_MIXED &
 inline bool mixed_operator(const tg_ctx* c) { return c->precision == TG_MIXED; }
 // the stiffness shape in effect (the measured fastest untuned choice at 128^3)
 inline int stiffness_shape(const tg_ctx* c) { return c->stiff_variant >= 0 ? c->stiff_variant : (mixed_stiffness(c) ? 17 : 9); }
+// PCG variants in effect.  Untuned: a single problem of 2^17 nodes or more
+// reads lower blocks through their partners (XCD row ranges, unrolled loop)
+// and forms the direction in its own pass, the fastest pair at 128^3 in
+// FP64 and mixed precision; batches and smaller problems read the full
+// operator in hardware order with the direction formed inside the SpMV
+// (the symmetric read is slower on small cubes, whose rows are mostly
+// boundary rows).  Every choice gives bitwise the same iterates.
+inline bool large_single(const tg_ctx* c) { return c->seg.nseg <= 1 && c->nnd >= (1 << 17); }
+inline int spmv_variant(const tg_ctx* c) { return c->spmv_xcd >= 0 ? c->spmv_xcd : (large_single(c) ? 7 : 0); }
+inline bool pcg_fused(const tg_ctx* c) { return c->pcg_fused >= 0 ? c->pcg_fused != 0 : !large_single(c); }
 
 // host setup (tg_setup.cpp)
 int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
