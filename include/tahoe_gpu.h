@@ -319,9 +319,16 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      buffer of 300 values per element and a list of
  *                      contributions (4 bytes each, 300 per element) built
  *                      on first use. */
+/* TG_TUNE_PCG_PERSIST  segments of at most this many slices (64 nodes
+ *                      each) run their whole PCG inside one persistent
+ *                      workgroup: no launch per iteration, the segment's
+ *                      vectors and operator kept in the CU's caches; larger
+ *                      segments use the kernel-per-step iteration.  The
+ *                      iterates are bitwise the same either way.  0 off,
+ *                      at most 16 (1024 nodes); -1 (default): 12. */
 enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3, TG_TUNE_PCG_GRAPH = 4,
        TG_TUNE_SPMV_XCD = 5, TG_TUNE_AMG_DEGREE = 6, TG_TUNE_AMG_WEIGHT = 7, TG_TUNE_PCG_FUSED = 8,
-       TG_TUNE_STIFF_GATHER = 9 };
+       TG_TUNE_STIFF_GATHER = 9, TG_TUNE_PCG_PERSIST = 10 };
 int tg_set_tuning(tg_ctx* ctx, int knob, long value);
 int tg_get_tuning(tg_ctx* ctx, int knob, long* value);
 

@@ -23,8 +23,13 @@ HOST_SOURCES = ["TotalLagrangianHIPT.cpp", "HIPMSRMatrixT.cpp", "NLSolverHIP.cpp
 SOURCES = ["tg_setup.cpp", "tg_amg.cpp", "tg_api.cpp", "k_element.hip", "k_q1p0.hip", "k_gather.hip", "k_solver.hip",
            "k_pcg.hip", "k_amg.hip", "k_gmres.hip"]
 HEADERS = ["tg_internal.h", "tg_kinematics.h", "tg_element.h", "../../include/tahoe_gpu.h"]
-# per-source extra flags
-EXTRA = {}
+# per-source extra flags.  The PCG kernels are built without FMA
+# contraction: every variant of the iteration (SpMV forms, fused direction,
+# persistent workgroup) then rounds each expression as written, so variants
+# that write the same expressions give bitwise the same iterates whatever
+# code surrounds them (with contraction the compiler may fuse a*b + c*d
+# either way round, kernel by kernel).  The kernels are memory-bound.
+EXTRA = {"k_pcg.hip": ["-ffp-contract=off"]}
 
 ARCH = os.environ.get("TAHOE_GPU_ARCH", "gfx950")
 # packed FP32 instructions are disabled for all device code

@@ -8,6 +8,10 @@
 // slice order, so a segment's arithmetic -- and its result -- is the same
 // alone and inside any batch.  A single problem is the one-segment case.
 //
+// This file is compiled without FMA contraction (tahoe_amd/build.py): each
+// expression rounds as written, so the iteration's variants below, which
+// share their expressions, agree bit for bit.
+//
 // The reference has no linear CG; this is the textbook method (Saad,
 // Iterative Methods for Sparse Linear Systems, Alg. 9.1) on the reference's
 // building blocks: Multx (MSRMatrixT.cpp:385-420), Jacobi scaling
@@ -338,6 +342,206 @@ __global__ void __launch_bounds__(PB) k_seg_direction(int nnd, const int* __rest
     }
 }
 
+// ---- persistent iteration: one workgroup per small segment ----------------
+//
+// A segment of at most PERSIST_MAX slices (PW = 1024 nodes) keeps its
+// whole CG loop inside one workgroup of PW threads, thread t owning node
+// 64 sb + t: its x and r stay in registers, the direction p in LDS
+// (the SpMV gathers neighbours from there), the per-slice partials and the
+// segment scalars in LDS and in thread 0's registers; only the operator is
+// read from memory (L2) each iteration, and no kernel is launched per
+// iteration.  The arithmetic is the five-kernel iteration's, in the same
+// order: the row sums of k_seg_spmv (its unrolled form, padding slots
+// multiplying zeros), the updates of k_seg_update and k_seg_direction, and
+// each per-segment total summed as seg_total sums it with FB threads (here
+// FB = PW: thread t holds slice sb + t's partial).  So the iterates are
+// bitwise those of the kernel-per-step iteration.  The loop ends on the
+// segment's own stopping test or iteration cap, as k_seg_fin_beta decides;
+// the final x, r, p and scalars are written back as that iteration leaves
+// them.
+constexpr int PW = FB;
+constexpr int PERSIST_MAX = PW / 64;
+
+// per-segment totals of NV partials held by threads 0..nsl-1 in lp[k][t],
+// summed in seg_total's order
+template <int NV>
+__device__ __forceinline__ void seg_total_lds(const double (&lp)[2][PERSIST_MAX], int nsl, double (&red)[2][PW / 64],
+                                              double (&tot)[NV])
+{
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double x = 0.0;
+        if (t < nsl) x += lp[k][t];
+        x = wsum(x);
+        if ((t & 63) == 0) red[k][t >> 6] = x;
+    }
+    __syncthreads();
+    if (t == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double x = 0.0;
+            for (int w = 0; w < PW / 64; w++) x += red[k][w];
+            tot[k] = x;
+        }
+}
+
+template <class MT, int PC>
+__global__ void __launch_bounds__(PW) k_pcg_persistent(const int* __restrict__ segs, int nnd,
+                                                       const int* __restrict__ seg_slice,
+                                                       const int* __restrict__ sbase, const int* __restrict__ swidth,
+                                                       const int* __restrict__ bcol, const MT* __restrict__ bval,
+                                                       const double* __restrict__ m, double* __restrict__ x,
+                                                       double* __restrict__ r, double* __restrict__ p, int nseg,
+                                                       double* __restrict__ sc, int* __restrict__ ndone)
+{
+    __shared__ double sp[PW][3];
+    __shared__ double lp[2][PERSIST_MAX];
+    __shared__ double red[2][PW / 64];
+    __shared__ double bc[2];
+    __shared__ int bdone;
+    const int g = segs[blockIdx.x];
+    const int sb = seg_slice[g], nsl = seg_slice[g + 1] - sb;
+    const int t = threadIdx.x, l = t & 63;
+    const int s = sb + (t >> 6);          // this thread's slice
+    const int base = 64 * sb;             // first node of the segment
+    const int a = base + t;               // this thread's node
+    const bool own = (t >> 6) < nsl && a < nnd;
+    // thread 0: the segment's scalars
+    double rz = 0.0, thresh = 0.0, maxit = 0.0, iter = 0.0, beta = 0.0, pq = 0.0, alpha = 0.0, rr = 0.0;
+    if (t == 0) {
+        bdone = *sf(sc, S_DONE, nseg, g) != 0.0;
+        rz = *sf(sc, S_RZ, nseg, g);
+        thresh = *sf(sc, S_THRESH, nseg, g);
+        maxit = *sf(sc, S_MAXIT, nseg, g);
+        iter = *sf(sc, S_ITER, nseg, g);
+        rr = *sf(sc, S_RR, nseg, g);
+    }
+    double xi[3] = {0, 0, 0}, ri[3] = {0, 0, 0}, zi[3] = {0, 0, 0}, pi[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (own) {
+            xi[i] = x[3L * a + i];
+            ri[i] = r[3L * a + i];
+            pi[i] = p[3L * a + i];
+        }
+        sp[t][i] = pi[i];
+    }
+    __syncthreads();
+    if (bdone) return;
+    for (;;) {
+        // q = A p and the slice partial of p.q (k_seg_spmv, unrolled form)
+        double qi[3] = {0, 0, 0};
+        {
+            double dv = 0.0;
+            if (own) {
+                const long b0 = sbase[s];
+                const int wd = swidth[s];
+                double y0 = 0, y1 = 0, y2 = 0;
+#pragma unroll 3
+                for (int k = 0; k < wd; k++) {
+                    const long blk = b0 + k;
+                    const int cr = bcol[blk * 64 + l];
+                    const int c = (cr < 0 ? a : cr) - base;
+                    const double x0 = sp[c][0], x1 = sp[c][1], x2 = sp[c][2];
+                    double v[9];
+                    block_values<MT, false>(bval, nullptr, blk, l, false, v);
+                    y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+                    y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+                    y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+                }
+                qi[0] = y0;
+                qi[1] = y1;
+                qi[2] = y2;
+                dv = y0 * pi[0] + y1 * pi[1] + y2 * pi[2];
+            }
+            dv = wsum(dv);
+            if (l == 0 && (t >> 6) < nsl) lp[0][t >> 6] = dv;
+        }
+        __syncthreads();
+        {
+            double tot[1];
+            seg_total_lds<1>(lp, nsl, red, tot);
+            if (t == 0) {
+                pq = tot[0];
+                alpha = rz / tot[0];  // k_seg_fin_alpha
+                bc[0] = alpha;
+            }
+        }
+        __syncthreads();
+        const double al = bc[0];
+        // x += alpha p, r -= alpha q, z = M^-1 r; slice partials of r.z, r.r (k_seg_update)
+        {
+            double v0 = 0.0, v1 = 0.0;
+            if (own) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    xi[i] += al * pi[i];
+                    ri[i] = ri[i] - al * qi[i];
+                }
+                precondition<PC>(m, a, ri, zi);
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    v0 += ri[i] * zi[i];
+                    v1 += ri[i] * ri[i];
+                }
+            }
+            v0 = wsum(v0);
+            v1 = wsum(v1);
+            if (l == 0 && (t >> 6) < nsl) {
+                lp[0][t >> 6] = v0;
+                lp[1][t >> 6] = v1;
+            }
+        }
+        __syncthreads();
+        {
+            double tot[2];
+            seg_total_lds<2>(lp, nsl, red, tot);
+            if (t == 0) {
+                // k_seg_fin_beta
+                iter = iter + 1.0;
+                beta = tot[0] / rz;
+                rz = tot[0];
+                rr = tot[1];
+                bc[1] = beta;
+                bdone = tot[1] <= thresh || iter >= maxit;
+            }
+        }
+        __syncthreads();
+        if (bdone) break;
+        const double be = bc[1];
+        // p = M^-1 r + beta p, with z formed again here as k_seg_direction
+        // forms it (the same expression, so the same contraction into FMAs)
+        if (own) {
+            double zd[3];
+            precondition<PC>(m, a, ri, zd);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                pi[i] = zd[i] + be * pi[i];
+                sp[t][i] = pi[i];
+            }
+        }
+        __syncthreads();
+    }
+    if (own)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            x[3L * a + i] = xi[i];
+            r[3L * a + i] = ri[i];
+            p[3L * a + i] = pi[i];
+        }
+    if (t == 0) {
+        *sf(sc, S_ITER, nseg, g) = iter;
+        *sf(sc, S_PQ, nseg, g) = pq;
+        *sf(sc, S_ALPHA, nseg, g) = alpha;
+        *sf(sc, S_BETA, nseg, g) = beta;
+        *sf(sc, S_RZ, nseg, g) = rz;
+        *sf(sc, S_RR, nseg, g) = rr;
+        *sf(sc, S_DONE, nseg, g) = 1.0;
+        atomicAdd(ndone, 1);
+    }
+}
+
 // ---- fused iteration: four kernels per PCG iteration -----------------------
 //
 // The direction update p = z + beta p_old is formed inside the SpMV -- for
@@ -518,6 +722,17 @@ static void pcg_loop(tg_ctx* c, const MT* val, const double* dinv, const double*
         hipLaunchKernelGGL(k_seg_fin_init, dim3(nseg), dim3(FB), 0, c->stream, W.part, ns, S.d_seg_slice.p,
                            d_active, nseg, W.scal, rtol, maxit, W.ndone);
     }
+    // small segments: the whole iteration in one persistent workgroup each
+    // (the list is built at set-up and when the threshold changes)
+    const int npers = PC == 2 || S.persist_built != pcg_persist(c) ? 0 : (int)S.h_persist.size();
+    if (npers > 0) {
+        TimedRegion tr(c, T_SPMV);
+        hipLaunchKernelGGL((k_pcg_persistent<MT, PC>), dim3(npers), dim3(PW), 0, c->stream, S.d_persist.p, c->nnd,
+                           S.d_seg_slice.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, val, dinv, x,
+                           r, p, nseg, W.scal, W.ndone);
+        TG_HIP(hipGetLastError());
+    }
+    if (npers < nseg) {
     const int CHUNK = PC == 2 ? 8 : 16;  // even: the fused iteration alternates its two direction buffers
     const bool fused = pcg_fused(c) && PC != 1;
     double* pbuf[2] = {p, nullptr};
@@ -642,6 +857,7 @@ spmv_dispatch(variant, [&](auto V) {
             if (c->hstatus[2 + prev] >= nseg) break;
         }
     }
+    }  // npers < nseg
     TG_HIP(hipStreamSynchronize(c->stream));
     // per-segment results
     if (S.h_scal.size() != (size_t)S_NSCAL * nseg) S.h_scal.resize((size_t)S_NSCAL * nseg);

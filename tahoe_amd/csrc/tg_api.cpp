@@ -121,6 +121,8 @@ static void require_init(const tg_ctx* c)
     if (!c->initialized) fail(TG_GENERAL_FAIL, "tg_initialize has not been called");
 }
 
+static void build_persist_list(tg_ctx* c);
+
 // Segment tables: slice -> segment, segment -> first slice, element -> segment.
 static void setup_segments(tg_ctx* c)
 {
@@ -160,6 +162,7 @@ static void setup_segments(tg_ctx* c)
     S.d_ndone.alloc(1);
     S.d_segstat.alloc((size_t)S.nseg);
     S.h_scal.assign((size_t)S_NSCAL * S.nseg, 0.0);
+    S.d_persist.alloc((size_t)S.nseg);
     std::vector<int> ones((size_t)S.nseg, 1);
     auto up = [&](int* dst, const std::vector<int>& v) {
         TG_HIP(hipMemcpyAsync(dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, c->stream));
@@ -174,6 +177,29 @@ static void setup_segments(tg_ctx* c)
     // null-stream operation is not ordered with a non-blocking stream), and
     // the host vectors copied from must outlive the copies
     TG_HIP(hipStreamSynchronize(c->stream));
+    build_persist_list(c);
+}
+
+// Segments of at most pcg_persist(c) slices, for the persistent PCG kernel.
+// The device list holds every segment's room (allocated with the segment
+// tables), so a rebuild only copies.
+static void build_persist_list(tg_ctx* c)
+{
+    Segments& S = c->seg;
+    const int thr = std::min(pcg_persist(c), PCG_PERSIST_MAX);
+    S.h_persist.clear();
+    if (thr > 0)
+        for (int g = 0; g < S.nseg; g++) {
+            const long sb = S.node_begin[g] / 64;
+            const long se = g + 1 < S.nseg ? S.node_begin[g + 1] / 64 : S.nslices;
+            if (se - sb <= thr) S.h_persist.push_back(g);
+        }
+    if (!S.h_persist.empty()) {
+        TG_HIP(hipMemcpyAsync(S.d_persist.p, S.h_persist.data(), sizeof(int) * S.h_persist.size(),
+                              hipMemcpyHostToDevice, c->stream));
+        TG_HIP(hipStreamSynchronize(c->stream));  // h_persist outlives its copy
+    }
+    S.persist_built = pcg_persist(c);
 }
 
 void drop_pcg_graph(tg_ctx* c)
@@ -1102,6 +1128,11 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
         } else if (knob == TG_TUNE_PCG_FUSED) {
             if (value < -1 || value > 1) fail(TG_OUT_OF_RANGE, "PCG fused must be 0 or 1, or -1 for the default");
             c->pcg_fused = (int)value;
+        } else if (knob == TG_TUNE_PCG_PERSIST) {
+            if (value < -1 || value > PCG_PERSIST_MAX)
+                fail(TG_OUT_OF_RANGE, "PCG persistent slices must be -1.." + std::to_string(PCG_PERSIST_MAX));
+            c->pcg_persist = (int)value;
+            if (c->initialized) build_persist_list(c);
         } else if (knob == TG_TUNE_STIFF_GATHER) {
             if (value < 0 || value > 1) fail(TG_OUT_OF_RANGE, "stiffness gather must be 0 or 1");
             c->stiff_gather = (int)value;
@@ -1136,6 +1167,8 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
             *value = pcg_fused(c) ? 1 : 0;
         else if (knob == TG_TUNE_STIFF_GATHER)
             *value = c->stiff_gather;
+        else if (knob == TG_TUNE_PCG_PERSIST)
+            *value = pcg_persist(c);
         else if (knob == TG_TUNE_AMG_DEGREE)
             *value = c->amg.degree;
         else if (knob == TG_TUNE_AMG_WEIGHT)

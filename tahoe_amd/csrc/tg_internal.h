@@ -225,6 +225,11 @@ struct Segments {
     DBuf<int> d_ndone;
     DBuf<int> d_segstat;                  // [nseg] 7 when an element of the segment inverted in a residual pass
     std::vector<double> h_scal;
+    // segments solved by the persistent PCG kernel (at most persist_built
+    // slices each), rebuilt when the threshold or the segments change
+    long persist_built = -2;
+    std::vector<int> h_persist;
+    DBuf<int> d_persist;
 };
 
 }  // namespace tg
@@ -271,6 +276,7 @@ struct tg_ctx {
     int pcg_graph = 1;                  // replay PCG chunks as a captured HIP graph
     int spmv_xcd = -1;                  // PCG SpMV variant (bits: XCD rows, unrolled, symmetric); -1 auto
     int pcg_fused = -1;                 // PCG direction formed inside the SpMV; -1 auto
+    int pcg_persist = -1;               // max slices of a segment run by the persistent PCG; 0 off, -1 auto
     tg::DBuf<double> vp2;               // second direction buffer of the fused iteration
     struct {
         hipGraph_t graph = nullptr;
@@ -344,6 +350,14 @@ inline int stiffness_shape(const tg_ctx* c) { return c->stiff_variant >= 0 ? c->
 inline bool large_single(const tg_ctx* c) { return c->seg.nseg <= 1 && c->nnd >= (1 << 17); }
 inline int spmv_variant(const tg_ctx* c) { return c->spmv_xcd >= 0 ? c->spmv_xcd : (large_single(c) ? 7 : 0); }
 inline bool pcg_fused(const tg_ctx* c) { return c->pcg_fused >= 0 ? c->pcg_fused != 0 : !large_single(c); }
+// segments of at most this many slices run their whole PCG in one
+// persistent workgroup of 1024 threads, one node each (0: none).  Default
+// 12 slices (768 nodes, an 8^3-element cube): there it halves a PCG
+// iteration of a batch of such problems; a segment of 16 slices is already
+// faster with a kernel per step.
+constexpr int PCG_PERSIST_MAX = 16;
+constexpr int PCG_PERSIST_DEFAULT = 12;
+inline int pcg_persist(const tg_ctx* c) { return c->pcg_persist >= 0 ? c->pcg_persist : PCG_PERSIST_DEFAULT; }
 
 // host setup (tg_setup.cpp)
 int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
