@@ -112,6 +112,26 @@ def test_multx_copy_diagonal_and_solve_match_oracle():
     assert np.linalg.norm(r) <= 1e-11 * np.linalg.norm(rhs)
 
 
+@pytest.mark.parametrize("name", ["tl_n8", "tl_n16"])
+def test_pcg_iteration_count_matches_oracle(name):
+    """Jacobi-PCG on a Newton system (K and R at a dumped iterate) takes the
+    oracle's iteration count to 1e-10: the same algorithm, so only rounding
+    may move the count (by one iteration or 1 %)."""
+    from oracle import oracle as O
+    g = load_golden(name)
+    s = make_solid(g)
+    u = g["step_u"][0]
+    s.set_displacement(u)
+    s.form_lhs()
+    R, _ = s.form_rhs()
+    val, b = s.msr_values(), s.msr_bindx()
+    xs, st = s.solve(R, rtol=1e-10)
+    xo, its, rr, rc = O.pcg_jacobi(b, val, R, rtol=1e-10)
+    assert rc == 0 and st.converged
+    assert abs(st.iterations - its) <= max(1, its // 100), (st.iterations, its)
+    assert np.abs(xs - xo).max() <= 1e-8 * np.abs(xo).max()
+
+
 def test_bad_jacobian_is_reported():
     """An inverted element gives status 7 (kBadJacobianDet), as
     TotalLagrangianT::FormKd throws (TotalLagrangianT.cpp:127-130)."""
