@@ -85,23 +85,27 @@ def size_class(req) -> int:
     return 0 if req.L <= 8 else (1 if req.L < 16 else 2)
 
 
-def solve_request(req, precision, opts, device=0, log=None, precond=None):
+def solve_request(req, precision, opts, device=0, log=None, precond=None, one_step_failed=False):
     """One request on its own context: one load step from the affine
     predictor, and when that fails the adaptive load path (batch.solve_adaptive).
     Returns (u, Newton corrections, load steps).  A request's result does not
     depend on which batch, if any, it was served in.  precond None: the
-    server's choice (precond_for)."""
+    server's choice (precond_for).  one_step_failed: the one-step solve is
+    known to fail (it failed inside a batch, whose arithmetic is the same
+    alone), so go straight to the adaptive path, which starts from a new
+    context either way: the result is the same."""
     pc = precond_for(req) if precond is None else precond
     pk = batch.pack([req.problem()])
-    s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device, precond=pc)
-    try:
+    if not one_step_failed:
+        s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device, precond=pc)
         try:
-            us, st = batch.static_solve_batch(s, pk, 1, opts)
-            return us[0], st[-1][0].iterations, 1
-        except api.TahoeGpuError:
-            pass
-    finally:
-        s.close()
+            try:
+                us, st = batch.static_solve_batch(s, pk, 1, opts)
+                return us[0], st[-1][0].iterations, 1
+            except api.TahoeGpuError:
+                pass
+        finally:
+            s.close()
     s = batch.make_solid(pk, formulation=req.formulation, precision=precision, device=device, precond=pc)
     try:
         us, steps, newton = batch.solve_adaptive(
@@ -220,8 +224,10 @@ class RVEServer:
         is reported with its status and u None; a worker never dies on one."""
         t_adm = time.perf_counter() - t0
         done = {}
+        batch_ran = False
         try:
             us, its = self._solve_group(reqs)
+            batch_ran = True
             t_batch = time.perf_counter() - t0
             for r, u, n in zip(reqs, us, its):
                 if u is not None:
@@ -233,7 +239,10 @@ class RVEServer:
             self.on_batch(bid, len(retry), reqs[0].formulation, -1.0)  # -1: members re-solved one by one
         for r in retry:
             try:
-                u, n, steps = solve_request(r, self.precision, self.opts, self.device, self.log)
+                # a member that failed in a batch that ran fails the same way
+                # alone in one step: straight to the adaptive load path
+                u, n, steps = solve_request(r, self.precision, self.opts, self.device, self.log,
+                                            one_step_failed=batch_ran)
                 done[r.rid] = (u, n, 0, steps, time.perf_counter() - t0)
             except api.TahoeGpuError as e:
                 done[r.rid] = (None, 0, e.code, 0, time.perf_counter() - t0)

@@ -112,3 +112,25 @@ def test_context_set_up_again_equals_a_new_one(form):
                 assert a is not None and np.array_equal(a, b)
     finally:
         s.close()
+
+
+def test_resolved_member_equals_alone():
+    """A request whose one-step solve fails (its predictor u0 = -2 X inverts
+    every element) fails inside its batch and is re-solved with cut load
+    steps; the server skips the one-step attempt it knows will fail, and the
+    result is bitwise that of the request solved alone (which makes the
+    attempt first, then takes the same adaptive path on a new context)."""
+    import dataclasses
+    from tahoe_amd import api, rve, serve
+    reqs = rve.batch_requests(3, lo=4, hi=5, seed=41)
+    p = reqs[1].problem()
+    reqs[1].__dict__["_problem"] = ((1000.0, 10.0, 0.3, 0.02), dataclasses.replace(p, u0=-2.0 * p.mesh.X))
+    srv = serve.RVEServer(precision=api.MIXED, workers=1, max_batch=3)
+    st = srv.run(reqs, realtime=False)
+    res = st.results[reqs[1].rid]
+    assert res.status == 0 and res.load_steps > 1 and res.batch_size == 3
+    alone, _, steps = serve.solve_request(reqs[1], api.MIXED, srv.opts)
+    assert steps == res.load_steps
+    assert np.array_equal(res.u, alone)
+    for r in (reqs[0], reqs[2]):
+        assert st.results[r.rid].load_steps == 1
