@@ -361,6 +361,7 @@ __global__ void __launch_bounds__(PB) k_seg_direction(int nnd, const int* __rest
 // them.
 constexpr int PW = FB;
 constexpr int PERSIST_MAX = PW / 64;
+static_assert(PERSIST_MAX == PCG_PERSIST_MAX, "the host's threshold cap is the kernel's slice capacity");
 
 // per-segment totals of NV partials held by threads 0..nsl-1 in lp[k][t],
 // summed in seg_total's order
