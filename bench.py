@@ -87,6 +87,11 @@ def parse_args():
                     help="1: the PCG chunks are replayed as a captured HIP graph (default); 0: direct launches of "
                          "the same kernels (for tracing tools that do not follow graph launches)")
     ap.add_argument("--step-n", type=int, default=128, help="newton-step: cube edge")
+    ap.add_argument("--amg-reuse", type=int, default=None,
+                    help="newton-step, mixed-steps: tangents of a Newton solve served by one multigrid numeric "
+                         "set-up (TG_TUNE_AMG_REUSE; default: the library's, 6)")
+    ap.add_argument("--step-legs", default="plain_fp64,fp64_jacobi_1e-3,fp64_multigrid_1e-3,mixed",
+                    help="newton-step: the solver paths to run (comma list; the factors need all four)")
     ap.add_argument("--step-stretch", type=float, default=0.005,
                     help="newton-step: top-face displacement of the one load step, fraction of the height")
     ap.add_argument("--workload", choices=["cube", "mixed-steps", "rve-batch", "rve-serve", "newton-step"],
@@ -530,6 +535,8 @@ def bench_mixed_steps(a):
         s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
         s.set_preconditioner(PRECOND[a.precond])
         n_eq, _ = s.initialize()
+        if a.precond == "amg" and a.amg_reuse is not None:
+            s.set_tuning(api.TUNE_AMG_REUSE, a.amg_reuse)
         log(f"{prec}: context ready ({n_eq} equations)")
         # large meshes: an inner solve is capped at 15,000 iterations (about a
         # minute with the multigrid at 128^3), so no correction runs unbounded
@@ -621,7 +628,10 @@ def bench_newton_step(a):
             ("fp64_jacobi_1e-3", api.FP64, api.PC_JACOBI, 1e-3, 100000),
             ("fp64_multigrid_1e-3", api.FP64, api.PC_AMG, 1e-3, 20000),
             ("mixed", api.MIXED, api.PC_AMG, 1e-3, 20000))
+    wanted = set(a.step_legs.split(","))
     for leg, prec, pc, rt, cap in legs:
+        if leg not in wanted:
+            continue
         s = api.Solid(0)
         s.set_mesh(m.X, m.conn, m.mat)
         s.set_material(0, *api.mu_kappa(1000.0, 0.3))
@@ -629,6 +639,9 @@ def bench_newton_step(a):
         s.set_kbc(kbc.nodes, kbc.dofs, kbc.values)
         s.set_preconditioner(pc)
         n_eq, _ = s.initialize()
+        s.set_tuning(api.TUNE_PCG_GRAPH, a.pcg_graph)
+        if pc == api.PC_AMG and a.amg_reuse is not None:
+            s.set_tuning(api.TUNE_AMG_REUSE, a.amg_reuse)
         log(f"{leg}: context ready ({n_eq} equations)")
         s.apply_kbc(1.0)
         s.synchronize()
@@ -643,6 +656,10 @@ def bench_newton_step(a):
             raise RuntimeError(f"{leg}: Newton did not converge ({st.iterations} corrections)")
         res[leg] = {"u": s.displacement(), "wall_s": wall, "newton": st.iterations, "pcg": st.lin_iterations}
         s.close()
+    if len(res) < len(legs):
+        # a subset of the paths (e.g. one under a profiler): timings only
+        print(json.dumps({leg: {k: v for k, v in r.items() if k != "u"} for leg, r in res.items()}), flush=True)
+        return
     ud = res["plain_fp64"]["u"]
     for leg in res:
         res[leg]["rel_err_vs_plain_fp64"] = float(np.abs(res[leg]["u"] - ud).max() / np.abs(ud).max())

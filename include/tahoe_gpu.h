@@ -326,9 +326,18 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      segments use the kernel-per-step iteration.  The
  *                      iterates are bitwise the same either way.  0 off,
  *                      at most 16 (1024 nodes); -1 (default): 12. */
+/* TG_TUNE_AMG_REUSE    multigrid: one numeric set-up of the coarse levels
+ *                      (Galerkin products, smoother intervals, coarsest
+ *                      inverse) serves up to this many further tangents of
+ *                      the same Newton solve; the fine level always smooths
+ *                      with the current operator.  tg_apply_kbc,
+ *                      tg_set_displacement, tg_close_step and tg_reset_step
+ *                      start a new solve (the next tangent sets the coarse
+ *                      levels up again).  0: a new set-up for every
+ *                      tangent; default 6. */
 enum { TG_TUNE_STIFF_CHUNK = 1, TG_TUNE_STIFF_SHAPE = 2, TG_TUNE_NEWTON_ASSEMBLY = 3, TG_TUNE_PCG_GRAPH = 4,
        TG_TUNE_SPMV_XCD = 5, TG_TUNE_AMG_DEGREE = 6, TG_TUNE_AMG_WEIGHT = 7, TG_TUNE_PCG_FUSED = 8,
-       TG_TUNE_STIFF_GATHER = 9, TG_TUNE_PCG_PERSIST = 10 };
+       TG_TUNE_STIFF_GATHER = 9, TG_TUNE_PCG_PERSIST = 10, TG_TUNE_AMG_REUSE = 11 };
 int tg_set_tuning(tg_ctx* ctx, int knob, long value);
 int tg_get_tuning(tg_ctx* ctx, int knob, long* value);
 

@@ -19,6 +19,7 @@ MAT_SIMO_ISOTROPIC, MAT_STVENANT = 0, 1
 FP64, MIXED = 0, 1
 TUNE_STIFF_CHUNK, TUNE_STIFF_SHAPE, TUNE_NEWTON_ASSEMBLY, TUNE_PCG_GRAPH, TUNE_SPMV_XCD = 1, 2, 3, 4, 5
 TUNE_AMG_DEGREE, TUNE_AMG_WEIGHT, TUNE_PCG_FUSED, TUNE_STIFF_GATHER, TUNE_PCG_PERSIST = 6, 7, 8, 9, 10
+TUNE_AMG_REUSE = 11
 PC_JACOBI, PC_BLOCK_JACOBI, PC_AMG = 0, 1, 2
 LIN_PCG, LIN_GMRES = 0, 1
 STIFF_SHAPE_EPB = (16, 12, 8, 4, 16, 8, 16, 8, 4, 12, 8, 4, 12, 8, 4, 16, 8, 4, 8)

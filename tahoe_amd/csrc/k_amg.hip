@@ -571,10 +571,25 @@ static void amg_numeric_t(tg_ctx* c, const MT* val0)
 // precision (the solver's own operator when it is FP32; refreshed from the
 // FP64 one by the caller otherwise): a preconditioner needs no more, and the
 // smoother's four fine-level products per cycle move half the bytes.
+// The coarse levels' numeric set-up (Galerkin products, smoother intervals,
+// coarsest inverse) for the current fine operator.  With reuse = k > 0 one
+// set-up serves the next k tangents of the same Newton solve too: the
+// V-cycle then smooths with the current fine operator and corrects with the
+// coarse operators of an earlier tangent of that solve (Galerkin products of
+// an SPD matrix, so the V-cycle stays an SPD preconditioner; only the PCG
+// count may change).  A new load, state or step (tg_apply_kbc,
+// tg_set_displacement, tg_close_step, tg_reset_step) clears `since`: the
+// tangents of a failed attempt never serve the next one.
 void amg_numeric(tg_ctx* c)
 {
-    if (c->amg.lv.empty()) amg_setup(c);
+    Amg& G = c->amg;
+    if (G.lv.empty()) amg_setup(c);
+    if (G.reuse > 0 && G.since >= 0 && G.since < G.reuse) {
+        G.since++;
+        return;
+    }
     amg_numeric_t<float>(c, c->op.d_val32.p);
+    G.since = 0;
 }
 
 template <class Op>

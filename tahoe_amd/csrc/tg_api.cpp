@@ -623,6 +623,7 @@ int tg_set_displacement(tg_ctx* c, const double* u)
         require_init(c);
         TG_HIP(hipMemcpyAsync(c->du.p, u, sizeof(double) * 3 * (size_t)c->nnd, hipMemcpyHostToDevice, c->stream));
         TG_HIP(hipStreamSynchronize(c->stream));
+        c->amg.since = -1;  // a new state: the next tangent sets the coarse levels up
     });
 }
 
@@ -675,6 +676,7 @@ int tg_apply_kbc(tg_ctx* c, double t)
     return guarded(c, [&] {
         require_init(c);
         launch_apply_kbc(c, t);
+        c->amg.since = -1;  // a new load: the next tangent sets the coarse levels up
     });
 }
 
@@ -690,6 +692,7 @@ int tg_close_step(tg_ctx* c)
         require_init(c);
         TG_HIP(hipMemcpyAsync(c->du_last.p, c->du.p, sizeof(double) * 3 * (size_t)c->nnd, hipMemcpyDeviceToDevice,
                               c->stream));
+        c->amg.since = -1;
     });
 }
 
@@ -700,6 +703,7 @@ int tg_reset_step(tg_ctx* c)
         require_init(c);
         TG_HIP(hipMemcpyAsync(c->du.p, c->du_last.p, sizeof(double) * 3 * (size_t)c->nnd, hipMemcpyDeviceToDevice,
                               c->stream));
+        c->amg.since = -1;  // back to the last converged state (a failed step): no coarse levels of its tangents
     });
 }
 
@@ -1128,6 +1132,10 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
         } else if (knob == TG_TUNE_PCG_FUSED) {
             if (value < -1 || value > 1) fail(TG_OUT_OF_RANGE, "PCG fused must be 0 or 1, or -1 for the default");
             c->pcg_fused = (int)value;
+        } else if (knob == TG_TUNE_AMG_REUSE) {
+            if (value < 0 || value > 64) fail(TG_OUT_OF_RANGE, "multigrid reuse must be 0..64");
+            c->amg.reuse = (int)value;
+            c->amg.since = -1;
         } else if (knob == TG_TUNE_PCG_PERSIST) {
             if (value < -1 || value > PCG_PERSIST_MAX)
                 fail(TG_OUT_OF_RANGE, "PCG persistent slices must be -1.." + std::to_string(PCG_PERSIST_MAX));
@@ -1140,10 +1148,12 @@ int tg_set_tuning(tg_ctx* c, int knob, long value)
             if (value < 2 || value > 4) fail(TG_OUT_OF_RANGE, "multigrid smoother degree must be 2..4");
             c->amg.degree = (int)value;
             c->amg.gen++;
+            c->amg.since = -1;
         } else if (knob == TG_TUNE_AMG_WEIGHT) {
             if (value < 100 || value > 190) fail(TG_OUT_OF_RANGE, "multigrid coarse weight must be 100..190");
             c->amg.weight = (int)value;
             c->amg.gen++;
+            c->amg.since = -1;
         } else
             fail(TG_BAD_INPUT, "unknown tuning knob " + std::to_string(knob));
     });
@@ -1169,6 +1179,8 @@ int tg_get_tuning(tg_ctx* c, int knob, long* value)
             *value = c->stiff_gather;
         else if (knob == TG_TUNE_PCG_PERSIST)
             *value = pcg_persist(c);
+        else if (knob == TG_TUNE_AMG_REUSE)
+            *value = c->amg.reuse;
         else if (knob == TG_TUNE_AMG_DEGREE)
             *value = c->amg.degree;
         else if (knob == TG_TUNE_AMG_WEIGHT)

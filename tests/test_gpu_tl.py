@@ -283,13 +283,16 @@ def test_spmv_xcd_mapping_bitwise(precision):
         assert np.array_equal(o[0], out[0][0])
 
 
+@pytest.mark.parametrize("reuse", [0, 6])
 @pytest.mark.parametrize("name,precision", [("tl_n8", 0), ("tl_n16", 0), ("rve_n4", 0), ("q1p0_n4", 0),
                                             ("tl_n16", 1)])
-def test_amg_newton(name, precision):
+def test_amg_newton(name, precision, reuse):
     """The aggregation-multigrid preconditioner changes only the inner
     solve: the static solve reaches the reference's displacements (1e-9)
     with the reference's Newton counts, in far fewer PCG iterations than
-    point Jacobi on the larger meshes."""
+    point Jacobi on the larger meshes -- with a numeric set-up of the coarse
+    levels for every tangent (reuse 0) and with one set-up per Newton solve
+    serving its later tangents (reuse 6, the default)."""
     from tahoe_amd import api
     g = load_golden(name)
     its = {}
@@ -299,6 +302,8 @@ def test_amg_newton(name, precision):
             s.set_precision(api.MIXED)
             s.initialize()
         s.set_preconditioner(pc)
+        s.set_tuning(api.TUNE_AMG_REUSE, reuse)
+        assert s.tuning(api.TUNE_AMG_REUSE) == reuse
         lin = []
         us, iters = api.static_solve(s, int(g["num_steps"]), s.opts(lin_rtol=1e-12),
                                      on_step=lambda k, st: lin.append(st.lin_iterations))
