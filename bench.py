@@ -682,20 +682,23 @@ def bench_newton_step(a):
     print(json.dumps(line), flush=True)
 
 
-def solve_alone(req, precision, opts, problem=None):
-    """One request in its own context (the plain one-problem-at-a-time path).
-    Returns (u, Newton stats, seconds for pack + context + solve, seconds for
-    the solve alone)."""
-    from tahoe_amd import batch
+def solve_alone(req, precision, opts, problem=None, ctx=None):
+    """One request set up and solved alone (the plain one-problem-at-a-time
+    path), in context ctx (set up again for it) or a new one.  Returns (u,
+    Newton stats, seconds for pack + set-up + solve, seconds for the solve
+    alone)."""
+    from tahoe_amd import api, batch
     p = problem if problem is not None else req.problem()
     t0 = time.perf_counter()
     pk = batch.pack([p])
-    s = batch.make_solid(pk, formulation=req.formulation, precision=precision)
+    s = batch.configure(ctx if ctx is not None else api.Solid(0), pk, formulation=req.formulation,
+                        precision=precision)
     t1 = time.perf_counter()
     us, st = batch.static_solve_batch(s, pk, 1, opts)
     s.synchronize()
     t2 = time.perf_counter()
-    s.close()
+    if ctx is None:
+        s.close()
     return us[0], st[0][0], t2 - t0, t2 - t1
 
 
@@ -728,18 +731,23 @@ def bench_rve_batch(a):
     # plain FP64, one at a time, on every k-th request
     k = max(1, len(reqs) // max(1, a.plain_sample))
     sample = reqs[::k][:a.plain_sample]
-    solve_alone(sample[0], api.FP64, fp64)  # warm-up (library load, first context)
+    # one context, set up again for every RVE (as the batched legs reuse one
+    # for every batch): the comparison is of batching and precision, not of
+    # context creation
+    plain_ctx = api.Solid(0)
+    solve_alone(sample[0], api.FP64, fp64, ctx=plain_ctx)  # warm-up (library load)
     plain_u, plain_wall, plain_solve, plain_el = {}, 0.0, 0.0, 0
     probs = {r.rid: r.problem() for r in reqs}   # mesh generation: the client's, not timed
     for i, r in enumerate(sample):
         if i % 32 == 0:
             print(f"# plain FP64 RVE {i}/{len(sample)}", file=sys.stderr, flush=True)
-        u, st, wall, solve = solve_alone(r, api.FP64, fp64, probs[r.rid])
+        u, st, wall, solve = solve_alone(r, api.FP64, fp64, probs[r.rid], ctx=plain_ctx)
         assert st.status == 0
         plain_u[r.rid] = u
         plain_wall += wall
         plain_solve += solve
         plain_el += r.elements
+    plain_ctx.close()
     print(f"# plain FP64: {len(sample)} RVEs, {plain_el} elements in {plain_wall:.2f} s", file=sys.stderr,
           flush=True)
     # batched over every request, three ways: FP64 with the plain inner
@@ -750,19 +758,20 @@ def bench_rve_batch(a):
     def run_batched(tag, precision, opts):
         t0 = time.perf_counter()
         solve_s, errs, newton_max = 0.0, [], 0
+        s = api.Solid(0)   # one context, set up again for every batch
         for b in batches:
             pk = batch.pack([probs[r.rid] for r in b])
-            s = batch.make_solid(pk, formulation=api.TOTAL_LAGRANGIAN, precision=precision)
+            batch.configure(s, pk, formulation=api.TOTAL_LAGRANGIAN, precision=precision)
             t1 = time.perf_counter()
             us, st = batch.static_solve_batch(s, pk, 1, opts)
             s.synchronize()
             solve_s += time.perf_counter() - t1
-            s.close()
             newton_max = max(newton_max, max(x.iterations for x in st[0]))
             for r, u in zip(b, us):
                 if r.rid in plain_u:
                     ref = plain_u[r.rid]
                     errs.append(float(np.abs(u - ref).max() / max(np.abs(ref).max(), 1e-300)))
+        s.close()
         wall = time.perf_counter() - t0
         log(f"{tag}: {len(batches)} batches in {wall:.2f} s (Newton alone {solve_s:.2f} s)")
         return {"wall_s": wall, "solve_s": solve_s, "elements_per_s": tot_el / wall,
@@ -793,13 +802,14 @@ def bench_rve_batch(a):
         "solve_only_elements_per_s": main_leg["solve_only_elements_per_s"],
         "plain_fp64": {"rves": len(sample), "elements": plain_el, "wall_s": plain_wall,
                        "elements_per_s": plain_rate, "solve_only_elements_per_s": plain_el / plain_solve,
-                       "note": "every k-th request, one context per RVE, FP64 K + Jacobi-PCG to 1e-10"},
+                       "note": "every k-th request, one at a time (one context set up again per RVE), FP64 K + "
+                               "Jacobi-PCG to 1e-10"},
         "legs": legs,
         "contributions": {"batching (one at a time -> batched, FP64 PCG to 1e-10)": r64 / plain_rate,
                           f"inexact inner solve (1e-10 -> {a.batch_lin_rtol:g}, batched FP64)": r64i / r64,
                           "precision (FP64 -> mixed, batched, same tolerance)": value / r64i},
-        "timed": "every path: pack + context setup (tg_initialize) + Newton to convergence; mesh generation "
-                 "excluded; solve_only: Newton alone",
+        "timed": "every path: pack + set-up of one reused context (tg_set_mesh ... tg_initialize) + Newton to "
+                 "convergence; mesh generation excluded; solve_only: Newton alone",
         "speedup_solve_only": main_leg["solve_only_elements_per_s"] / (plain_el / plain_solve),
         "speedup_vs_plain_fp64": value / plain_rate,
         "max_rel_err_vs_fp64": main_leg["max_rel_err_vs_fp64"], "compared_rves": main_leg["compared_rves"],

@@ -39,49 +39,54 @@ class Packed:
 
 
 def pack(problems: list) -> Packed:
-    mats = []
-    index = {}
-    Xs, conns, matl, kn, kd, kv, u0s = [], [], [], [], [], [], []
-    begins = [0]
-    counts = []
-    off = 0
+    """The sub-problems one after another in one mesh: each starts on a
+    multiple of 64 nodes (a slice of the solver operator), the padding nodes
+    fully prescribed to zero; material ids map into one global (E, nu)
+    table.  Each array is one concatenation of the problems' own arrays and
+    views of shared padding arrays, plus one offset pass."""
+    mats, index, gids, gbeg = [], {}, [], [0]
     for p in problems:
-        gid = []
         for E, nu in zip(p.E, p.nu):
             key = (float(E), float(nu))
             if key not in index:
                 index[key] = len(mats)
                 mats.append(key)
-            gid.append(index[key])
-        gid = np.asarray(gid, dtype=np.int32)
-        nnd = p.mesh.nnd
-        pad = (-nnd) % 64
-        X = np.zeros((nnd + pad, 3))
-        X[:nnd] = p.mesh.X
-        Xs.append(X)
-        u0 = np.zeros((nnd + pad, 3))
-        if p.u0 is not None:
-            u0[:nnd] = p.u0
-        u0s.append(u0)
-        conns.append(p.mesh.conn + off)
-        matl.append(gid[p.mesh.mat])
-        kn.append(p.kbc.nodes + off)
-        kd.append(p.kbc.dofs)
-        kv.append(p.kbc.values)
-        if pad:
-            padn = np.arange(off + nnd, off + nnd + pad, dtype=np.int32)
-            kn.append(np.repeat(padn, 3))
-            kd.append(np.tile(np.arange(3, dtype=np.int32), pad))
-            kv.append(np.zeros(3 * pad))
-        counts.append(nnd)
-        off += nnd + pad
-        begins.append(off)
+            gids.append(index[key])
+        gbeg.append(len(gids))
+    nnds = np.array([p.mesh.nnd for p in problems], dtype=np.int64)
+    pads = (-nnds) % 64
+    begins = np.zeros(len(problems) + 1, dtype=np.int64)
+    begins[1:] = np.cumsum(nnds + pads)
+    nels = np.array([p.mesh.conn.shape[0] for p in problems], dtype=np.int64)
+    zeros3 = np.zeros((64, 3))
+    X = np.concatenate([a for p, q in zip(problems, pads) for a in (p.mesh.X, zeros3[:q])])
+    u0 = None
+    if any(p.u0 is not None for p in problems):
+        u0 = np.concatenate([a for p, q in zip(problems, pads)
+                             for a in (p.u0 if p.u0 is not None else np.zeros_like(p.mesh.X), zeros3[:q])])
+    conn = np.concatenate([p.mesh.conn for p in problems]).astype(np.int32, copy=False)
+    conn += np.repeat(begins[:-1].astype(np.int32), nels)[:, None]
+    local = np.concatenate([p.mesh.mat for p in problems]).astype(np.int32, copy=False)
+    local += np.repeat(np.asarray(gbeg[:-1], dtype=np.int32), nels)
+    mat = np.asarray(gids, dtype=np.int32)[local]
+    # each problem's cards, then its padding nodes' (3 dofs each): node
+    # numbers local to the problem's padding start, then one offset pass
+    node3 = np.repeat(np.arange(64, dtype=np.int32), 3)
+    dof3 = np.tile(np.arange(3, dtype=np.int32), 64)
+    zero_v = np.zeros(192)
+    nks = np.array([len(p.kbc.nodes) for p in problems], dtype=np.int64)
+    kn = np.concatenate([a for p, q in zip(problems, pads) for a in (p.kbc.nodes, node3[:3 * q])]).astype(np.int32, copy=False)
+    kd = np.concatenate([a for p, q in zip(problems, pads) for a in (p.kbc.dofs, dof3[:3 * q])]).astype(np.int32, copy=False)
+    kv = np.concatenate([a for p, q in zip(problems, pads) for a in (p.kbc.values, zero_v[:3 * q])]).astype(np.float64, copy=False)
+    shift = np.empty(2 * len(problems), dtype=np.int32)
+    shift[0::2] = begins[:-1]               # a problem's own cards
+    shift[1::2] = begins[:-1] + nnds        # its padding nodes
+    lens = np.empty(2 * len(problems), dtype=np.int64)
+    lens[0::2] = nks
+    lens[1::2] = 3 * pads
+    kn += np.repeat(shift, lens)
     # the last sub-problem needs no trailing padding, but keeping it is harmless
-    any_u0 = any(p.u0 is not None for p in problems)
-    return Packed(np.concatenate(Xs), np.concatenate(conns).astype(np.int32), np.concatenate(matl).astype(np.int32),
-                  KBC(np.concatenate(kn), np.concatenate(kd), np.concatenate(kv)),
-                  np.asarray(begins, dtype=np.int32), np.asarray(counts, dtype=np.int32), mats,
-                  np.concatenate(u0s) if any_u0 else None)
+    return Packed(X, conn, mat, KBC(kn, kd, kv), begins.astype(np.int32), nnds.astype(np.int32), mats, u0)
 
 
 def make_solid(pk: Packed, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64, device=0,

@@ -1161,6 +1161,34 @@ void launch_build_operator_diag(tg_ctx* c)
     TG_HIP(hipGetLastError());
 }
 
+// Columns of the upper MSR rows of node A (slice s, lane l), from the block
+// operator's neighbour list (ascending, padding -1 at the end): row
+// r = eq(A, i) takes every active eq(B, j) > r.  Equation numbers grow with
+// (node, dof), so the columns come out ascending and their count is the row
+// length the host counted (bindx[r + 1] - bindx[r], already on the device);
+// the write stays inside the row's range whatever the count.
+__global__ void k_msr_cols(int nnd, const int* __restrict__ sbase, const int* __restrict__ swidth,
+                           const int* __restrict__ bcol, const int* __restrict__ eqn, int* __restrict__ bindx)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnd) return;
+    const int s = a >> 6, l = a & 63;
+    for (int i = 0; i < 3; i++) {
+        const int r = eqn[3L * a + i];
+        if (r < 0) continue;
+        int pos = bindx[r];
+        const int end = bindx[r + 1];
+        for (int k = 0; k < swidth[s]; k++) {
+            const int B = bcol[((long)sbase[s] + k) * 64 + l];
+            if (B < 0) break;
+            for (int j = 0; j < 3; j++) {
+                const int cc = eqn[3L * B + j];
+                if (cc > r && pos < end) bindx[pos++] = cc;
+            }
+        }
+    }
+}
+
 // Operator value -> MSR value it mirrors: for row node A (slice s, lane l)
 // and the column node of each slot, entry (i, j) is the upper MSR entry
 // (min(r, c), max(r, c)) of r = eq(A, i), c = eq(B, j); -1 when either dof is
@@ -1230,6 +1258,15 @@ __global__ void k_build_tpos(int nnd, const int* __restrict__ sbase, const int* 
         }
         tpos[g * 64 + l] = out;
     }
+}
+
+void launch_msr_cols(tg_ctx* c)
+{
+    const int bs = 256;
+    if (c->nnd == 0) return;
+    hipLaunchKernelGGL(k_msr_cols, dim3((unsigned)((c->nnd + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nnd,
+                       c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->deqn.p, c->dbindx.p);
+    TG_HIP(hipGetLastError());
 }
 
 void launch_build_src(tg_ctx* c)

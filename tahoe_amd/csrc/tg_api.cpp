@@ -11,6 +11,7 @@
 #include <cstring>
 #include <exception>
 #include <new>
+#include <thread>
 
 #include "tg_internal.h"
 
@@ -355,12 +356,12 @@ int tg_set_mesh(tg_ctx* c, int nnd, int nel, const double* X, const int* conn, c
         // leaves the previous one (or none) in place
         c->nnd = nnd;
         c->nel = nel;
-        c->X.assign(X, X + 3L * nnd);
         c->seg.nseg = 1;
         c->seg.node_begin = {0, nnd};
-        c->conn.assign(conn, conn + 8L * nel);
+        parallel_copy(c->X, X, 3L * nnd);
+        parallel_copy(c->conn, conn, 8L * nel);
         if (mat)
-            c->mat.assign(mat, mat + nel);
+            parallel_copy(c->mat, mat, nel);
         else
             c->mat.assign((size_t)nel, 0);
         c->initialized = false;
@@ -474,6 +475,52 @@ int tg_set_kbc(tg_ctx* c, int nkbc, const int* nodes, const int* dofs, const dou
     });
 }
 
+// Device storage of the mesh and the vectors, and the uploads that need
+// neither the node graph nor the colouring (tg_initialize runs those on
+// other host threads meanwhile).
+static void upload_mesh(tg_ctx* c, int nmat)
+{
+    // device storage
+    const size_t nn = 3 * (size_t)c->nnd;
+    c->dX.alloc(nn);
+    c->du.alloc(nn);
+    c->du_last.alloc(nn);
+    c->dR.alloc(nn);
+    c->dconn.alloc(8 * (size_t)c->nel);
+    c->dmat.alloc((size_t)c->nel);
+    c->deqn.alloc(nn);
+    c->dcolor_elems.alloc((size_t)c->nel);
+    c->dmatprops.alloc((size_t)std::max(1, nmat));
+    c->dslot.alloc(300 * (size_t)c->nel);
+    for (auto* v : {&c->vx, &c->vr, &c->vz, &c->vp, &c->vq, &c->vdinv, &c->vwork}) v->alloc(nn);
+    const size_t sblocks = ((size_t)c->nnd + 255) / 256;
+    c->dpart.alloc(2 * std::max<size_t>(sblocks, 1024) + 16);
+    std::vector<int> eqn0(nn);
+    for (size_t k = 0; k < nn; k++) eqn0[k] = c->eqnos[k] > 0 ? c->eqnos[k] - 1 : -1;
+    std::vector<double2> props((size_t)std::max(1, nmat));
+    for (int m = 0; m < nmat; m++) props[m] = make_double2(c->mu[m], c->kappa[m]);
+    std::vector<int> kidx(c->kbc_nodes.size());
+    for (size_t k = 0; k < kidx.size(); k++) kidx[k] = 3 * c->kbc_nodes[k] + c->kbc_dofs[k];
+    c->dkbc_idx.alloc(kidx.size());
+    c->dkbc_val.alloc(kidx.size());
+    TG_HIP(hipMemcpyAsync(c->dX.p, c->X.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    TG_HIP(hipMemsetAsync(c->du.p, 0, sizeof(double) * nn, c->stream));
+    TG_HIP(hipMemsetAsync(c->du_last.p, 0, sizeof(double) * nn, c->stream));
+    TG_HIP(hipMemcpyAsync(c->dconn.p, c->conn.data(), sizeof(int) * c->conn.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    TG_HIP(hipMemcpyAsync(c->dmat.p, c->mat.data(), sizeof(int) * c->mat.size(), hipMemcpyHostToDevice, c->stream));
+    TG_HIP(hipMemcpyAsync(c->deqn.p, eqn0.data(), sizeof(int) * nn, hipMemcpyHostToDevice, c->stream));
+    TG_HIP(hipMemcpyAsync(c->dmatprops.p, props.data(), sizeof(double2) * props.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    if (!kidx.empty()) {
+        TG_HIP(hipMemcpyAsync(c->dkbc_idx.p, kidx.data(), sizeof(int) * kidx.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        TG_HIP(hipMemcpyAsync(c->dkbc_val.p, c->kbc_vals.data(), sizeof(double) * kidx.size(),
+                              hipMemcpyHostToDevice, c->stream));
+    }
+    TG_HIP(hipStreamSynchronize(c->stream));  // the host sources above are locals
+}
+
 int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
 {
     if (!c) return TG_BAD_INPUT;
@@ -505,73 +552,60 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
                          std::chrono::duration<double, std::milli>(now - t_last).count(), c->nel);
             t_last = now;
         };
-        // equations, pattern, colours
+        // equations; then the node graph and MSR rows, and the colouring, on
+        // host threads of their own while this thread allocates and uploads
+        // what needs neither
         c->n_eq = equation_numbers(c->nnd, c->kbc_nodes, c->kbc_dofs, c->eqnos);
         phase("equation numbers");
-        node_graph(c->nnd, c->nel, c->conn.data(), c->nptr, c->nlist);
-        phase("node graph");
-        msr_pattern(c->n_eq, c->nnd, c->eqnos.data(), c->nptr, c->nlist, c->bindx);
-        c->msr_len = (long)c->bindx.size();
-        phase("MSR pattern");
-        c->ncolors = color_elements(c->nnd, c->nel, c->conn.data(), c->color_ptr, c->color_elems);
-        phase("colouring");
         c->stiff_chunk_built = -2;
         c->gather_built = -2;
         c->dgptr.release();
         c->dglist.release();
         c->debuf.release();
         c->debuf32.release();
-        // device storage
-        const size_t nn = 3 * (size_t)c->nnd;
-        c->dX.alloc(nn);
-        c->du.alloc(nn);
-        c->du_last.alloc(nn);
-        c->dR.alloc(nn);
-        c->dconn.alloc(8 * (size_t)c->nel);
-        c->dmat.alloc((size_t)c->nel);
-        c->deqn.alloc(nn);
-        c->dcolor_elems.alloc((size_t)c->nel);
-        c->dmatprops.alloc((size_t)std::max(1, nmat));
+        {
+            std::exception_ptr graph_err, colour_err;
+            std::thread graph([&] {
+                try {
+                    node_graph(c->nnd, c->nel, c->conn.data(), c->nptr, c->nlist);
+                    msr_rows(c->n_eq, c->nnd, c->eqnos.data(), c->nptr, c->nlist, c->bindx);
+                } catch (...) {
+                    graph_err = std::current_exception();
+                }
+            });
+            std::thread colour([&] {
+                try {
+                    c->ncolors = color_elements(c->nnd, c->nel, c->conn.data(), c->color_ptr, c->color_elems);
+                } catch (...) {
+                    colour_err = std::current_exception();
+                }
+            });
+            std::exception_ptr main_err;
+            try {
+                upload_mesh(c, nmat);
+            } catch (...) {
+                main_err = std::current_exception();
+            }
+            graph.join();
+            colour.join();
+            for (auto& e : {main_err, graph_err, colour_err})
+                if (e) std::rethrow_exception(e);
+        }
+        c->msr_len = (long)c->bindx[(size_t)c->n_eq];
         c->dbindx.alloc((size_t)c->msr_len);
         c->dval.alloc((size_t)c->msr_len);
-        c->dslot.alloc(300 * (size_t)c->nel);
-        for (auto* v : {&c->vx, &c->vr, &c->vz, &c->vp, &c->vq, &c->vdinv, &c->vwork}) v->alloc(nn);
-        const bool mixed = mixed_operator(c);
         c->dval32.alloc(mixed_stiffness(c) ? (size_t)c->msr_len : 0);
-        const size_t sblocks = ((size_t)c->nnd + 255) / 256;
-        c->dpart.alloc(2 * std::max<size_t>(sblocks, 1024) + 16);
-        std::vector<int> eqn0(nn);
-        for (size_t k = 0; k < nn; k++) eqn0[k] = c->eqnos[k] > 0 ? c->eqnos[k] - 1 : -1;
-        std::vector<double2> props((size_t)std::max(1, nmat));
-        for (int m = 0; m < nmat; m++) props[m] = make_double2(c->mu[m], c->kappa[m]);
-        std::vector<int> kidx(c->kbc_nodes.size());
-        for (size_t k = 0; k < kidx.size(); k++) kidx[k] = 3 * c->kbc_nodes[k] + c->kbc_dofs[k];
-        c->dkbc_idx.alloc(kidx.size());
-        c->dkbc_val.alloc(kidx.size());
-        TG_HIP(hipMemcpyAsync(c->dX.p, c->X.data(), sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
-        TG_HIP(hipMemsetAsync(c->du.p, 0, sizeof(double) * nn, c->stream));
-        TG_HIP(hipMemsetAsync(c->du_last.p, 0, sizeof(double) * nn, c->stream));
-        TG_HIP(hipMemcpyAsync(c->dconn.p, c->conn.data(), sizeof(int) * c->conn.size(), hipMemcpyHostToDevice,
-                              c->stream));
-        TG_HIP(hipMemcpyAsync(c->dmat.p, c->mat.data(), sizeof(int) * c->mat.size(), hipMemcpyHostToDevice, c->stream));
-        TG_HIP(hipMemcpyAsync(c->deqn.p, eqn0.data(), sizeof(int) * nn, hipMemcpyHostToDevice, c->stream));
+        // row pointers; the columns are written on the device with the block operator
+        TG_HIP(hipMemcpyAsync(c->dbindx.p, c->bindx.data(), sizeof(int) * ((size_t)c->n_eq + 1),
+                              hipMemcpyHostToDevice, c->stream));
         TG_HIP(hipMemcpyAsync(c->dcolor_elems.p, c->color_elems.data(), sizeof(int) * c->nel, hipMemcpyHostToDevice,
                               c->stream));
-        TG_HIP(hipMemcpyAsync(c->dmatprops.p, props.data(), sizeof(double2) * props.size(), hipMemcpyHostToDevice,
-                              c->stream));
-        TG_HIP(hipMemcpyAsync(c->dbindx.p, c->bindx.data(), sizeof(int) * c->msr_len, hipMemcpyHostToDevice,
-                              c->stream));
-        if (!kidx.empty()) {
-            TG_HIP(hipMemcpyAsync(c->dkbc_idx.p, kidx.data(), sizeof(int) * kidx.size(), hipMemcpyHostToDevice,
-                                  c->stream));
-            TG_HIP(hipMemcpyAsync(c->dkbc_val.p, c->kbc_vals.data(), sizeof(double) * kidx.size(),
-                                  hipMemcpyHostToDevice, c->stream));
-        }
-        phase("allocate + upload");
+        const bool mixed = mixed_operator(c);
+        phase("graph + colours | upload");
+        build_bsell(c);
+        phase("block operator + MSR");
         launch_build_slots(c);
         phase("slot map");
-        build_bsell(c);
-        phase("block operator");
         c->op.d_diag.alloc((size_t)c->nnd);
         launch_build_operator_diag(c);
         if (c->formulation == TG_TOTAL_LAGRANGIAN) {
@@ -612,8 +646,12 @@ int tg_get_msr_bindx(const tg_ctx* c, int* bindx)
 {
     if (!c || !bindx) return TG_BAD_INPUT;
     if (!c->initialized) return TG_GENERAL_FAIL;
-    std::memcpy(bindx, c->bindx.data(), sizeof(int) * c->bindx.size());
-    return TG_OK;
+    // the pattern lives on the device (its columns are written there)
+    tg_ctx* m = const_cast<tg_ctx*>(c);
+    return guarded(m, [&] {
+        TG_HIP(hipMemcpyAsync(bindx, m->dbindx.p, sizeof(int) * (size_t)m->msr_len, hipMemcpyDeviceToHost, m->stream));
+        TG_HIP(hipStreamSynchronize(m->stream));
+    });
 }
 
 int tg_set_displacement(tg_ctx* c, const double* u)

@@ -81,6 +81,32 @@ void parallel_for(long n, F f)
     for (auto& x : th) x.join();
 }
 
+// dst = src[0, n), copied by the set-up threads
+template <class V, class T>
+void parallel_copy(V& dst, const T* src, long n)
+{
+    dst.resize((size_t)n);
+    parallel_for(n, [&](long b, long e) { std::copy(src + b, src + e, dst.data() + b); });
+}
+
+// f(t, begin, end) for the t-th of `parts` contiguous ranges of [0, n), one
+// thread each (t = 0 on the calling thread when parts == 1).
+template <class F>
+void parallel_parts(long n, int parts, F f)
+{
+    const long chunk = (n + parts - 1) / std::max(parts, 1);
+    if (parts <= 1) {
+        f(0, 0L, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < parts; t++) {
+        const long b = std::min(n, t * chunk), e = std::min(n, b + chunk);
+        th.emplace_back([=] { f(t, b, e); });
+    }
+    for (auto& x : th) x.join();
+}
+
 // Device buffer owned by a context.  alloc keeps an allocation that is
 // already large enough (n shrinks, the memory stays), so a context set up
 // again for a mesh no larger than its largest so far calls no hipMalloc or
@@ -262,7 +288,7 @@ struct tg_ctx {
     bool initialized = false;
     int n_eq = 0;
     std::vector<int> eqnos;   // [nnd*3] 1-based, -1 prescribed
-    tg::ivec bindx;           // upper MSR pattern
+    tg::ivec bindx;           // upper MSR row pointers bindx[0..n_eq] (the columns: dbindx)
     long msr_len = 0;
     std::vector<int> color_ptr, color_elems;
     int ncolors = 0;
@@ -367,7 +393,7 @@ inline int pcg_persist(const tg_ctx* c) { return c->pcg_persist >= 0 ? c->pcg_pe
 // host setup (tg_setup.cpp)
 int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
 void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist);
-void msr_pattern(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const ivec& nlist, ivec& bindx);
+void msr_rows(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const ivec& nlist, ivec& bindx);
 int color_elements(int nnd, int nel, const int* conn, std::vector<int>& ptr, std::vector<int>& elems);
 void build_bsell(tg_ctx* c);
 
@@ -380,6 +406,7 @@ void build_gather_lists(tg_ctx* c, const int* dorder);
 void launch_gather(tg_ctx* c);
 void launch_stiffness_operator(tg_ctx* c, const double* u);
 void launch_build_operator_slots(tg_ctx* c);
+void launch_msr_cols(tg_ctx* c);
 void launch_build_src(tg_ctx* c);
 void launch_operator_dinv(tg_ctx* c);
 void launch_block_inverse(tg_ctx* c);

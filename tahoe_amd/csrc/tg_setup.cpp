@@ -43,93 +43,84 @@ static void node_elements(int nnd, int nel, const int* conn, std::vector<int>& p
 }
 
 // Node adjacency as CSR (sorted unique neighbours including the node itself),
-// built once and shared by the MSR pattern and the block operator.  Each
-// worker de-duplicates with a per-node stamp array, so only the unique
-// neighbours are sorted.
+// built once and shared by the MSR pattern and the block operator.  One pass:
+// each worker de-duplicates the neighbours of its node range with a per-node
+// stamp array, sorts them into a list of its own, and the lists are then
+// concatenated in node order.
 void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist)
 {
     std::vector<int> ptr;
     ivec list;
     node_elements(nnd, nel, conn, ptr, list);
-    const int MAXNB = 8 * 64;
     for (int a = 0; a < nnd; a++)
-        if (ptr[a + 1] - ptr[a] > MAXNB / 8) fail(TG_OUT_OF_RANGE, "a node is shared by more than 64 elements");
-    // out == nullptr: count only
-    auto unique_neighbours = [&](int a, std::vector<int>& stamp, int* out) {
-        int n = 0;
-        for (int p = ptr[a]; p < ptr[a + 1]; p++) {
-            const int* c = conn + 8L * list[p];
-            for (int b = 0; b < 8; b++)
-                if (stamp[c[b]] != a) {
-                    stamp[c[b]] = a;
-                    if (out) out[n] = c[b];
-                    n++;
-                }
-        }
-        if (out) std::sort(out, out + n);
-        return n;
-    };
+        if (ptr[a + 1] - ptr[a] > 64) fail(TG_OUT_OF_RANGE, "a node is shared by more than 64 elements");
+    const int parts = nnd < 4096 ? 1 : worker_count();
+    std::vector<ivec> part((size_t)parts);
     std::vector<int> deg((size_t)nnd);
-    parallel_for(nnd, [&](long b, long e) {
+    parallel_parts(nnd, parts, [&](int t, long b, long e) {
+        if (b >= e) return;
         std::vector<int> stamp((size_t)nnd, -1);
-        for (long a = b; a < e; a++) deg[a] = unique_neighbours((int)a, stamp, nullptr);
+        ivec& out = part[(size_t)t];
+        out.resize((size_t)(8L * (ptr[e] - ptr[b])));  // the candidates bound the unique count
+        int* o = out.data();
+        size_t m = 0;
+        for (long a = b; a < e; a++) {
+            const size_t n0 = m;
+            for (int p = ptr[a]; p < ptr[a + 1]; p++) {
+                const int* c = conn + 8L * list[p];
+                for (int k = 0; k < 8; k++)
+                    if (stamp[c[k]] != (int)a) {
+                        stamp[c[k]] = (int)a;
+                        o[m++] = c[k];
+                    }
+            }
+            std::sort(o + n0, o + m);
+            deg[a] = (int)(m - n0);
+        }
+        out.resize(m);
     });
     nptr.assign((size_t)nnd + 1, 0);
     for (int a = 0; a < nnd; a++) nptr[a + 1] = nptr[a] + deg[a];
     nlist.resize((size_t)nptr[nnd]);
-    parallel_for(nnd, [&](long b, long e) {
-        std::vector<int> stamp((size_t)nnd, -1);
-        for (long a = b; a < e; a++) unique_neighbours((int)a, stamp, nlist.data() + nptr[a]);
+    parallel_parts(nnd, parts, [&](int t, long b, long e) {
+        if (b < e) std::memcpy(nlist.data() + nptr[b], part[(size_t)t].data(), sizeof(int) * part[(size_t)t].size());
     });
 }
 
-// Upper-triangle MSR pattern of GraphT::MakeGraph(active, add_self, upper_only)
-// (toolbox/src/graph/GraphT.cpp:376-456) packed by MSRBuilderT::GenerateMSR
-// (toolbox/src/graph/MSRBuilderT.cpp:134-181): row i lists every active
-// equation j > i sharing an element with i, ascending; bindx[0] = n_eq + 1.
-void msr_pattern(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const ivec& nlist, ivec& bindx)
+// Row pointers of the upper-triangle MSR pattern of GraphT::MakeGraph(active,
+// add_self, upper_only) (toolbox/src/graph/GraphT.cpp:376-456) packed by
+// MSRBuilderT::GenerateMSR (toolbox/src/graph/MSRBuilderT.cpp:134-181): row r
+// lists every active equation > r sharing an element with r, ascending;
+// bindx[0] = n_eq + 1 and bindx[r + 1] - bindx[r] is row r's length.
+// Equation numbers are node-major and dof-minor (equation_numbers), so the
+// columns of row (a, i) are the active dofs j > i of node a and every active
+// dof of each neighbour b > a: a row's length is counted per node, and the
+// columns themselves are written on the device (launch_msr_cols) from the
+// block operator's neighbour lists.
+void msr_rows(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const ivec& nlist, ivec& bindx)
 {
     std::vector<int> rowlen((size_t)n_eq, 0);
-    auto columns = [&](long a, int row, int* cols) {
-        int m = 0;
-        for (long k = nptr[a]; k < nptr[a + 1]; k++)
-            for (int j = 0; j < 3; j++) {
-                int col = eqnos[3L * nlist[k] + j];
-                if (col > row) cols[m++] = col - 1;
-            }
-        // equation numbers need not follow node order in general (they do
-        // for the node-major numbering, and then the columns come sorted)
-        if (!std::is_sorted(cols, cols + m)) std::sort(cols, cols + m);
-        return m;
-    };
-    const int MAXNB = 8 * 64;
     parallel_for(nnd, [&](long b, long e) {
-        for (long a = b; a < e; a++)
+        for (long a = b; a < e; a++) {
+            int later = 0;  // active dofs of the neighbours after a
+            for (long k = nptr[a]; k < nptr[a + 1]; k++)
+                if (nlist[k] > a)
+                    for (int j = 0; j < 3; j++) later += eqnos[3L * nlist[k] + j] > 0;
             for (int i = 0; i < 3; i++) {
                 const int row = eqnos[3 * a + i];
                 if (row <= 0) continue;
-                int m = 0;  // the count of columns() without building the list
-                for (long k = nptr[a]; k < nptr[a + 1]; k++)
-                    for (int j = 0; j < 3; j++) m += eqnos[3L * nlist[k] + j] > row;
-                rowlen[row - 1] = m;
+                int own = 0;
+                for (int j = i + 1; j < 3; j++) own += eqnos[3 * a + j] > 0;
+                rowlen[row - 1] = later + own;
             }
+        }
     });
     long total = 0;
     for (int r = 0; r < n_eq; r++) total += rowlen[r];
     if ((long)n_eq + 1 + total > 0x7fffffffL) fail(TG_OUT_OF_RANGE, "MSR pattern exceeds 32-bit indexing");
-    bindx.resize((size_t)(n_eq + 1 + total));  // every entry is written below
+    bindx.resize((size_t)n_eq + 1);
     bindx[0] = n_eq + 1;
     for (int r = 0; r < n_eq; r++) bindx[r + 1] = bindx[r] + rowlen[r];
-    parallel_for(nnd, [&](long b, long e) {
-        std::vector<int> cols(3 * MAXNB);
-        for (long a = b; a < e; a++)
-            for (int i = 0; i < 3; i++) {
-                int row = eqnos[3 * a + i];
-                if (row <= 0) continue;
-                int m = columns(a, row, cols.data());
-                std::memcpy(&bindx[(size_t)bindx[row - 1]], cols.data(), sizeof(int) * (size_t)m);
-            }
-    });
 }
 
 // First-fit greedy colouring: no two elements of one colour share a node, so
@@ -173,16 +164,17 @@ void build_bsell(tg_ctx* c)
     B.nslices = (nnd + 63) / 64;
     B.slice_base.assign((size_t)B.nslices, 0);
     B.slice_width.assign((size_t)B.nslices, 0);
+    parallel_for(B.nslices, [&](long sb, long se) {
+        for (long s = sb; s < se; s++) {
+            int w = 0;
+            for (long a = 64 * s; a < std::min((long)nnd, 64 * s + 64); a++) w = std::max(w, (int)(nptr[a + 1] - nptr[a]));
+            B.slice_width[s] = w;
+        }
+    });
     long base = 0;
     for (int s = 0; s < B.nslices; s++) {
-        int w = 0;
-        for (int l = 0; l < 64 && 64 * s + l < nnd; l++) {
-            const long a = 64L * s + l;
-            w = std::max(w, (int)(nptr[a + 1] - nptr[a]));
-        }
         B.slice_base[s] = (int)base;
-        B.slice_width[s] = w;
-        base += w;
+        base += B.slice_width[s];
     }
     B.nblocks_padded = base * 64;
     if (B.nblocks_padded * 9 > 0x7fffffffL) fail(TG_OUT_OF_RANGE, "block operator exceeds 32-bit indexing");
@@ -209,6 +201,7 @@ void build_bsell(tg_ctx* c)
     TG_HIP(hipMemcpyAsync(B.d_slice_base.p, B.slice_base.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice, c->stream));
     TG_HIP(hipMemcpyAsync(B.d_slice_width.p, B.slice_width.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice, c->stream));
     TG_HIP(hipMemcpyAsync(B.d_col.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice, c->stream));
+    launch_msr_cols(c);  // the MSR columns from the neighbour lists, before anything reads them
     launch_build_src(c);
     // MSR ranges of the rows of each half slice (node-major equations: the
     // active rows of 32 consecutive nodes are consecutive equations, their

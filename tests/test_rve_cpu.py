@@ -77,6 +77,43 @@ def test_pack_segments_on_64_node_boundaries():
         assert np.array_equal(part, u[b:b + n])
 
 
+def test_pack_contents_problem_by_problem():
+    """Each sub-problem's arrays land at its node offset unchanged: X rows,
+    connectivity shifted by the offset, materials through the global table,
+    its own cards (shifted) and then one card per dof of each padding node;
+    predictors where given, zeros elsewhere.  Edge 3 gives 64 nodes: no
+    padding."""
+    import dataclasses
+    reqs = [rve.Request(i, L, api.TOTAL_LAGRANGIAN, nu, seed=20 + i)
+            for i, (L, nu) in enumerate(((4, 0.3), (3, 0.3), (5, 0.25), (3, 0.3)))]
+    probs = [r.problem() for r in reqs]
+    probs[1] = dataclasses.replace(probs[1], u0=None)
+    pk = batch.pack(probs)
+    assert pk.X.dtype == np.float64 and pk.conn.dtype == np.int32 and pk.mat.dtype == np.int32
+    assert pk.kbc.nodes.dtype == np.int32 and pk.kbc.dofs.dtype == np.int32
+    assert pk.node_begin[-1] == pk.X.shape[0] == pk.u0.shape[0]
+    e0, k0 = 0, 0
+    for p, b, e in zip(probs, pk.node_begin[:-1], pk.node_begin[1:]):
+        n, ne, nk = p.mesh.nnd, p.mesh.conn.shape[0], len(p.kbc.nodes)
+        assert np.array_equal(pk.X[b:b + n], p.mesh.X) and not pk.X[b + n:e].any()
+        assert np.array_equal(pk.u0[b:b + n], p.u0 if p.u0 is not None else 0 * p.mesh.X)
+        assert not pk.u0[b + n:e].any()
+        assert np.array_equal(pk.conn[e0:e0 + ne], p.mesh.conn + b)
+        got = [pk.materials[m] for m in pk.mat[e0:e0 + ne]]
+        assert got == [(float(p.E[m]), float(p.nu[m])) for m in p.mesh.mat]
+        assert np.array_equal(pk.kbc.nodes[k0:k0 + nk], p.kbc.nodes + b)
+        assert np.array_equal(pk.kbc.dofs[k0:k0 + nk], p.kbc.dofs)
+        assert np.array_equal(pk.kbc.values[k0:k0 + nk], p.kbc.values)
+        pad = e - b - n
+        assert np.array_equal(pk.kbc.nodes[k0 + nk:k0 + nk + 3 * pad], np.repeat(np.arange(b + n, e), 3))
+        assert np.array_equal(pk.kbc.dofs[k0 + nk:k0 + nk + 3 * pad], np.tile([0, 1, 2], pad))
+        assert not pk.kbc.values[k0 + nk:k0 + nk + 3 * pad].any()
+        e0, k0 = e0 + ne, k0 + nk + 3 * pad
+    assert e0 == pk.conn.shape[0] and k0 == len(pk.kbc.nodes)
+    assert pk.node_begin[2] - pk.node_begin[1] == 64   # the edge-3 cube fills its slice
+    assert batch.pack([probs[1]]).u0 is None
+
+
 def test_footprint_grows_with_size_and_precision():
     small = rve.Request(0, 4, api.TOTAL_LAGRANGIAN, 0.3, 1)
     big = rve.Request(1, 32, api.TOTAL_LAGRANGIAN, 0.3, 1)
