@@ -1189,10 +1189,15 @@ __global__ void k_msr_cols(int nnd, const int* __restrict__ sbase, const int* __
     }
 }
 
-// Operator value -> MSR value it mirrors: for row node A (slice s, lane l)
-// and the column node of each slot, entry (i, j) is the upper MSR entry
+// Operator value -> MSR value it mirrors, for the diagonal and upper blocks
+// of row node A (slice s, lane l): entry (i, j) is the upper MSR entry
 // (min(r, c), max(r, c)) of r = eq(A, i), c = eq(B, j); -1 when either dof is
-// prescribed or the slot is padding.
+// prescribed, for padding, and for lower blocks (column B < A), which the
+// expansion copies from their upper partners.  Equation numbers grow with
+// (node, dof), so row r = eq(A, i) holds the active dofs of A after i, then
+// the active dofs of each neighbour B > A in ascending order: an entry's
+// position is the row start plus a running count over the slots, with no
+// search.
 __global__ void k_build_src(int nnd, const int* __restrict__ sbase, const int* __restrict__ swidth,
                             const int* __restrict__ bcol, const int* __restrict__ eqn,
                             const int* __restrict__ bindx, int* __restrict__ src)
@@ -1200,37 +1205,37 @@ __global__ void k_build_src(int nnd, const int* __restrict__ sbase, const int* _
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = a >> 6, l = a & 63;
     if (s >= (nnd + 63) / 64) return;
+    int r[3], first[3];  // row of each dof of A (-1 prescribed); its first column after A's own dofs
+#pragma unroll
+    for (int i = 0; i < 3; i++) r[i] = a < nnd ? eqn[3L * a + i] : -1;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        int own = 0;
+        for (int j = i + 1; j < 3; j++) own += r[j] >= 0;
+        first[i] = r[i] >= 0 ? bindx[r[i]] + own : 0;
+    }
+    int run = 0;  // active dofs of the neighbours after A and before B
     for (int k = 0; k < swidth[s]; k++) {
         const long g = (long)sbase[s] + k;
         const int B = a < nnd ? bcol[g * 64 + l] : -1;
+        int cb[3] = {-1, -1, -1};
+        if (B > a)
+#pragma unroll
+            for (int j = 0; j < 3; j++) cb[j] = eqn[3L * B + j];
+#pragma unroll
         for (int i = 0; i < 3; i++)
+#pragma unroll
             for (int j = 0; j < 3; j++) {
                 int v = -1;
-                if (B >= 0) {
-                    const int r = eqn[3L * a + i], cc = eqn[3L * B + j];
-                    if (r >= 0 && cc >= 0) {
-                        const int row = r < cc ? r : cc, col = r < cc ? cc : r;
-                        if (row == col)
-                            v = row;
-                        else {
-                            int lo = bindx[row], hi = bindx[row + 1] - 1;
-                            while (lo <= hi) {
-                                const int mid = (lo + hi) >> 1;
-                                const int x = bindx[mid];
-                                if (x == col) {
-                                    v = mid;
-                                    break;
-                                }
-                                if (x < col)
-                                    lo = mid + 1;
-                                else
-                                    hi = mid - 1;
-                            }
-                        }
-                    }
+                if (B == a && r[i] >= 0 && r[j] >= 0) {
+                    const int lo = i < j ? i : j, hi = i < j ? j : i;
+                    v = i == j ? r[i] : bindx[r[lo]] + (hi - lo == 2 && r[lo + 1] >= 0 ? 1 : 0);
+                } else if (B > a && r[i] >= 0 && cb[j] >= 0) {
+                    v = first[i] + run + (j > 0 && cb[0] >= 0) + (j > 1 && cb[1] >= 0);
                 }
                 src[(g * 9 + 3 * i + j) * 64 + l] = v;
             }
+        if (B > a) run += (cb[0] >= 0) + (cb[1] >= 0) + (cb[2] >= 0);
     }
 }
 
@@ -1249,12 +1254,22 @@ __global__ void k_build_tpos(int nnd, const int* __restrict__ sbase, const int* 
         const int B = a < nnd ? bcol[g * 64 + l] : -1;
         int out = -1;
         if (B >= 0 && B < a) {
+            // row B lists its neighbours ascending, then padding (-1): a
+            // binary search with padding taken as larger than any node
             const int sB = B >> 6, lB = B & 63;
-            for (int kk = 0; kk < swidth[sB]; kk++)
-                if (bcol[((long)sbase[sB] + kk) * 64 + lB] == a) {
-                    out = (int)((((long)sbase[sB] + kk) * 9) * 64 + lB);
+            int lo = 0, hi = swidth[sB] - 1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                const int x = bcol[((long)sbase[sB] + mid) * 64 + lB];
+                if (x == a) {
+                    out = (int)((((long)sbase[sB] + mid) * 9) * 64 + lB);
                     break;
                 }
+                if (x >= 0 && x < a)
+                    lo = mid + 1;
+                else
+                    hi = mid - 1;
+            }
         }
         tpos[g * 64 + l] = out;
     }

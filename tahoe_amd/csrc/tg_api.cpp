@@ -245,6 +245,23 @@ static void check_status_segments(tg_ctx* c, std::vector<int>& bad)
     clear_status(c);
 }
 
+SetupTrace::SetupTrace(tg_ctx* ctx, const char* name) : c(ctx), who(name)
+{
+    static const bool trace = std::getenv("TG_SETUP_TRACE") != nullptr;
+    on = trace;
+    t = std::chrono::steady_clock::now();
+}
+
+void SetupTrace::operator()(const char* what)
+{
+    if (!on) return;
+    TG_HIP(hipStreamSynchronize(c->stream));
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[%s] %-26s %8.1f ms (nel %d)\n", who, what,
+                 std::chrono::duration<double, std::milli>(now - t).count(), c->nel);
+    t = now;
+}
+
 }  // namespace tg
 
 using namespace tg;
@@ -541,17 +558,7 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         // a captured PCG chunk refers to buffers that are about to be replaced
         drop_pcg_graph(c);
         c->dfext.release();
-        // TG_SETUP_TRACE=1 prints the host-side setup phases to stderr
-        static const bool trace = std::getenv("TG_SETUP_TRACE") != nullptr;
-        auto t_last = std::chrono::steady_clock::now();
-        auto phase = [&](const char* what) {
-            if (!trace) return;
-            TG_HIP(hipStreamSynchronize(c->stream));
-            const auto now = std::chrono::steady_clock::now();
-            std::fprintf(stderr, "[tg_initialize] %-22s %8.1f ms (nel %d)\n", what,
-                         std::chrono::duration<double, std::milli>(now - t_last).count(), c->nel);
-            t_last = now;
-        };
+        SetupTrace phase(c, "tg_initialize");
         // equations; then the node graph and MSR rows, and the colouring, on
         // host threads of their own while this thread allocates and uploads
         // what needs neither
@@ -565,13 +572,17 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         c->debuf32.release();
         {
             std::exception_ptr graph_err, colour_err;
+            using clk = std::chrono::steady_clock;
+            const auto t0 = clk::now();
+            clk::time_point t_graph, t_colour, t_upload;
             std::thread graph([&] {
                 try {
-                    node_graph(c->nnd, c->nel, c->conn.data(), c->nptr, c->nlist);
+                    node_graph(c->nnd, c->nel, c->conn.data(), c->nptr, c->nlist, &c->graph_work);
                     msr_rows(c->n_eq, c->nnd, c->eqnos.data(), c->nptr, c->nlist, c->bindx);
                 } catch (...) {
                     graph_err = std::current_exception();
                 }
+                t_graph = clk::now();
             });
             std::thread colour([&] {
                 try {
@@ -579,6 +590,7 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
                 } catch (...) {
                     colour_err = std::current_exception();
                 }
+                t_colour = clk::now();
             });
             std::exception_ptr main_err;
             try {
@@ -586,10 +598,16 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
             } catch (...) {
                 main_err = std::current_exception();
             }
+            t_upload = clk::now();
             graph.join();
             colour.join();
             for (auto& e : {main_err, graph_err, colour_err})
                 if (e) std::rethrow_exception(e);
+            if (phase.on) {
+                auto ms = [&](clk::time_point t) { return std::chrono::duration<double, std::milli>(t - t0).count(); };
+                std::fprintf(stderr, "[tg_initialize]   graph + MSR rows %.1f ms, colouring %.1f ms, upload %.1f ms\n",
+                             ms(t_graph), ms(t_colour), ms(t_upload));
+            }
         }
         c->msr_len = (long)c->bindx[(size_t)c->n_eq];
         c->dbindx.alloc((size_t)c->msr_len);
@@ -623,8 +641,8 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         } else {
             c->amg.clear();
         }
-        std::vector<long>().swap(c->nptr);
-        ivec().swap(c->nlist);
+        c->nptr.clear();  // the capacity stays for the next set-up
+        c->nlist.clear();
         setup_segments(c);
         TG_HIP(hipStreamSynchronize(c->stream));
         phase("segments");

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <memory>
 #include <utility>
@@ -41,6 +42,14 @@ struct NoInit : std::allocator<T> {
     }
 };
 using ivec = std::vector<int, NoInit<int>>;
+
+// host buffers of node_graph (capacity kept across set-ups)
+struct GraphWork {
+    std::vector<int> ptr, deg;
+    ivec list;
+    std::vector<ivec> part;
+    std::vector<std::vector<int>> stamp;
+};
 
 // Status raised inside the library; converted to the C status at the ABI.
 struct Error {
@@ -152,7 +161,7 @@ struct Bsell {
     ivec h_col;  // host staging of d_col, kept across set-ups
     DBuf<int> d_slice_base, d_slice_width;
     DBuf<int> d_col;                       // [nblocks_padded] column node, -1 = padding
-    DBuf<int> d_src;                       // [nblocks_padded * 9] MSR val index, -1 = zero
+    DBuf<int> d_src;                       // [nblocks_padded * 9] MSR val index, -1 = zero (and lower blocks: copied from partners)
     // per half slice (32 block rows): the contiguous MSR ranges its rows own
     // -- off-diagonal values [e0, e1) and diagonal values [r0, r1) -- staged
     // in LDS by the expansion; lds_values = the largest (e1 - e0 + r1 - r0)
@@ -292,8 +301,9 @@ struct tg_ctx {
     long msr_len = 0;
     std::vector<int> color_ptr, color_elems;
     int ncolors = 0;
-    std::vector<long> nptr;             // node adjacency (setup only, released after tg_initialize)
-    tg::ivec nlist;
+    std::vector<long> nptr;             // node adjacency (set-up only; emptied after tg_initialize,
+    tg::ivec nlist;                     // capacity kept)
+    tg::GraphWork graph_work;           // node_graph's buffers, kept across set-ups
     // stiffness launch order: every segment's elements in ranges of
     // stiff_chunk (by rank inside the segment), all colours of range j
     // before range j+1 (0: whole colours)
@@ -390,9 +400,20 @@ constexpr int PCG_PERSIST_MAX = 16;
 constexpr int PCG_PERSIST_DEFAULT = 12;
 inline int pcg_persist(const tg_ctx* c) { return c->pcg_persist >= 0 ? c->pcg_persist : PCG_PERSIST_DEFAULT; }
 
+// TG_SETUP_TRACE=1 prints host-side set-up phases to stderr, each timed
+// to a stream synchronisation; otherwise a no-op.
+struct SetupTrace {
+    tg_ctx* c;
+    const char* who;
+    bool on;
+    std::chrono::steady_clock::time_point t;
+    SetupTrace(tg_ctx* ctx, const char* name);
+    void operator()(const char* what);
+};
+
 // host setup (tg_setup.cpp)
 int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
-void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist);
+void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist, GraphWork* w = nullptr);
 void msr_rows(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const ivec& nlist, ivec& bindx);
 int color_elements(int nnd, int nel, const int* conn, std::vector<int>& ptr, std::vector<int>& elems);
 void build_bsell(tg_ctx* c);

@@ -46,21 +46,27 @@ static void node_elements(int nnd, int nel, const int* conn, std::vector<int>& p
 // built once and shared by the MSR pattern and the block operator.  One pass:
 // each worker de-duplicates the neighbours of its node range with a per-node
 // stamp array, sorts them into a list of its own, and the lists are then
-// concatenated in node order.
-void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist)
+// concatenated in node order.  The buffers live in w (a context's, kept
+// across set-ups so that a set-up of a mesh no larger than before touches no
+// new host pages) or, without one, in a local workspace.
+void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist, GraphWork* w)
 {
-    std::vector<int> ptr;
-    ivec list;
-    node_elements(nnd, nel, conn, ptr, list);
+    GraphWork local;
+    GraphWork& g = w ? *w : local;
+    node_elements(nnd, nel, conn, g.ptr, g.list);
+    const std::vector<int>& ptr = g.ptr;
+    const ivec& list = g.list;
     for (int a = 0; a < nnd; a++)
         if (ptr[a + 1] - ptr[a] > 64) fail(TG_OUT_OF_RANGE, "a node is shared by more than 64 elements");
     const int parts = nnd < 4096 ? 1 : worker_count();
-    std::vector<ivec> part((size_t)parts);
-    std::vector<int> deg((size_t)nnd);
+    if ((int)g.part.size() < parts) g.part.resize((size_t)parts);
+    if ((int)g.stamp.size() < parts) g.stamp.resize((size_t)parts);
+    g.deg.resize((size_t)nnd);
     parallel_parts(nnd, parts, [&](int t, long b, long e) {
         if (b >= e) return;
-        std::vector<int> stamp((size_t)nnd, -1);
-        ivec& out = part[(size_t)t];
+        std::vector<int>& stamp = g.stamp[(size_t)t];
+        stamp.assign((size_t)nnd, -1);
+        ivec& out = g.part[(size_t)t];
         out.resize((size_t)(8L * (ptr[e] - ptr[b])));  // the candidates bound the unique count
         int* o = out.data();
         size_t m = 0;
@@ -75,15 +81,17 @@ void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec
                     }
             }
             std::sort(o + n0, o + m);
-            deg[a] = (int)(m - n0);
+            g.deg[a] = (int)(m - n0);
         }
         out.resize(m);
     });
-    nptr.assign((size_t)nnd + 1, 0);
-    for (int a = 0; a < nnd; a++) nptr[a + 1] = nptr[a] + deg[a];
+    nptr.resize((size_t)nnd + 1);
+    nptr[0] = 0;
+    for (int a = 0; a < nnd; a++) nptr[a + 1] = nptr[a] + g.deg[a];
     nlist.resize((size_t)nptr[nnd]);
     parallel_parts(nnd, parts, [&](int t, long b, long e) {
-        if (b < e) std::memcpy(nlist.data() + nptr[b], part[(size_t)t].data(), sizeof(int) * part[(size_t)t].size());
+        const ivec& src = g.part[(size_t)t];
+        if (b < e) std::memcpy(nlist.data() + nptr[b], src.data(), sizeof(int) * src.size());
     });
 }
 
@@ -157,6 +165,7 @@ int color_elements(int nnd, int nel, const int* conn, std::vector<int>& cptr, st
 // partner.
 void build_bsell(tg_ctx* c)
 {
+    SetupTrace phase(c, "build_bsell");
     Bsell& B = c->op;
     const int nnd = c->nnd;
     const std::vector<long>& nptr = c->nptr;
@@ -192,6 +201,7 @@ void build_bsell(tg_ctx* c)
             }
         }
     });
+    phase("slice widths + columns");
     B.d_slice_base.alloc(B.nslices);
     B.d_slice_width.alloc(B.nslices);
     B.d_col.alloc((size_t)B.nblocks_padded);
@@ -201,8 +211,10 @@ void build_bsell(tg_ctx* c)
     TG_HIP(hipMemcpyAsync(B.d_slice_base.p, B.slice_base.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice, c->stream));
     TG_HIP(hipMemcpyAsync(B.d_slice_width.p, B.slice_width.data(), sizeof(int) * B.nslices, hipMemcpyHostToDevice, c->stream));
     TG_HIP(hipMemcpyAsync(B.d_col.p, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice, c->stream));
+    phase("allocate + upload");
     launch_msr_cols(c);  // the MSR columns from the neighbour lists, before anything reads them
     launch_build_src(c);
+    phase("MSR columns, src, tpos");
     // MSR ranges of the rows of each half slice (node-major equations: the
     // active rows of 32 consecutive nodes are consecutive equations, their
     // off-diagonal values consecutive)
@@ -230,6 +242,7 @@ void build_bsell(tg_ctx* c)
     B.d_srange.alloc(rng.size());
     TG_HIP(hipMemcpyAsync(B.d_srange.p, rng.data(), sizeof(int4) * rng.size(), hipMemcpyHostToDevice, c->stream));
     TG_HIP(hipStreamSynchronize(c->stream));  // col and rng (host) outlive their copies
+    phase("row ranges");
 }
 
 
