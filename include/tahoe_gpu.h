@@ -295,7 +295,8 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      recent reads serve the rest from cache.  Every
  *                      variant gives bitwise the same product.  0 .. 7, or
  *                      -1 (default): 7 for a single problem of 2^17 nodes
- *                      or more, else 0. */
+ *                      or more, 2 for a batch of 2^17 nodes or more, else
+ *                      0. */
 /* TG_TUNE_AMG_DEGREE   multigrid: Chebyshev degree of each pre- and
  *                      post-smoothing pass, 2 (default) .. 4. */
 /* TG_TUNE_AMG_WEIGHT   multigrid: weight of the coarse-grid correction in
@@ -307,7 +308,7 @@ int tg_calib_copy(int device, long bytes, int reps, double* ms);
  *                      separate direction pass) for the Jacobi and
  *                      multigrid preconditioners, in any SpMV variant;
  *                      0: the five-kernel iteration; -1 (default): 0 for a
- *                      single problem of 2^17 nodes or more, else 1. */
+ *                      problem or batch of 2^17 nodes or more, else 1. */
 /* TG_TUNE_STIFF_GATHER 0 (default): the total-Lagrangian stiffness is
  *                      added into the MSR element by element (colour-
  *                      ordered read-modify-write through the slot map);

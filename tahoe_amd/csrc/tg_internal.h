@@ -384,13 +384,21 @@ inline int stiffness_shape(const tg_ctx* c) { return c->stiff_variant >= 0 ? c->
 // PCG variants in effect.  Untuned: a single problem of 2^17 nodes or more
 // reads lower blocks through their partners (XCD row ranges, unrolled loop)
 // and forms the direction in its own pass, the fastest pair at 128^3 in
-// FP64 and mixed precision; batches and smaller problems read the full
-// operator in hardware order with the direction formed inside the SpMV
-// (the symmetric read is slower on small cubes, whose rows are mostly
-// boundary rows).  Every choice gives bitwise the same iterates.
+// FP64 and mixed precision.  A batch of 2^17 nodes or more reads the full
+// operator in index order with the unrolled loop, direction in its own pass
+// (the fastest pair on 2 M-element batches of RVE cubes of edge 4-24: the
+// symmetric read is slower on small cubes, whose rows are mostly boundary
+// rows, and the fused direction's extra gathers cost more than the launch
+// it saves).  Smaller problems and batches, which are launch-bound, read in
+// hardware order with the direction formed inside the SpMV.  Every choice
+// gives bitwise the same iterates.
 inline bool large_single(const tg_ctx* c) { return c->seg.nseg <= 1 && c->nnd >= (1 << 17); }
-inline int spmv_variant(const tg_ctx* c) { return c->spmv_xcd >= 0 ? c->spmv_xcd : (large_single(c) ? 7 : 0); }
-inline bool pcg_fused(const tg_ctx* c) { return c->pcg_fused >= 0 ? c->pcg_fused != 0 : !large_single(c); }
+inline bool large_total(const tg_ctx* c) { return c->nnd >= (1 << 17); }
+inline int spmv_variant(const tg_ctx* c)
+{
+    return c->spmv_xcd >= 0 ? c->spmv_xcd : (large_single(c) ? 7 : large_total(c) ? 2 : 0);
+}
+inline bool pcg_fused(const tg_ctx* c) { return c->pcg_fused >= 0 ? c->pcg_fused != 0 : !large_total(c); }
 // segments of at most this many slices run their whole PCG in one
 // persistent workgroup of 1024 threads, one node each (0: none).  Default
 // 12 slices (768 nodes, an 8^3-element cube): there it halves a PCG

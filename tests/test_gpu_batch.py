@@ -82,3 +82,23 @@ def test_graph_replayed_pcg_equals_direct_launches(precision):
     assert out[0][1] == out[1][1]
     for a, b in zip(out[0][0], out[1][0]):
         assert np.array_equal(a, b)
+
+
+def test_default_pcg_variant_by_size():
+    """Untuned PCG variant: a single problem of 2^17 nodes or more reads the
+    lower blocks through their partners with the direction in its own pass
+    (7, unfused); a batch of that size reads the full operator with the
+    unrolled loop, direction in its own pass (2, unfused); anything smaller
+    fuses the direction into the hardware-order SpMV (0, fused)."""
+    from tahoe_amd import api, batch
+    small = batch.pack([rve_problem(4, 1), rve_problem(5, 2)])
+    big = batch.pack([rve_problem(14, 10 + k) for k in range(40)])
+    assert big.X.shape[0] >= 1 << 17
+    single = batch.pack([rve_problem(50, 3)])
+    assert single.X.shape[0] >= 1 << 17
+    for pk, want in ((small, (0, 1)), (big, (2, 0)), (single, (7, 0))):
+        s = batch.make_solid(pk, precision=api.MIXED)
+        try:
+            assert (s.tuning(api.TUNE_SPMV_XCD), s.tuning(api.TUNE_PCG_FUSED)) == want
+        finally:
+            s.close()
