@@ -21,7 +21,7 @@ HOST_SOURCES = ["TotalLagrangianHIPT.cpp", "HIPMSRMatrixT.cpp", "NLSolverHIP.cpp
                 "FEManagerHIP.cpp"]
 
 SOURCES = ["tg_setup.cpp", "tg_amg.cpp", "tg_api.cpp", "k_element.hip", "k_q1p0.hip", "k_gather.hip", "k_solver.hip",
-           "k_pcg.hip", "k_amg.hip", "k_gmres.hip"]
+           "k_pcg.hip", "k_amg.hip", "k_gmres.hip", "k_graph.hip"]
 HEADERS = ["tg_internal.h", "tg_kinematics.h", "tg_element.h", "../../include/tahoe_gpu.h"]
 # per-source extra flags.  The PCG kernels are built without FMA
 # contraction: every variant of the iteration (SpMV forms, fused direction,

@@ -1165,7 +1165,7 @@ void launch_build_operator_diag(tg_ctx* c)
 // operator's neighbour list (ascending, padding -1 at the end): row
 // r = eq(A, i) takes every active eq(B, j) > r.  Equation numbers grow with
 // (node, dof), so the columns come out ascending and their count is the row
-// length the host counted (bindx[r + 1] - bindx[r], already on the device);
+// length counted beforehand (bindx[r + 1] - bindx[r], already on the device);
 // the write stays inside the row's range whatever the count.
 __global__ void k_msr_cols(int nnd, const int* __restrict__ sbase, const int* __restrict__ swidth,
                            const int* __restrict__ bcol, const int* __restrict__ eqn, int* __restrict__ bindx)

@@ -155,13 +155,8 @@ void amg_setup(tg_ctx* c)
     {
         std::vector<long> ptr;
         ivec lst;
-        if (c->nptr.empty()) {
-            node_graph(nnd, c->nel, c->conn.data(), ptr, lst);
-            g.adj.assign(lst.begin(), lst.end());
-        } else {
-            ptr = c->nptr;
-            g.adj.assign(c->nlist.begin(), c->nlist.end());
-        }
+        node_graph(nnd, c->nel, c->conn.data(), ptr, lst, &c->graph_work);
+        g.adj.assign(lst.begin(), lst.end());
         g.ptr.swap(ptr);
     }
     std::vector<double> xyz(c->X);

@@ -559,9 +559,9 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         drop_pcg_graph(c);
         c->dfext.release();
         SetupTrace phase(c, "tg_initialize");
-        // equations; then the node graph and MSR rows, and the colouring, on
-        // host threads of their own while this thread allocates and uploads
-        // what needs neither
+        // equations; then the element colouring on a host thread of its own
+        // while this thread uploads the mesh and builds the structure on the
+        // device
         c->n_eq = equation_numbers(c->nnd, c->kbc_nodes, c->kbc_dofs, c->eqnos);
         phase("equation numbers");
         c->stiff_chunk_built = -2;
@@ -571,19 +571,10 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         c->debuf.release();
         c->debuf32.release();
         {
-            std::exception_ptr graph_err, colour_err;
+            std::exception_ptr colour_err, main_err;
             using clk = std::chrono::steady_clock;
             const auto t0 = clk::now();
-            clk::time_point t_graph, t_colour, t_upload;
-            std::thread graph([&] {
-                try {
-                    node_graph(c->nnd, c->nel, c->conn.data(), c->nptr, c->nlist, &c->graph_work);
-                    msr_rows(c->n_eq, c->nnd, c->eqnos.data(), c->nptr, c->nlist, c->bindx);
-                } catch (...) {
-                    graph_err = std::current_exception();
-                }
-                t_graph = clk::now();
-            });
+            clk::time_point t_colour, t_upload, t_graph;
             std::thread colour([&] {
                 try {
                     c->ncolors = color_elements(c->nnd, c->nel, c->conn.data(), c->color_ptr, c->color_elems);
@@ -592,36 +583,29 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
                 }
                 t_colour = clk::now();
             });
-            std::exception_ptr main_err;
             try {
                 upload_mesh(c, nmat);
+                t_upload = clk::now();
+                device_node_graph(c);
+                device_structure(c);
+                TG_HIP(hipStreamSynchronize(c->stream));
+                t_graph = clk::now();
             } catch (...) {
                 main_err = std::current_exception();
             }
-            t_upload = clk::now();
-            graph.join();
             colour.join();
-            for (auto& e : {main_err, graph_err, colour_err})
+            for (auto& e : {main_err, colour_err})
                 if (e) std::rethrow_exception(e);
             if (phase.on) {
                 auto ms = [&](clk::time_point t) { return std::chrono::duration<double, std::milli>(t - t0).count(); };
-                std::fprintf(stderr, "[tg_initialize]   graph + MSR rows %.1f ms, colouring %.1f ms, upload %.1f ms\n",
-                             ms(t_graph), ms(t_colour), ms(t_upload));
+                std::fprintf(stderr, "[tg_initialize]   upload %.1f ms, + device structure %.1f ms; colouring %.1f ms\n",
+                             ms(t_upload), ms(t_graph), ms(t_colour));
             }
         }
-        c->msr_len = (long)c->bindx[(size_t)c->n_eq];
-        c->dbindx.alloc((size_t)c->msr_len);
-        c->dval.alloc((size_t)c->msr_len);
-        c->dval32.alloc(mixed_stiffness(c) ? (size_t)c->msr_len : 0);
-        // row pointers; the columns are written on the device with the block operator
-        TG_HIP(hipMemcpyAsync(c->dbindx.p, c->bindx.data(), sizeof(int) * ((size_t)c->n_eq + 1),
-                              hipMemcpyHostToDevice, c->stream));
         TG_HIP(hipMemcpyAsync(c->dcolor_elems.p, c->color_elems.data(), sizeof(int) * c->nel, hipMemcpyHostToDevice,
                               c->stream));
         const bool mixed = mixed_operator(c);
-        phase("graph + colours | upload");
-        build_bsell(c);
-        phase("block operator + MSR");
+        phase("structure | colours");
         launch_build_slots(c);
         phase("slot map");
         c->op.d_diag.alloc((size_t)c->nnd);
@@ -641,8 +625,6 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
         } else {
             c->amg.clear();
         }
-        c->nptr.clear();  // the capacity stays for the next set-up
-        c->nlist.clear();
         setup_segments(c);
         TG_HIP(hipStreamSynchronize(c->stream));
         phase("segments");

@@ -158,7 +158,6 @@ struct Bsell {
     int nslices = 0;
     long nblocks_padded = 0;               // sum over slices of width * 64
     std::vector<int> slice_base, slice_width;
-    ivec h_col;  // host staging of d_col, kept across set-ups
     DBuf<int> d_slice_base, d_slice_width;
     DBuf<int> d_col;                       // [nblocks_padded] column node, -1 = padding
     DBuf<int> d_src;                       // [nblocks_padded * 9] MSR val index, -1 = zero (and lower blocks: copied from partners)
@@ -297,13 +296,14 @@ struct tg_ctx {
     bool initialized = false;
     int n_eq = 0;
     std::vector<int> eqnos;   // [nnd*3] 1-based, -1 prescribed
-    tg::ivec bindx;           // upper MSR row pointers bindx[0..n_eq] (the columns: dbindx)
     long msr_len = 0;
     std::vector<int> color_ptr, color_elems;
     int ncolors = 0;
-    std::vector<long> nptr;             // node adjacency (set-up only; emptied after tg_initialize,
-    tg::ivec nlist;                     // capacity kept)
-    tg::GraphWork graph_work;           // node_graph's buffers, kept across set-ups
+    tg::GraphWork graph_work;           // host node_graph's buffers (multigrid set-up), kept across set-ups
+    // device structure build (k_graph.hip), kept across set-ups
+    tg::DBuf<int> g_neptr, g_nelist, g_fill, g_flag, g_nlist;
+    tg::DBuf<long long> g_deg, g_nptr, g_rowlen, g_rows, g_most;
+    tg::DBuf<unsigned char> g_scan;
     // stiffness launch order: every segment's elements in ranges of
     // stiff_chunk (by rank inside the segment), all colours of range j
     // before range j+1 (0: whole colours)
@@ -422,9 +422,11 @@ struct SetupTrace {
 // host setup (tg_setup.cpp)
 int equation_numbers(int nnd, const std::vector<int>& kn, const std::vector<int>& kd, std::vector<int>& eqnos);
 void node_graph(int nnd, int nel, const int* conn, std::vector<long>& nptr, ivec& nlist, GraphWork* w = nullptr);
-void msr_rows(int n_eq, int nnd, const int* eqnos, const std::vector<long>& nptr, const ivec& nlist, ivec& bindx);
 int color_elements(int nnd, int nel, const int* conn, std::vector<int>& ptr, std::vector<int>& elems);
-void build_bsell(tg_ctx* c);
+// device structure (k_graph.hip): the node graph, then the operator layout,
+// block columns, MSR row pointers and columns, value map and partners
+void device_node_graph(tg_ctx* c);
+void device_structure(tg_ctx* c);
 
 // kernels launched from the ABI (k_element.hip, k_solver.hip)
 void launch_residual(tg_ctx* c, const double* u, double* R_node);
