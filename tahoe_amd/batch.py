@@ -11,6 +11,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import api
@@ -59,34 +61,57 @@ def pack(problems: list) -> Packed:
     begins[1:] = np.cumsum(nnds + pads)
     nels = np.array([p.mesh.conn.shape[0] for p in problems], dtype=np.int64)
     zeros3 = np.zeros((64, 3))
-    X = np.concatenate([a for p, q in zip(problems, pads) for a in (p.mesh.X, zeros3[:q])])
-    u0 = None
+
+    def nodal(field):
+        return np.concatenate([a for p, q in zip(problems, pads) for a in (field(p), zeros3[:q])])
+
+    def build_conn():
+        conn = np.concatenate([p.mesh.conn for p in problems]).astype(np.int32, copy=False)
+        conn += np.repeat(begins[:-1].astype(np.int32), nels)[:, None]
+        return conn
+
+    def build_mat():
+        local = np.concatenate([p.mesh.mat for p in problems]).astype(np.int32, copy=False)
+        local += np.repeat(np.asarray(gbeg[:-1], dtype=np.int32), nels)
+        return np.asarray(gids, dtype=np.int32)[local]
+
+    def build_kbc():
+        # each problem's cards, then its padding nodes' (3 dofs each): node
+        # numbers local to the problem's padding start, then one offset pass
+        node3 = np.repeat(np.arange(64, dtype=np.int32), 3)
+        dof3 = np.tile(np.arange(3, dtype=np.int32), 64)
+        zero_v = np.zeros(192)
+        nks = np.array([len(p.kbc.nodes) for p in problems], dtype=np.int64)
+        kn = np.concatenate([a for p, q in zip(problems, pads)
+                             for a in (p.kbc.nodes, node3[:3 * q])]).astype(np.int32, copy=False)
+        kd = np.concatenate([a for p, q in zip(problems, pads)
+                             for a in (p.kbc.dofs, dof3[:3 * q])]).astype(np.int32, copy=False)
+        kv = np.concatenate([a for p, q in zip(problems, pads)
+                             for a in (p.kbc.values, zero_v[:3 * q])]).astype(np.float64, copy=False)
+        shift = np.empty(2 * len(problems), dtype=np.int32)
+        shift[0::2] = begins[:-1]               # a problem's own cards
+        shift[1::2] = begins[:-1] + nnds        # its padding nodes
+        lens = np.empty(2 * len(problems), dtype=np.int64)
+        lens[0::2] = nks
+        lens[1::2] = 3 * pads
+        kn += np.repeat(shift, lens)
+        return KBC(kn, kd, kv)
+
+    jobs = [lambda: nodal(lambda p: p.mesh.X), build_conn, build_mat, build_kbc]
     if any(p.u0 is not None for p in problems):
-        u0 = np.concatenate([a for p, q in zip(problems, pads)
-                             for a in (p.u0 if p.u0 is not None else np.zeros_like(p.mesh.X), zeros3[:q])])
-    conn = np.concatenate([p.mesh.conn for p in problems]).astype(np.int32, copy=False)
-    conn += np.repeat(begins[:-1].astype(np.int32), nels)[:, None]
-    local = np.concatenate([p.mesh.mat for p in problems]).astype(np.int32, copy=False)
-    local += np.repeat(np.asarray(gbeg[:-1], dtype=np.int32), nels)
-    mat = np.asarray(gids, dtype=np.int32)[local]
-    # each problem's cards, then its padding nodes' (3 dofs each): node
-    # numbers local to the problem's padding start, then one offset pass
-    node3 = np.repeat(np.arange(64, dtype=np.int32), 3)
-    dof3 = np.tile(np.arange(3, dtype=np.int32), 64)
-    zero_v = np.zeros(192)
-    nks = np.array([len(p.kbc.nodes) for p in problems], dtype=np.int64)
-    kn = np.concatenate([a for p, q in zip(problems, pads) for a in (p.kbc.nodes, node3[:3 * q])]).astype(np.int32, copy=False)
-    kd = np.concatenate([a for p, q in zip(problems, pads) for a in (p.kbc.dofs, dof3[:3 * q])]).astype(np.int32, copy=False)
-    kv = np.concatenate([a for p, q in zip(problems, pads) for a in (p.kbc.values, zero_v[:3 * q])]).astype(np.float64, copy=False)
-    shift = np.empty(2 * len(problems), dtype=np.int32)
-    shift[0::2] = begins[:-1]               # a problem's own cards
-    shift[1::2] = begins[:-1] + nnds        # its padding nodes
-    lens = np.empty(2 * len(problems), dtype=np.int64)
-    lens[0::2] = nks
-    lens[1::2] = 3 * pads
-    kn += np.repeat(shift, lens)
+        jobs.append(lambda: nodal(lambda p: p.u0 if p.u0 is not None else np.zeros_like(p.mesh.X)))
+    if int(begins[-1]) >= max(1 << 16, 2048 * len(problems)):
+        # large problems: the copies release the interpreter lock, so the
+        # arrays are built side by side (many small problems would contend
+        # for the lock in the list building instead)
+        with ThreadPoolExecutor(len(jobs)) as ex:
+            out = [f.result() for f in [ex.submit(j) for j in jobs]]
+    else:
+        out = [j() for j in jobs]
+    X, conn, mat, kbc = out[:4]
+    u0 = out[4] if len(out) > 4 else None
     # the last sub-problem needs no trailing padding, but keeping it is harmless
-    return Packed(X, conn, mat, KBC(kn, kd, kv), begins.astype(np.int32), nnds.astype(np.int32), mats, u0)
+    return Packed(X, conn, mat, kbc, begins.astype(np.int32), nnds.astype(np.int32), mats, u0)
 
 
 def make_solid(pk: Packed, formulation=api.TOTAL_LAGRANGIAN, precision=api.FP64, device=0,

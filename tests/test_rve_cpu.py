@@ -77,15 +77,17 @@ def test_pack_segments_on_64_node_boundaries():
         assert np.array_equal(part, u[b:b + n])
 
 
-def test_pack_contents_problem_by_problem():
+@pytest.mark.parametrize("edges", [(4, 3, 5, 3), (24, 3, 23, 22, 24, 24)])
+def test_pack_contents_problem_by_problem(edges):
     """Each sub-problem's arrays land at its node offset unchanged: X rows,
     connectivity shifted by the offset, materials through the global table,
     its own cards (shifted) and then one card per dof of each padding node;
     predictors where given, zeros elsewhere.  Edge 3 gives 64 nodes: no
-    padding."""
+    padding.  The larger set (over 2048 nodes per problem) builds its
+    arrays on threads."""
     import dataclasses
-    reqs = [rve.Request(i, L, api.TOTAL_LAGRANGIAN, nu, seed=20 + i)
-            for i, (L, nu) in enumerate(((4, 0.3), (3, 0.3), (5, 0.25), (3, 0.3)))]
+    reqs = [rve.Request(i, L, api.TOTAL_LAGRANGIAN, (0.3, 0.3, 0.25, 0.3, 0.25, 0.3)[i], seed=20 + i)
+            for i, L in enumerate(edges)]
     probs = [r.problem() for r in reqs]
     probs[1] = dataclasses.replace(probs[1], u0=None)
     pk = batch.pack(probs)
@@ -111,6 +113,7 @@ def test_pack_contents_problem_by_problem():
         e0, k0 = e0 + ne, k0 + nk + 3 * pad
     assert e0 == pk.conn.shape[0] and k0 == len(pk.kbc.nodes)
     assert pk.node_begin[2] - pk.node_begin[1] == 64   # the edge-3 cube fills its slice
+    assert (pk.X.shape[0] >= max(1 << 16, 2048 * len(probs))) == (edges[0] == 24)
     assert batch.pack([probs[1]]).u0 is None
 
 
