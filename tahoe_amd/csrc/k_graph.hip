@@ -150,7 +150,8 @@ __global__ void k_block_columns(int nnd, int nslices, const int* __restrict__ sb
 
 // MSR row lengths (equation numbers grow with (node, dof)): row (a, i)
 // holds the active dofs of a after i and every active dof of each
-// neighbour b > a.  rl[0] = n_eq + 1 and rl[r + 1] = the length of row r,
+// neighbour b > a; a node that no element uses has no neighbours, itself
+// included, and empty rows.  rl[0] = n_eq + 1 and rl[r + 1] = the length of row r,
 // so that the inclusive sum of rl is bindx[0 .. n_eq].
 __global__ void k_msr_rowlen(int nnd, int n_eq, const long long* __restrict__ nptr, const int* __restrict__ nlist,
                              const int* __restrict__ eqn, long long* __restrict__ rl)
@@ -158,6 +159,7 @@ __global__ void k_msr_rowlen(int nnd, int n_eq, const long long* __restrict__ np
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a == 0) rl[0] = n_eq + 1;
     if (a >= nnd) return;
+    if (nptr[a + 1] == nptr[a]) return;  // a node of no element couples nothing: empty rows
     int later = 0;
     for (long long k = nptr[a]; k < nptr[a + 1]; k++) {
         const int b = nlist[k];
