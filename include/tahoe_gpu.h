@@ -158,8 +158,11 @@ int tg_set_segments(tg_ctx* ctx, int nseg, const int* node_begin);
  * Replaces FEManagerT::SetEquationSystem (FEManagerT.cpp:2248-2313) ->
  * NodeManagerT::SetEquationNumbers (NodeManagerT.cpp:712-760) and
  * GlobalMatrixT::Initialize -> MSRMatrixT::Initialize (MSRMatrixT.cpp:35-44).
- * With the environment variable TG_SETUP_TRACE set, the time of each setup
- * phase is printed to stderr. */
+ * The equation numbers and the colouring are formed on the host; the node
+ * graph, the MSR pattern and the solver operator's structure on the device
+ * (so tg_get_msr_bindx copies from the device).  A node shared by more than
+ * 64 elements is refused (TG_OUT_OF_RANGE).  With the environment variable
+ * TG_SETUP_TRACE set, the time of each setup phase is printed to stderr. */
 int tg_initialize(tg_ctx* ctx, int* n_eq, long* msr_length);
 /* FieldT::Equations(): [nnd][3], 1-based, -1 prescribed (print_eqnos). */
 int tg_get_eqnos(const tg_ctx* ctx, int* eqnos);
