@@ -95,11 +95,13 @@ def parse_args():
                     help="newton-step: the solver paths to run (comma list; the factors need all four)")
     ap.add_argument("--step-stretch", type=float, default=0.005,
                     help="newton-step: top-face displacement of the one load step, fraction of the height")
-    ap.add_argument("--workload", choices=["cube", "mixed-steps", "rve-batch", "rve-serve", "newton-step"],
+    ap.add_argument("--workload", choices=["cube", "mixed-steps", "rve-batch", "rve-serve", "newton-step",
+                                           "rve-setup"],
                     default="cube",
                     help="cube: one Newton correction on the n^3 cube (default); mixed-steps: the 20%%/10-step "
                          "load path in mixed precision vs FP64, error per step; rve-batch: 4096 RVEs batched; "
-                         "rve-serve: Poisson stream of RVE requests")
+                         "rve-serve: Poisson stream of RVE requests; rve-setup: the set-up of rve-batch's "
+                         "batches alone")
     ap.add_argument("--rve-count", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=256,
                     help="RVEs of the batch solved one at a time in plain FP64 (rate baseline + error check)")
@@ -272,6 +274,8 @@ def main():
         return bench_mixed_steps(a)
     if a.workload == "rve-batch":
         return bench_rve_batch(a)
+    if a.workload == "rve-setup":
+        return bench_rve_setup(a)
     if a.workload == "rve-serve":
         return bench_rve_serve(a)
     if a.workload == "newton-step":
@@ -824,6 +828,42 @@ def bench_rve_batch(a):
         "max_rel_err_vs_fp64": main_leg["max_rel_err_vs_fp64"], "compared_rves": main_leg["compared_rves"],
         "newton_max": main_leg["newton_max"],
     })
+    print(json.dumps(line), flush=True)
+
+
+def bench_rve_setup(a):
+    """The set-up of rve-batch's batches (the 4096 RVE cubes cut into batches
+    of at most --batch-elements elements) on one reused context, mixed
+    precision, no solve: pack, then tg_set_mesh ... tg_initialize; twice
+    over the batches, the second pass reported (the context's buffers then
+    have their size)."""
+    from tahoe_amd import api, batch, rve
+    reqs = rve.batch_requests(a.rve_count)
+    probs = {r.rid: r.problem() for r in reqs}
+    batches = size_batches(reqs, a.batch_elements)
+    s = api.Solid(0)
+    rows = []
+    for rep in range(2):
+        rows = []
+        for b in batches:
+            t0 = time.perf_counter()
+            pk = batch.pack([probs[r.rid] for r in b])
+            t1 = time.perf_counter()
+            batch.configure(s, pk, formulation=api.TOTAL_LAGRANGIAN, precision=api.MIXED)
+            s.synchronize()
+            t2 = time.perf_counter()
+            rows.append({"rves": len(b), "elements": int(sum(r.elements for r in b)),
+                         "pack_ms": (t1 - t0) * 1e3, "setup_ms": (t2 - t1) * 1e3})
+    s.close()
+    tot_el = sum(r["elements"] for r in rows)
+    setup_s = sum(r["setup_ms"] for r in rows) / 1e3
+    line = base_line(tot_el / setup_s, "elements/s", len(batches), len(batches), setup_s * 1e3 / len(batches),
+                     "f32/f64", {"workload": f"set-up of {len(reqs)} RVE cubes (edge log-uniform 4-24, seed 4242) "
+                                             f"in {len(batches)} batches of at most {a.batch_elements} elements, "
+                                             "one reused context, mixed precision",
+                                 "rves": len(reqs), "elements": tot_el, "batches": len(batches)})
+    line.update({"metric": "elements/s set up (tg_set_mesh ... tg_initialize)", "per_batch": rows,
+                 "pack_elements_per_s": tot_el / (sum(r["pack_ms"] for r in rows) / 1e3)})
     print(json.dumps(line), flush=True)
 
 
