@@ -1057,75 +1057,71 @@ void launch_stiffness_operator(tg_ctx* c, const double* u)
 // slot map: element upper entry (m <= n) -> MSR value index
 // ---------------------------------------------------------------------------
 
-// MSRMatrixT::Assemble symmetric branch (MSRMatrixT.cpp:110-165) sends the
-// element value at local (m, n) to the global upper entry
-// (min(eq_m, eq_n), max(eq_m, eq_n)); the diagonal lives in val[row], an
-// off-diagonal at its position in the sorted row (AssembleRow, :532-604).
-__global__ void k_build_slots(int nel, const int* __restrict__ conn, const int* __restrict__ eqn,
-                              const int* __restrict__ bindx, int* __restrict__ slot)
+// Position (value index of component 0) of operator block (row R, column C):
+// slice R / 64, lane R % 64, the slot whose column is C.  A row lists its
+// columns ascending, then padding (-1, taken as larger than any node); -1
+// when C is not a neighbour of R.
+__device__ __forceinline__ int block_pos(int R, int C, const int* __restrict__ sbase, const int* __restrict__ swidth,
+                                         const int* __restrict__ bcol)
 {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= 300L * nel) return;
-    const int e = (int)(idx / 300), tt = (int)(idx % 300);
-    int m = 0, rem = tt;
-    while (rem >= 24 - m) {
-        rem -= 24 - m;
-        m++;
+    const int s = R >> 6, l = R & 63;
+    int lo = 0, hi = swidth[s] - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int x = bcol[((long)sbase[s] + mid) * 64 + l];
+        if (x == C) return (int)((((long)sbase[s] + mid) * 9) * 64 + l);
+        if (x >= 0 && x < C)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
     }
-    const int n = m + rem;
-    const int rm = eqn[3L * conn[8L * e + m / 3] + m % 3];
-    const int rn = eqn[3L * conn[8L * e + n / 3] + n % 3];
-    int s = -1;
-    if (rm >= 0 && rn >= 0) {
-        const int row = rm < rn ? rm : rn, col = rm < rn ? rn : rm;
-        if (row == col)
-            s = row;
-        else {
-            int lo = bindx[row], hi = bindx[row + 1] - 1;
-            while (lo <= hi) {
-                const int mid = (lo + hi) >> 1;
-                const int v = bindx[mid];
-                if (v == col) {
-                    s = mid;
-                    break;
-                }
-                if (v < col)
-                    lo = mid + 1;
-                else
-                    hi = mid - 1;
-            }
-        }
-    }
-    slot[idx] = s;
+    return -1;
 }
 
-// Element block (a, b), a <= b, local upper order p -> value index of
-// component 0 of operator block (row A, column B) and (row B, column A):
-// slice A/64, lane A%64, the slot k whose column is B.
-__global__ void k_build_operator_slots(int nel, const int* __restrict__ conn, const int* __restrict__ sbase,
-                                       const int* __restrict__ swidth, const int* __restrict__ bcol,
-                                       int* __restrict__ oslot)
+// Slot maps of element e, one thread per node pair p = (a, b), a <= b in
+// local upper order, nodes A = conn[a], B = conn[b]:
+//  - slot (MSR path): MSRMatrixT::Assemble's symmetric branch
+//    (MSRMatrixT.cpp:110-165) sends the element value at local (m, n),
+//    m <= n, to the upper entry (min(eq_m, eq_n), max(eq_m, eq_n)) -- the
+//    diagonal val[row], an off-diagonal its place in the sorted row
+//    (AssembleRow, :532-604).  That is the expansion's value map (d_src) of
+//    entry (i, j) of the upper one of the blocks (A, B) and (B, A): node-
+//    major equation numbers order the two nodes' equations as the nodes.
+//  - oslot (operator path, when given): the positions of blocks (A, B) and
+//    (B, A); the second is -1 for a diagonal pair.
+// Two searches in the sorted block rows per pair replace a search in the
+// MSR row per entry.
+__global__ void k_element_slots(int nel, const int* __restrict__ conn, const int* __restrict__ sbase,
+                                const int* __restrict__ swidth, const int* __restrict__ bcol,
+                                const int* __restrict__ src, int* __restrict__ slot, int* __restrict__ oslot)
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= 72L * nel) return;
-    const int e = (int)(idx / 72), t = (int)(idx % 72), p = t >> 1, dir = t & 1;
-    int a = 0, rem = p;
-    while (rem >= 8 - a) {
-        rem -= 8 - a;
-        a++;
+    if (idx >= 36L * nel) return;
+    const int e = (int)(idx / 36), p = (int)(idx % 36);
+    int a, b;
+    upper_block(p, a, b);
+    const int A = conn[8L * e + a], B = conn[8L * e + b];
+    const int pab = block_pos(A, B, sbase, swidth, bcol);
+    const int pba = a == b ? -1 : block_pos(B, A, sbase, swidth, bcol);
+    if (oslot) {
+        oslot[72L * e + 2 * p] = pab;
+        oslot[72L * e + 2 * p + 1] = pba;
     }
-    const int b = a + rem;
-    int out = -1;
-    if (!(dir == 1 && a == b)) {
-        const int row = conn[8L * e + (dir ? b : a)], col = conn[8L * e + (dir ? a : b)];
-        const int s = row >> 6, l = row & 63;
-        for (int k = 0; k < swidth[s]; k++)
-            if (bcol[((long)sbase[s] + k) * 64 + l] == col) {
-                out = (int)((((long)sbase[s] + k) * 9) * 64 + l);
-                break;
+    int* es = slot + 300L * e;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int m = 3 * a + i, n = 3 * b + j;
+            if (n < m) continue;  // lower half of a diagonal block
+            int v = -1;
+            if (A <= B) {
+                if (pab >= 0) v = src[pab + (3 * i + j) * 64];
+            } else if (pba >= 0) {
+                v = src[pba + (3 * j + i) * 64];
             }
-    }
-    oslot[idx] = out;
+            es[upper_index(m, n)] = v;
+        }
 }
 
 // value index of component 0 of each row's diagonal block
@@ -1134,22 +1130,18 @@ __global__ void k_build_operator_diag(int nnd, const int* __restrict__ sbase, co
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nnd) return;
-    const int s = a >> 6, l = a & 63;
-    int out = -1;
-    for (int k = 0; k < swidth[s]; k++)
-        if (bcol[((long)sbase[s] + k) * 64 + l] == a) {
-            out = (int)((((long)sbase[s] + k) * 9) * 64 + l);
-            break;
-        }
-    diag[a] = out;
+    diag[a] = block_pos(a, a, sbase, swidth, bcol);
 }
 
-void launch_build_operator_slots(tg_ctx* c)
+// both slot maps (the operator's only when d_oslot is allocated: the
+// total-Lagrangian element), from the block columns and the value map
+void launch_element_slots(tg_ctx* c)
 {
-    const long n = 72L * c->nel;
+    const long n = 36L * c->nel;
     const int bs = 256;
-    hipLaunchKernelGGL(k_build_operator_slots, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nel,
-                       c->dconn.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->op.d_oslot.p);
+    hipLaunchKernelGGL(k_element_slots, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nel,
+                       c->dconn.p, c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->op.d_src.p,
+                       c->dslot.p, c->op.d_oslot.n ? c->op.d_oslot.p : nullptr);
     TG_HIP(hipGetLastError());
 }
 
@@ -1293,15 +1285,6 @@ void launch_build_src(tg_ctx* c)
                        c->op.d_src.p);
     hipLaunchKernelGGL(k_build_tpos, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nnd,
                        c->op.d_slice_base.p, c->op.d_slice_width.p, c->op.d_col.p, c->op.d_tpos.p);
-    TG_HIP(hipGetLastError());
-}
-
-void launch_build_slots(tg_ctx* c)
-{
-    const long n = 300L * c->nel;
-    const int bs = 256;
-    hipLaunchKernelGGL(k_build_slots, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, c->stream, c->nel,
-                       c->dconn.p, c->deqn.p, c->dbindx.p, c->dslot.p);
     TG_HIP(hipGetLastError());
 }
 

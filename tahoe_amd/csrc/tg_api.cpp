@@ -606,19 +606,16 @@ int tg_initialize(tg_ctx* c, int* n_eq, long* msr_len)
                               c->stream));
         const bool mixed = mixed_operator(c);
         phase("structure | colours");
-        launch_build_slots(c);
-        phase("slot map");
         c->op.d_diag.alloc((size_t)c->nnd);
         launch_build_operator_diag(c);
-        if (c->formulation == TG_TOTAL_LAGRANGIAN) {
+        if (c->formulation == TG_TOTAL_LAGRANGIAN)
             c->op.d_oslot.alloc(72 * (size_t)c->nel);
-            launch_build_operator_slots(c);
-        } else {
+        else
             c->op.d_oslot.release();
-        }
+        launch_element_slots(c);
         c->vbinv.alloc(c->precond == TG_PC_BLOCK_JACOBI ? 9 * (size_t)c->nnd : 0);
         c->op.d_val32.alloc(mixed || c->precond == TG_PC_AMG ? (size_t)c->op.nblocks_padded * 9 : 0);
-        phase("operator slot map");
+        phase("slot maps");
         if (c->precond == TG_PC_AMG) {
             amg_setup(c);
             phase("multigrid structure");

@@ -436,13 +436,12 @@ void launch_stiffness(tg_ctx* c, const double* u);
 void build_gather_lists(tg_ctx* c, const int* dorder);
 void launch_gather(tg_ctx* c);
 void launch_stiffness_operator(tg_ctx* c, const double* u);
-void launch_build_operator_slots(tg_ctx* c);
+void launch_element_slots(tg_ctx* c);
 void launch_msr_cols(tg_ctx* c);
 void launch_build_src(tg_ctx* c);
 void launch_operator_dinv(tg_ctx* c);
 void launch_block_inverse(tg_ctx* c);
 void launch_build_operator_diag(tg_ctx* c);
-void launch_build_slots(tg_ctx* c);
 void launch_expand(tg_ctx* c);
 void launch_spmv(tg_ctx* c, const double* x, double* y);
 void launch_expand32(tg_ctx* c);
