@@ -493,8 +493,8 @@ int tg_set_kbc(tg_ctx* c, int nkbc, const int* nodes, const int* dofs, const dou
 }
 
 // Device storage of the mesh and the vectors, and the uploads that need
-// neither the node graph nor the colouring (tg_initialize runs those on
-// other host threads meanwhile).
+// neither the node graph nor the colouring (tg_initialize colours the
+// elements on another host thread meanwhile).
 static void upload_mesh(tg_ctx* c, int nmat)
 {
     // device storage
